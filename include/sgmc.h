@@ -1,0 +1,208 @@
+/*
+ * sgmc.h — C-ABI of the MI355X (gfx950) SG-MCMC hot path.
+ *
+ * The reference (jax-sgmc 0.1.5) has no FFI: its hot path is Python over JAX. Each
+ * entry point below replaces one piece of that path and cites the reference
+ * function it stands in for. Conventions:
+ *   - every pointer is device memory owned by the caller (PyTorch tensors in the
+ *     package's host layer), unless a comment says "host";
+ *   - `stream` is a hipStream_t passed as void* (torch's current stream);
+ *   - return value 0 = SGMC_OK, otherwise an SGMC_E_* code; sgmc_last_error()
+ *     gives the message (thread-local). Nothing aborts, nothing allocates;
+ *   - keys are uint32[2] jax PRNG keys (legacy Threefry-2x32 derivation); an
+ *     entry point that advances a key reads key_in and writes key_out, which must
+ *     not alias (the reference is functional: state in, state out);
+ *   - workspaces hold per-block partials and an arrival counter: the caller
+ *     zero-fills a workspace once; every call leaves its counter at zero again, so a
+ *     workspace can be reused across calls and in captured HIP graphs. Calls that
+ *     share one workspace must be ordered on one stream.
+ */
+#ifndef SGMC_H_
+#define SGMC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGMC_ABI_VERSION 1
+
+#define SGMC_OK 0
+#define SGMC_E_ARG 1 /* invalid argument: null pointer, shape, alignment, size */
+#define SGMC_E_HIP 2 /* HIP runtime error at launch */
+
+/* Counter pairs (two values) covered by one block of a leaf-table kernel. */
+#define SGMC_PAIRS_PER_BLOCK 1024
+
+#define SGMC_LIK_LINEAR_GAUSSIAN 0
+#define SGMC_LIK_LOGISTIC 1
+
+#define SGMC_MODE_SGLD 0
+#define SGMC_MODE_SGLD_RMS 1
+#define SGMC_MODE_SGHMC 2
+
+const char* sgmc_last_error(void);
+int sgmc_abi_version(void);
+
+/* ---------------------------------------------------------------- RNG ----
+ * jax.random with the original (non-partitionable) Threefry-2x32 derivation, the
+ * generator behind every random number of the reference's hot path:
+ * integrator.py:119-135 (random_tree), integrator.py:208, 630, 736, 871 (splits),
+ * data/numpy_loader.py:132-134 (randint), solver.py:283-284, 528 (uniform). */
+
+/* jax.random.split(key, num): out is uint32[num][2]. */
+int sgmc_split(const uint32_t* key, uint32_t* out, int num, void* stream);
+/* jax.random.bits(key, (n,)) for 32-bit words (threefry_2x32(key, iota(n))). */
+int sgmc_random_bits(const uint32_t* key, uint32_t* out, int64_t n, void* stream);
+/* jax.random.uniform(key, (n,), float32, minval, maxval). */
+int sgmc_uniform(const uint32_t* key, float* out, int64_t n, float minval, float maxval,
+                 void* stream);
+/* jax.random.normal(key, (n,), float32). */
+int sgmc_normal(const uint32_t* key, float* out, int64_t n, void* stream);
+/* jax.random.randint(key, (n,), minval, maxval) as int32. */
+int sgmc_randint(const uint32_t* key, int32_t* out, int64_t n, int32_t minval, int32_t maxval,
+                 void* stream);
+/* random_tree(key, tree) (integrator.py:119-135) over a flattened pytree.
+ * leaf_table (device int64): [L, B, (offset, size) x L, (leaf, first_pair) x B], with
+ * leaves in tree_flatten order; block b covers SGMC_PAIRS_PER_BLOCK counter pairs of
+ * one leaf; nblocks == B. */
+int sgmc_tree_normal(const uint32_t* key, const int64_t* leaf_table, int nblocks, float* out,
+                     void* stream);
+
+/* --------------------------------------------------------------- data ----
+ * DeviceNumpyDataLoader.get_random_data (data/numpy_loader.py:128-141):
+ * (dkey_out, sub) = split(dkey_in); idx = randint(sub, (n,), 0, N). */
+int sgmc_draw_batch(const uint32_t* dkey_in, uint32_t* dkey_out, int32_t* idx, int n,
+                    int64_t N, void* stream);
+/* tree_index (data/core.py:642-660) for one leaf: dst[r] = src[idx[r]] for rows of
+ * row_bytes bytes, source rows src_stride_bytes apart; rows whose index lies
+ * outside [0, src_rows) are left unwritten. */
+int sgmc_gather_rows(const void* src, int64_t src_rows, const int32_t* idx, void* dst, int n,
+                     int64_t row_bytes, int64_t src_stride_bytes, void* stream);
+
+/* ------------------------------------------------------ GLM potential ----
+ * minibatch_potential(prior, likelihood) (potential.py:94-216) evaluated with
+ * jax.value_and_grad (integrator.py:166, 593, 792) for the linear-Gaussian and
+ * logistic families, rows X[idx[j]] (idx == NULL: rows 0..n-1 of X, i.e. an already
+ * gathered batch). X is float32 [N, ldx] with ldx % 4 == 0, ldx >= D rounded up to 4,
+ * 16-byte aligned; D <= 1024. Outputs: U (scalar), grad [D], optional lik [n] (the
+ * `likelihoods=True` output, potential.py:208-211) and var(lik) (integrator.py:880).
+ * workspace: sgmc_glm_workspace_floats(n, D, nblocks) floats; nblocks from
+ * sgmc_glm_blocks(n, D) or any value >= 1. */
+int sgmc_glm_blocks(int n, int D);
+int64_t sgmc_glm_workspace_floats(int n, int D, int nblocks);
+int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
+                       const int32_t* idx, int n, int64_t N, int D, const float* theta,
+                       float sigma, int has_prior, float prior_scale, float temperature,
+                       float* out_U, float* out_grad, float* out_lik, float* out_var,
+                       float* workspace, int64_t workspace_floats, int nblocks, void* stream);
+
+/* full_potential (potential.py:219-293) over all N rows with FP64 accumulation of the
+ * per-row likelihoods and the prior; out_U is one double. workspace:
+ * sgmc_glm_full_workspace_doubles(N, rows_per_block) doubles. */
+int64_t sgmc_glm_full_workspace_doubles(int64_t N, int rows_per_block);
+int sgmc_glm_full_potential(int family, const float* X, int64_t ldx, const float* y, int64_t N,
+                            int D, const float* theta, float sigma, int has_prior,
+                            float prior_scale, double temperature, double* out_U,
+                            double* workspace, int64_t workspace_doubles, int rows_per_block,
+                            void* stream);
+
+/* ------------------------------------------------------ step updates ----
+ * Elementwise integrator updates over a flattened pytree; noise drawn in registers
+ * with random_tree semantics (leaf_table as for sgmc_tree_normal). */
+
+/* langevin_diffusion.update_fn (integrator.py:871-912) after the gradient:
+ * (key_out, sub) = split(key_in); xi = random_tree(sub); theta += -eps g +
+ * sqrt(2 T eps) xi. With v != NULL, rms_prop (adaption.py:254-291) first updates
+ * v = alpha v + (1 - alpha) g^2 and scales the drift by G = 1/(lmbd + sqrt(v)) and the
+ * noise by sqrt(G). In place on theta (and v). */
+int sgmc_sgld_step(float* theta, const float* grad, float* v, int64_t P,
+                   const int64_t* leaf_table, int nblocks, const uint32_t* key_in,
+                   uint32_t* key_out, float eps, float temperature, float alpha, float lmbd,
+                   void* stream);
+/* friction_leapfrog.integrate momentum resample (integrator.py:736-738):
+ * (key_out, sub) = split(key_in); p = sqrt_mass * random_tree(sub). sqrt_mass may be
+ * NULL (unit mass). */
+int sgmc_sghmc_resample(float* p, const float* sqrt_mass, const int64_t* leaf_table,
+                        int nblocks, const uint32_t* key_in, uint32_t* key_out, void* stream);
+/* friction_leapfrog._body_fun position part (integrator.py:609-618):
+ * s = inv_mass p; theta += eps s; p += (-eps C) s. friction_arr (per element) may be
+ * NULL, then the scalar friction applies. */
+int sgmc_sghmc_position(float* theta, float* p, const float* inv_mass, const float* friction_arr,
+                        float friction, int64_t P, float eps, void* stream);
+/* friction_leapfrog._body_fun momentum part without noise model (integrator.py:626-655):
+ * p += -eps g; (key_out, sub) = split(key_in); p += C * (sqrt(2 eps) xi). */
+int sgmc_sghmc_momentum(float* p, const float* grad, const float* friction_arr, float friction,
+                        const int64_t* leaf_table, int nblocks, const uint32_t* key_in,
+                        uint32_t* key_out, float eps, void* stream);
+/* obabo O-step (_momentum_resampling, integrator.py:192-200) with noise key
+ * noise_key (one of split(key, 3)), and the kinetic energy 0.5 p^T M^-1 p
+ * (integrator.py:177-179) of the input momentum (ke_of_input != 0, used for
+ * kinetic_energy_end, :261) or of the output momentum (kinetic_energy_start, :222),
+ * accumulated as *ke_out = *ke_in + KE. workspace: sgmc_obabo_workspace_floats(nblocks)
+ * floats. */
+int64_t sgmc_obabo_workspace_floats(int nblocks);
+int sgmc_obabo_o(float* p, const float* sqrt_mass, const float* inv_mass,
+                 const int64_t* leaf_table, int nblocks, const uint32_t* noise_key,
+                 float friction, float eps, float temperature, int ke_of_input,
+                 const float* ke_in, float* ke_out, float* workspace, void* stream);
+/* obabo B then A (integrator.py:230-240): p = (-eps/2) g + p; theta += eps M^-1 p. */
+int sgmc_obabo_ba(float* theta, float* p, const float* grad, const float* inv_mass, int64_t P,
+                  float eps, void* stream);
+/* obabo second B (integrator.py:248-251): p = (-eps/2) g + p. */
+int sgmc_obabo_b(float* p, const float* grad, int64_t P, float eps, void* stream);
+
+/* --------------------------------------------------- fused multi-chain ----
+ * One launch advances every chain of a chain table by one gradient evaluation:
+ * in-kernel index draw (data/numpy_loader.py:132-134), row gather, GLM forward and
+ * backward, cross-block reduction, and the integrator update with in-register noise:
+ *   SGMC_MODE_SGLD / _SGLD_RMS: one launch per iteration (integrator.py:859-922);
+ *   SGMC_MODE_SGHMC: `steps` launches per iteration, step = 0..steps-1; step 0 also
+ *   resamples the momentum (integrator.py:715-757 with _body_fun 603-666).
+ * It replaces solver.mcmc's per-chain sequential loop (solver.py:165-171, strategy
+ * 'map') for chains that share a likelihood family. The step size of iteration t is
+ * eps_table[t] with t = *iter (a device counter the launch advances after the last
+ * step of an iteration), so a captured graph of launches replays without host input.
+ * save_slot (nullable, int32[table_len]): where save_slot[t] >= 0 the chain's
+ * post-step theta is written to samples[save_slot[t]][D]. */
+typedef struct sgmc_glm_chain {
+  const float* X;      /* [N, ldx] rows of this chain's dataset */
+  const float* y;      /* [N] */
+  float* theta;        /* [D] position, updated in place */
+  float* mom;          /* [D] momentum (SGHMC) or NULL */
+  float* v;            /* [D] rms_prop second moment or NULL */
+  uint32_t* key;       /* [2] integrator key, advanced in place */
+  uint32_t* dkey;      /* [2] data-loader key, advanced in place */
+  float* U;            /* [1] potential at the evaluation point */
+  float* var;          /* [1] var(lik) of the minibatch, or NULL */
+  float* lik;          /* [n] scratch: per-row likelihoods */
+  float* partial;      /* [nblocks][4 ceil(D/4) + 1] scratch */
+  unsigned* counter;   /* [1] arrival counter, zero between launches */
+  float* samples;      /* [slots][D] or NULL */
+  int64_t N;           /* observations */
+  int64_t ldx;         /* row stride in floats */
+  int32_t D;           /* features (<= 1024) */
+  int32_t n;           /* minibatch size */
+  int32_t block_begin; /* first grid block of this chain */
+  int32_t nblocks;     /* grid blocks of this chain */
+  int32_t dclass;      /* sgmc_glm_dclass(D) */
+  float temperature;   /* schedule temperature of this chain (SGLD noise) */
+} sgmc_glm_chain;
+
+int sgmc_glm_dclass(int D);
+/* chains, block_chain (int32[nblocks], chain of each grid block), eps_table,
+ * save_slot, iter and done (a zeroed unsigned) are device memory. uniform_dclass is
+ * the common dclass of all chains, or -1 for a mixed table. */
+int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,
+                        int nblocks, int uniform_dclass, int family, float sigma, int has_prior,
+                        float prior_scale, float pot_temperature, int mode, int step, int steps,
+                        float friction, float alpha, float lmbd, const float* inv_mass,
+                        const float* sqrt_mass, const float* eps_table, int table_len,
+                        const int32_t* save_slot, int32_t* iter, unsigned* done, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGMC_H_ */

@@ -1,0 +1,152 @@
+"""ctypes binding of libsgmc.so (the HIP library built from csrc/).
+
+Every entry point of include/sgmc.h is declared here with its argument types.
+Loading fails loudly: there is no fallback implementation of any kernel.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgmc.so")
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+
+OK = 0
+E_ARG = 1
+E_HIP = 2
+PAIRS_PER_BLOCK = 1024
+LIK_LINEAR_GAUSSIAN = 0
+LIK_LOGISTIC = 1
+MODE_SGLD = 0
+MODE_SGLD_RMS = 1
+MODE_SGHMC = 2
+
+
+class GlmChain(ctypes.Structure):
+  """Mirror of ``sgmc_glm_chain`` (include/sgmc.h)."""
+  _fields_ = [
+      ("X", c_void_p), ("y", c_void_p), ("theta", c_void_p), ("mom", c_void_p),
+      ("v", c_void_p), ("key", c_void_p), ("dkey", c_void_p), ("U", c_void_p),
+      ("var", c_void_p), ("lik", c_void_p), ("partial", c_void_p), ("counter", c_void_p),
+      ("samples", c_void_p), ("N", c_int64), ("ldx", c_int64), ("D", c_int32),
+      ("n", c_int32), ("block_begin", c_int32), ("nblocks", c_int32), ("dclass", c_int32),
+      ("temperature", c_float),
+  ]
+
+
+_SIGNATURES = {
+    "sgmc_last_error": (ctypes.c_char_p, []),
+    "sgmc_abi_version": (c_int, []),
+    "sgmc_split": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "sgmc_random_bits": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "sgmc_uniform": (c_int, [c_void_p, c_void_p, c_int64, c_float, c_float, c_void_p]),
+    "sgmc_normal": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "sgmc_randint": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p]),
+    "sgmc_tree_normal": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "sgmc_draw_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64, c_void_p]),
+    "sgmc_gather_rows": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int64,
+                                 c_int64, c_void_p]),
+    "sgmc_glm_blocks": (c_int, [c_int, c_int]),
+    "sgmc_glm_workspace_floats": (c_int64, [c_int, c_int, c_int]),
+    "sgmc_glm_potential": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int64,
+                                   c_int, c_void_p, c_float, c_int, c_float, c_float, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int,
+                                   c_void_p]),
+    "sgmc_glm_full_workspace_doubles": (c_int64, [c_int64, c_int]),
+    "sgmc_glm_full_potential": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int,
+                                        c_void_p, c_float, c_int, c_float, c_double, c_void_p,
+                                        c_void_p, c_int64, c_int, c_void_p]),
+    "sgmc_sgld_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int, c_void_p,
+                               c_void_p, c_float, c_float, c_float, c_float, c_void_p]),
+    "sgmc_sghmc_resample": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                    c_void_p]),
+    "sgmc_sghmc_position": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int64,
+                                    c_float, c_void_p]),
+    "sgmc_sghmc_momentum": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_int,
+                                    c_void_p, c_void_p, c_float, c_void_p]),
+    "sgmc_obabo_workspace_floats": (c_int64, [c_int]),
+    "sgmc_obabo_o": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_float,
+                             c_float, c_float, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sgmc_obabo_ba": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float,
+                              c_void_p]),
+    "sgmc_obabo_b": (c_int, [c_void_p, c_void_p, c_int64, c_float, c_void_p]),
+    "sgmc_glm_dclass": (c_int, [c_int]),
+    "sgmc_glm_fused_step": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_float,
+                                    c_int, c_float, c_float, c_int, c_int, c_int, c_float,
+                                    c_float, c_float, c_void_p, c_void_p, c_void_p, c_int,
+                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+# Entry points added after ABI version 1 register themselves here.
+EXTRA_SIGNATURES = {}
+
+_lib = None
+
+
+class SgmcError(RuntimeError):
+  pass
+
+
+def load():
+  """Loads libsgmc.so once; raises ImportError if it has not been built."""
+  global _lib
+  if _lib is not None:
+    return _lib
+  if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: the HIP library must be built first "
+        "(python jax-sgmc_amd/build.py, or __graft_entry__.build()). There is no "
+        "CPU fallback.")
+  lib = ctypes.CDLL(LIB_PATH)
+  for name, (res, args) in list(_SIGNATURES.items()) + list(EXTRA_SIGNATURES.items()):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+  _lib = lib
+  return lib
+
+
+def exported_names():
+  return sorted(list(_SIGNATURES) + list(EXTRA_SIGNATURES))
+
+
+def call(name, *args):
+  """Calls an entry point and raises on a nonzero status."""
+  lib = load()
+  status = getattr(lib, name)(*args)
+  if status != OK:
+    msg = lib.sgmc_last_error().decode(errors="replace")
+    if status == E_ARG:
+      raise ValueError(f"{name}: {msg}")
+    raise SgmcError(f"{name} failed ({status}): {msg}")
+  return status
+
+
+def query(name, *args):
+  return getattr(load(), name)(*args)
+
+
+def ptr(t):
+  """Device pointer of a tensor (None -> NULL). Tensors must live on the GPU."""
+  if t is None:
+    return None
+  if not t.is_cuda:
+    raise ValueError("libsgmc entry points take device tensors; got a CPU tensor")
+  return c_void_p(t.data_ptr())
+
+
+def stream():
+  return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_gpu():
+  if not torch.cuda.is_available():
+    raise RuntimeError("the sgmc kernels need a ROCm GPU (torch.cuda.is_available() is False)")
+  load()

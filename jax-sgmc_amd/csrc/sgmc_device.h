@@ -1,0 +1,309 @@
+// Device-side building blocks shared by every kernel of the library (gfx950).
+//
+// Random numbers follow jax.random's original (non-partitionable) Threefry-2x32
+// derivation, which is what the reference draws through (jax 0.4.x):
+//   split(key, num)[i]   = words (2i, 2i+1) of threefry_2x32(key, iota(2 num))
+//   random_bits(key, S)  = threefry_2x32(key, iota(S)); with m = ceil(S/2), word i
+//                          comes from counter pair (j, j+m) (j = i mod m; the pair
+//                          (m-1, 0) when S is odd), output 0 if i < m else output 1
+//   uniform              = bitcast((bits >> 9) | 0x3F800000) - 1, scaled
+//   normal               = sqrt(2) * erfinv(uniform on [nextafter(-1,0), 1))
+//   randint(key, n, lo, hi): k1, k2 = split(key); ((hi%s)*mult + lo%s) % s
+// Reference call sites: integrator.py:119-135 (random_tree), integrator.py:208,
+// 630, 736, 871 (key splits), data/numpy_loader.py:132-134 (index draw).
+//
+// The float32 elementary functions (log, log1p, erfinv) are written out op by op
+// with floating-point contraction disabled, so a normal is a fixed function of its
+// 32 input bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgmc {
+
+constexpr int kWave = 64;
+
+struct Key {
+  uint32_t k0, k1;
+};
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) {
+  return (x << r) | (x >> (32 - r));
+}
+
+// Threefry-2x32, 20 rounds, on one counter pair (x0, x1).
+__device__ __forceinline__ void threefry2x32(uint32_t k0, uint32_t k1, uint32_t& x0,
+                                             uint32_t& x1) {
+  const uint32_t k2 = k0 ^ k1 ^ 0x1BD11BDAu;
+  x0 += k0;
+  x1 += k1;
+#define SGMC_TF_ROUND(r) \
+  x0 += x1;              \
+  x1 = rotl32(x1, r);    \
+  x1 ^= x0;
+  SGMC_TF_ROUND(13) SGMC_TF_ROUND(15) SGMC_TF_ROUND(26) SGMC_TF_ROUND(6)
+  x0 += k1; x1 += k2 + 1u;
+  SGMC_TF_ROUND(17) SGMC_TF_ROUND(29) SGMC_TF_ROUND(16) SGMC_TF_ROUND(24)
+  x0 += k2; x1 += k0 + 2u;
+  SGMC_TF_ROUND(13) SGMC_TF_ROUND(15) SGMC_TF_ROUND(26) SGMC_TF_ROUND(6)
+  x0 += k0; x1 += k1 + 3u;
+  SGMC_TF_ROUND(17) SGMC_TF_ROUND(29) SGMC_TF_ROUND(16) SGMC_TF_ROUND(24)
+  x0 += k1; x1 += k2 + 4u;
+  SGMC_TF_ROUND(13) SGMC_TF_ROUND(15) SGMC_TF_ROUND(26) SGMC_TF_ROUND(6)
+  x0 += k2; x1 += k0 + 5u;
+#undef SGMC_TF_ROUND
+}
+
+// Word q (0 <= q < 2 num) of threefry_2x32(key, iota(2 num)).
+__device__ __forceinline__ uint32_t split_word(Key k, uint32_t num, uint32_t q) {
+  const uint32_t pair = q < num ? q : q - num;
+  uint32_t x0 = pair, x1 = num + pair;
+  threefry2x32(k.k0, k.k1, x0, x1);
+  return q < num ? x0 : x1;
+}
+
+// split(key, num)[i].
+__device__ __forceinline__ Key split_key(Key k, uint32_t num, uint32_t i) {
+  if (num == 1) {
+    uint32_t x0 = 0, x1 = 1;
+    threefry2x32(k.k0, k.k1, x0, x1);
+    return Key{x0, x1};
+  }
+  return Key{split_word(k, num, 2 * i), split_word(k, num, 2 * i + 1)};
+}
+
+// (split(key)[0], split(key)[1]) with two Threefry blocks.
+__device__ __forceinline__ void split2(Key k, Key& a, Key& b) {
+  uint32_t p0 = 0, q0 = 2, p1 = 1, q1 = 3;
+  threefry2x32(k.k0, k.k1, p0, q0);
+  threefry2x32(k.k0, k.k1, p1, q1);
+  a = Key{p0, p1};
+  b = Key{q0, q1};
+}
+
+// split(key, 3) with three Threefry blocks.
+__device__ __forceinline__ void split3(Key k, Key& a, Key& b, Key& c) {
+  uint32_t a0 = 0, b0 = 3, a1 = 1, b1 = 4, a2 = 2, b2 = 5;
+  threefry2x32(k.k0, k.k1, a0, b0);
+  threefry2x32(k.k0, k.k1, a1, b1);
+  threefry2x32(k.k0, k.k1, a2, b2);
+  a = Key{a0, a1};
+  b = Key{a2, b0};
+  c = Key{b1, b2};
+}
+
+// Counter pair j (0 <= j < m = ceil(S/2)) of random_bits(key, S): returns the two
+// words that land at positions j and j + m.
+__device__ __forceinline__ void bits_pair(Key k, uint32_t S, uint32_t j, uint32_t& w_lo,
+                                          uint32_t& w_hi) {
+  const uint32_t m = (S + 1) >> 1;
+  uint32_t x0 = j, x1 = (j + m < S) ? j + m : 0u;
+  threefry2x32(k.k0, k.k1, x0, x1);
+  w_lo = x0;
+  w_hi = x1;
+}
+
+// Word i of random_bits(key, S).
+__device__ __forceinline__ uint32_t bits_word(Key k, uint32_t S, uint32_t i) {
+  const uint32_t m = (S + 1) >> 1;
+  const uint32_t j = i < m ? i : i - m;
+  uint32_t a, b;
+  bits_pair(k, S, j, a, b);
+  return i < m ? a : b;
+}
+
+__device__ __forceinline__ float unit_float(uint32_t bits) {
+  return __uint_as_float((bits >> 9) | 0x3F800000u) - 1.0f;
+}
+
+// ---- float32 elementary functions (contraction off) ---------------------------
+
+// Cephes logf for positive normal x.
+__device__ __forceinline__ float logf_cephes(float x) {
+#pragma clang fp contract(off)
+  const uint32_t xb = __float_as_uint(x);
+  float e = (float)((int)(xb >> 23) - 126);
+  const float m = __uint_as_float((xb & 0x807FFFFFu) | 0x3F000000u);
+  float t = m - 1.0f;
+  if (m < 0.707106781186547524f) {
+    t = t + m;
+    e = e - 1.0f;
+  }
+  const float x2 = t * t;
+  const float x3 = x2 * t;
+  float y = 7.0376836292e-2f * t + -1.1514610310e-1f;
+  float y1 = -1.2420140846e-1f * t + 1.4249322787e-1f;
+  float y2 = 2.0000714765e-1f * t + -2.4999993993e-1f;
+  y = y * t + 1.1676998740e-1f;
+  y1 = y1 * t + -1.6668057665e-1f;
+  y2 = y2 * t + 3.3333331174e-1f;
+  y = y * x3 + y1;
+  y = y * x3 + y2;
+  y = y * x3;
+  y = y + -2.12194440e-4f * e;
+  float r = t - 0.5f * x2;
+  r = r + y;
+  r = r + 0.693359375f * e;
+  return r;
+}
+
+// log1p: Cephes rational approximation for |x| < sqrt(2) - 1, else log(1 + x).
+__device__ __forceinline__ float log1p_cephes(float x) {
+#pragma clang fp contract(off)
+  if (!(fabsf(x) < 0.41421356237309504880f)) return logf_cephes(x + 1.0f);
+  const float x2 = x * x;
+  float num = 0.0f;
+  num = num * x + 4.5270000862445199635215e-5f;
+  num = num * x + 4.9854102823193375972212e-1f;
+  num = num * x + 6.5787325942061044846969e0f;
+  num = num * x + 2.9911919328553073277375e1f;
+  num = num * x + 6.0949667980987787057556e1f;
+  num = num * x + 5.7112963590585538103336e1f;
+  num = num * x + 2.0039553499201281259648e1f;
+  float den = 0.0f;
+  den = den * x + 1.0f;
+  den = den * x + 1.5062909083469192043167e1f;
+  den = den * x + 8.3047565967967209469434e1f;
+  den = den * x + 2.2176239823732856465394e2f;
+  den = den * x + 3.0909872225312059774938e2f;
+  den = den * x + 2.1642788614495947685003e2f;
+  den = den * x + 6.0118660497603843919306e1f;
+  const float ratio = num / den;
+  return x + ((-0.5f * x2) + ((x * x2) * ratio));
+}
+
+// erfinv (Giles 2010 single-precision approximation).
+__device__ __forceinline__ float erfinv_giles(float x) {
+#pragma clang fp contract(off)
+  float w = -log1p_cephes(x * (-x));
+  float p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = 3.43273939e-07f + p * w;
+    p = -3.5233877e-06f + p * w;
+    p = -4.39150654e-06f + p * w;
+    p = 0.00021858087f + p * w;
+    p = -0.00125372503f + p * w;
+    p = -0.00417768164f + p * w;
+    p = 0.246640727f + p * w;
+    p = 1.50140941f + p * w;
+  } else {
+    w = sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = 0.000100950558f + p * w;
+    p = 0.00134934322f + p * w;
+    p = -0.00367342844f + p * w;
+    p = 0.00573950773f + p * w;
+    p = -0.0076224613f + p * w;
+    p = 0.00943887047f + p * w;
+    p = 1.00167406f + p * w;
+    p = 2.83297682f + p * w;
+  }
+  if (fabsf(x) == 1.0f) return x * __builtin_inff();
+  return p * x;
+}
+
+// Standard normal from one 32-bit word (jax.random.normal, float32).
+__device__ __forceinline__ float bits_to_normal(uint32_t bits) {
+#pragma clang fp contract(off)
+  const float lo = -0.99999994f;  // nextafter(-1, 0)
+  const float f = unit_float(bits);
+  float u = f * 2.0f + lo;        // (1 - lo) rounds to 2 in float32
+  u = fmaxf(lo, u);
+  return 1.41421354f * erfinv_giles(u);
+}
+
+// jax.random.uniform(key, (), lo, hi) from one word.
+__device__ __forceinline__ float bits_to_uniform(uint32_t bits, float lo, float hi) {
+#pragma clang fp contract(off)
+  const float f = unit_float(bits);
+  return fmaxf(lo, f * (hi - lo) + lo);
+}
+
+// randint offset: ((hi % span) * mult + lo % span) % span in wrapping uint32.
+__host__ __device__ __forceinline__ uint32_t randint_mult(uint32_t span) {
+  uint32_t m = 65536u % span;
+  return (uint32_t)(m * m) % span;
+}
+
+__device__ __forceinline__ uint32_t randint_offset(uint32_t hi, uint32_t lo, uint32_t span,
+                                                   uint32_t mult) {
+  return ((hi % span) * mult + (lo % span)) % span;
+}
+
+// Index j (0 <= j < n) of randint(sub, (n,), 0, N) given k1, k2 = split(sub).
+__device__ __forceinline__ int32_t draw_index(Key k1, Key k2, uint32_t n, uint32_t j,
+                                              uint32_t span, uint32_t mult) {
+  const uint32_t hi = bits_word(k1, n, j);
+  const uint32_t lo = bits_word(k2, n, j);
+  return (int32_t)randint_offset(hi, lo, span, mult);
+}
+
+__device__ __forceinline__ Key load_key(const uint32_t* p) { return Key{p[0], p[1]}; }
+__device__ __forceinline__ void store_key(uint32_t* p, Key k) {
+  p[0] = k.k0;
+  p[1] = k.k1;
+}
+
+// ---- wavefront / block reductions ---------------------------------------------
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum over a block of NT threads (NT a multiple of 64); every thread gets the total.
+template <int NT>
+__device__ __forceinline__ float block_sum(float v, float* lds /* NT/64 floats */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = v;
+  __syncthreads();
+  float t = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NT / kWave; ++i) t += lds[i];
+  return t;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum_d(double v, double* lds) {
+  v = wave_sum_d(v);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / kWave; ++i) t += lds[i];
+  return t;
+}
+
+// Arrival counter for "last block finishes" reductions: every block calls this after
+// writing its partial results; exactly one (the last to arrive) gets true and sees
+// all partials. The last block resets the counter for the next launch.
+__device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned expected,
+                                            int* lds_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(counter, 1u);
+    const bool last = (prev == expected - 1);
+    if (last) atomicExch(counter, 0u);
+    *lds_flag = last ? 1 : 0;
+  }
+  __syncthreads();
+  const bool last = *lds_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+}  // namespace sgmc

@@ -1,0 +1,236 @@
+"""Step-size schedules, SG-MCMC integrators and solvers, restated (oracle only).
+
+The vector arithmetic runs in float64; every scalar coefficient the reference forms
+from float32 schedule values (step size, temperature) is formed in float32 exactly
+as the reference forms it, and the noise is the float32 jax.random stream
+(``oracle.jaxrandom``), so a float32 implementation with the same keys must agree
+with these trajectories to float32 rounding.
+
+``pot(theta, idx) -> (U, grad, lik)`` is any potential from ``oracle.model``.
+"""
+import numpy as np
+
+from oracle import jaxrandom as jr
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------- schedules ----
+
+def polynomial_step_size(a, b, gamma, iterations):
+  """scheduler.py:469-481: eps_i = a * (b + i)^(-gamma), float32."""
+  n = np.arange(iterations, dtype=np.int32)
+  unscaled = np.power(F32(b) + n.astype(np.float32), F32(-gamma)).astype(np.float32)
+  return (F32(a) * unscaled).astype(np.float32)
+
+
+def polynomial_step_size_first_last(first, last, gamma, iterations):
+  """scheduler.py:512-532: solve a, b so that eps_0 = first, eps_{I-1} = last."""
+  assert gamma > 0
+  assert first >= last
+  gamma = F32(gamma)
+  first = F32(first)
+  last = F32(last)
+  with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+    ginv = np.power(gamma, F32(-1.0))
+    fpow = np.power(first, -ginv)
+    lpow = np.power(last, -ginv)
+    apow = F32((lpow - fpow) / F32(iterations - 1))
+    a = np.power(apow, -gamma)
+    b = np.power(F32(first / a), -ginv)
+    n = np.arange(iterations).astype(np.float32)
+    return (a * np.power(b + n, -gamma)).astype(np.float32)
+
+
+def initial_burn_in(n, iterations):
+  """scheduler.py:578-587: 1.0 once iteration >= n."""
+  return np.where(n <= np.arange(iterations), F32(1.0), F32(0.0))
+
+
+# ------------------------------------------------------------------- data ----
+
+def draw_indices(dkey, N, n):
+  """DeviceNumpyDataLoader.get_random_data (data/numpy_loader.py:132-134)."""
+  dkey, sub = jr.split(dkey)
+  return dkey, jr.randint(sub, (n,), 0, N)
+
+
+# ------------------------------------------------------------ integrators ----
+
+def sgld(pot, theta0, iterations, eps, temperature, N, n, leaf_sizes,
+         key=None, dkey=None, rms=None):
+  """langevin_diffusion (integrator.py:859-922), optional rms_prop
+  (adaption.py:254-291, decorator at 120-222; v0 = ones, update before get)."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  theta = np.asarray(theta0, np.float64).copy()
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  temp = np.broadcast_to(np.asarray(temperature, np.float32), (iterations,))
+  v = np.ones_like(theta) if rms is not None else None
+  out = {"theta": [], "U": [], "variance": [], "idx": [], "noise": [], "grad": []}
+  for t in range(iterations):
+    key, sub = jr.split(key)
+    dkey, idx = draw_indices(dkey, N, n)
+    xi = jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
+    U, g, lik = pot(theta, idx)
+    var = np.var(lik)
+    e = eps[t]
+    sg = float(-e) * g
+    sn = float(np.sqrt(F32(F32(2) * temp[t]) * e)) * xi
+    if rms is None:
+      theta = theta + (sg + sn)
+    else:
+      alpha, lmbd = rms
+      v = alpha * v + (1.0 - alpha) * g * g
+      G = 1.0 / (lmbd + np.sqrt(v))
+      theta = theta + ((float(e) * 0.0 + G * sg) + np.sqrt(G) * sn)
+    for k, val in (("theta", theta), ("U", U), ("variance", var), ("idx", idx),
+                   ("noise", xi), ("grad", g)):
+      out[k].append(np.copy(val))
+  res = {k: np.asarray(v_) for k, v_ in out.items()}
+  res["key"], res["dkey"] = key, dkey
+  if rms is not None:
+    res["v"] = v
+  return res
+
+
+def sghmc(pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
+          key=None, dkey=None, inv_mass=None, sqrt_mass=None):
+  """friction_leapfrog.integrate (integrator.py:715-757) with _body_fun (603-666),
+  no noise model: momentum resampled each iteration, then ``steps`` x {position,
+  friction decay, gradient, C * sqrt(2 eps) noise}."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  theta = np.asarray(theta0, np.float64).copy()
+  P = theta.size
+  inv_mass = np.ones(P) if inv_mass is None else np.asarray(inv_mass, np.float64)
+  sqrt_mass = np.ones(P) if sqrt_mass is None else np.asarray(sqrt_mass, np.float64)
+  C = np.broadcast_to(np.asarray(friction, np.float32), (P,)).astype(np.float64)
+  Cf = np.broadcast_to(np.asarray(friction, np.float32), (P,))
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  out = {"theta": [], "p": [], "U": [], "idx": []}
+  for t in range(iterations):
+    e = eps[t]
+    key, sub = jr.split(key)
+    p = sqrt_mass * jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
+    neg_eC = (F32(-e) * Cf).astype(np.float64)
+    noise_scale = float(np.sqrt(F32(2) * e))
+    U = 0.0
+    for _ in range(steps):
+      s = inv_mass * p
+      theta = theta + float(e) * s
+      p = p + neg_eC * s
+      dkey, idx = draw_indices(dkey, N, n)
+      U, g, _ = pot(theta, idx)
+      p = p + float(-e) * g
+      key, sub = jr.split(key)
+      xi = jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
+      p = p + C * (noise_scale * xi)
+      out["idx"].append(idx)
+    out["theta"].append(theta.copy())
+    out["p"].append(p.copy())
+    out["U"].append(U)
+  res = {k: np.asarray(v_) for k, v_ in out.items()}
+  res["key"], res["dkey"] = key, dkey
+  return res
+
+
+def obabo_integrate(pot, theta, p, key, dkey, eps, temperature, friction, steps,
+                    N, n, leaf_sizes, inv_mass, sqrt_mass, ke_start=0.0, ke_end=0.0):
+  """obabo.integrate (integrator.py:317-338) over _leapfrog_steps (203-273)."""
+  e = F32(eps)
+  T = F32(temperature)
+  perm = np.exp(F32(-F32(friction) * e)).astype(np.float32)
+  a_noise = float(np.sqrt(F32((F32(1) - perm) * T)))
+  a_decay = float(np.sqrt(perm))
+  half = float(F32(0.5) * e)
+  U = 0.0
+  for _ in range(steps):
+    key, s1, s2 = jr.split(key, 3)
+    xi1 = jr.random_tree_flat(s1, leaf_sizes).astype(np.float64)
+    p1 = a_decay * p + a_noise * (sqrt_mass * xi1)
+    ke_start = ke_start + 0.5 * np.dot(p1, inv_mass * p1)
+    dkey, idx0 = draw_indices(dkey, N, n)
+    U0, g0, _ = pot(theta, idx0)
+    p2 = (-half) * g0 + p1
+    theta = theta + float(e) * (inv_mass * p2)
+    dkey, idx1 = draw_indices(dkey, N, n)
+    U1, g1, _ = pot(theta, idx1)
+    p3 = (-half) * g1 + p2
+    xi2 = jr.random_tree_flat(s2, leaf_sizes).astype(np.float64)
+    p = a_decay * p3 + a_noise * (sqrt_mass * xi2)
+    ke_end = ke_end + 0.5 * np.dot(p3, inv_mass * p3)
+    U = 0.5 * (U0 + U1)
+  return theta, p, key, dkey, U, ke_start, ke_end
+
+
+def obabo(pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
+          temperature=1.0, key=None, dkey=None, inv_mass=None, sqrt_mass=None):
+  """alias.obabo through solver.sgmc: p0 = 0 (integrator.py:303), KE carried."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  theta = np.asarray(theta0, np.float64).copy()
+  P = theta.size
+  inv_mass = np.ones(P) if inv_mass is None else np.asarray(inv_mass, np.float64)
+  sqrt_mass = np.ones(P) if sqrt_mass is None else np.asarray(sqrt_mass, np.float64)
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  p = np.zeros(P)
+  ks, ke = 0.0, 0.0
+  out = {"theta": [], "p": [], "U": [], "ke_start": [], "ke_end": []}
+  for t in range(iterations):
+    theta, p, key, dkey, U, ks, ke = obabo_integrate(
+        pot, theta, p, key, dkey, eps[t], temperature, friction, steps, N, n,
+        leaf_sizes, inv_mass, sqrt_mass, ks, ke)
+    for k, val in (("theta", theta), ("p", p), ("U", U), ("ke_start", ks),
+                   ("ke_end", ke)):
+      out[k].append(np.copy(val))
+  res = {k: np.asarray(v_) for k, v_ in out.items()}
+  res["key"], res["dkey"] = key, dkey
+  return res
+
+
+# ---------------------------------------------------------------- solvers ----
+
+def sggmc(pot, full_pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
+          temperature=1.0, key=None, dkey=None):
+  """solver.sggmc (solver.py:462-567): OBABO proposal, full-data MH test
+  log a = min(0, -(U_full' - U_full + KE_end - KE_start) / T), accept iff
+  log(u) < log a; on reject theta, p and U revert while keys and data state advance;
+  kinetic-energy accumulators reset after every test."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  theta = np.asarray(theta0, np.float64).copy()
+  P = theta.size
+  ones = np.ones(P)
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  U_full = full_pot(theta)
+  ikey, key = jr.split(key)
+  p = np.zeros(P)
+  out = {"theta": [], "accept": [], "log_alpha": [], "U_full": []}
+  for t in range(iterations):
+    key, sub = jr.split(key)
+    th_new, p_new, ikey, dkey, _, ks, ke = obabo_integrate(
+        pot, theta, p, ikey, dkey, eps[t], temperature, friction, steps, N, n,
+        leaf_sizes, ones, ones)
+    U_new = full_pot(th_new)
+    log_alpha = -1.0 / float(temperature) * (U_new - U_full + ke - ks)
+    log_alpha = min(log_alpha, 0.0)
+    u = jr.uniform(sub)
+    acc = float(np.log(np.float64(u))) < log_alpha
+    if acc:
+      theta, p, U_full = th_new, p_new, U_new
+    for k, val in (("theta", theta), ("accept", acc), ("log_alpha", log_alpha),
+                   ("U_full", U_full)):
+      out[k].append(np.copy(val))
+  return {k: np.asarray(v_) for k, v_ in out.items()}
+
+
+def resgld_swap_decision(U_normal, U_hot, T_normal, T_hot, ssq, F, key):
+  """solver.py:279-291: returns (swap?, new key). The reference keeps the pair when
+  log u < log s and swaps otherwise."""
+  temps = F32(F32(1.0) / F32(T_normal) - F32(1.0) / F32(T_hot))
+  correction = F32(temps * F32(ssq) / F32(F))
+  log_s = F32(temps * F32(F32(U_normal) - F32(U_hot) - correction))
+  key, sub = jr.split(key)
+  log_u = jr.logf_f32(jr.uniform(sub).reshape(1))[0]
+  return not (log_u < log_s), key
