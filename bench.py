@@ -1,0 +1,256 @@
+"""Headline benchmark: chain-steps/s of the fused multi-chain SGHMC step on one MI355X.
+
+Workload (the BASELINE config that fits one GPU and carries the metric): SGHMC
+(friction_leapfrog, 5 leapfrog steps, friction 0.5) on Bayesian logistic regression,
+X ~ N(0,1)/sqrt(D) float32 [N=1,000,000, D=512], y ~ Bernoulli(sigmoid(X w*)),
+w* ~ N(0, I), prior N(0, I), minibatch 4096, 64 independent chains advanced together.
+A step is one reference iteration for every chain (5 gradient evaluations each).
+Data is synthetic, drawn on the GPU with the library's jax-compatible Threefry
+normal/uniform from PRNGKey(1).
+
+Also reported: the plain single-chain path (separate index-draw, gather, potential
+and update kernels per gradient evaluation), the fused/plain ratio with a check that
+both compute the same chain, the roofline of the dominant kernel, and the oracle's C
+port timed on the host cores as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "chain-steps/sec on 1 MI355X, fused vs plain HIP, at matched posterior error"
+HBM_PEAK_GBS = 8000.0
+N_OBS, D, BATCH, CHAINS, LEAPFROG, FRICTION = 1_000_000, 512, 4096, 64, 5, 0.5
+EPS_FIRST, EPS_LAST = 1e-2, 1e-3
+# Algorithmic HBM bytes of one gradient evaluation of one chain: n rows of 4D bytes,
+# n labels and n indices, theta and momentum read and written.
+BYTES_PER_CHAIN_GRAD = BATCH * (4 * D + 8) + 16 * D
+
+
+def log(msg):
+  print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_data(torch, R):
+  """X ~ N(0,1)/sqrt(D), w* ~ N(0,1), y = 1[u < sigmoid(X w*)] from PRNGKey(1)."""
+  kx, kw, ku = R.split(R.PRNGKey(1), 3)
+  X = R.normal(kx, (N_OBS * D,)).reshape(N_OBS, D)
+  X.mul_(1.0 / np.sqrt(D))
+  w = R.normal(kw, (D,))
+  u = R.uniform(ku, (N_OBS,))
+  y = (u < torch.sigmoid(X @ w)).to(torch.float32)
+  return X, y, w
+
+
+def step_sizes(iters):
+  sys.path.insert(0, ROOT)
+  from jax_sgmc_amd import scheduler
+  return scheduler.polynomial_table(EPS_FIRST, EPS_LAST, iters)
+
+
+def chain_keys(R, C):
+  keys = R.key_words(R.split(R.PRNGKey(0), C))
+  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
+  return keys, dkeys
+
+
+def run_fused(torch, fused, R, X, y, K, W, graph_iters=10):
+  C = CHAINS
+  keys, dkeys = chain_keys(R, C)
+  eps = step_sizes(K + W + graph_iters)
+  chains = [dict(X=X, y=y, D=D, n=BATCH, theta0=np.zeros(D, np.float32), key=tuple(keys[c]),
+                 dkey=tuple(dkeys[c])) for c in range(C)]
+  eng = fused.FusedGLM(chains, "logistic", "sghmc", eps, steps=LEAPFROG, friction=FRICTION,
+                       prior_scale=1.0)
+  tuned = eng.autotune(reps=2)
+  log(f"autotune: {tuned}")
+  g = eng.capture(graph_iters)
+  # warmup (graph replays) then the timed region, bracketed by synchronizations
+  for _ in range(max(1, W // graph_iters)):
+    g.replay()
+  torch.cuda.synchronize()
+  reps = max(1, K // graph_iters)
+  stream = torch.cuda.current_stream()
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  t0 = time.perf_counter()
+  e0.record(stream)
+  for _ in range(reps):
+    g.replay()
+  e1.record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  ms = e0.elapsed_time(e1)
+  steps = reps * graph_iters
+  ok = bool(torch.isfinite(eng.theta_flat).all().item())
+  return eng, dict(ms=ms, wall=wall, steps=steps, launches=steps * LEAPFROG, finite=ok,
+                   nblocks=eng.nblocks, tuned=tuned)
+
+
+def run_plain(torch, L, R, T, X, y, iters, key, dkey):
+  """One chain through the modular kernels: per gradient evaluation an index draw,
+  a row gather, the potential/gradient kernel and the elementwise update kernels."""
+  dev = X.device
+  lay = T.Layout(torch.zeros(D))
+  table, nb = lay.leaf_table(dev, L.PAIRS_PER_BLOCK)
+  theta = torch.zeros(D, device=dev)
+  p = torch.zeros(D, device=dev)
+  g = torch.empty(D, device=dev)
+  U = torch.empty(1, device=dev)
+  k = [R.key_from_words(key), torch.empty(2, dtype=torch.int32, device=dev)]
+  dk = [R.key_from_words(dkey), torch.empty(2, dtype=torch.int32, device=dev)]
+  idx = torch.empty(BATCH, dtype=torch.int32, device=dev)
+  Xb = torch.empty(BATCH, D, device=dev)
+  yb = torch.empty(BATCH, device=dev)
+  blocks = L.query("sgmc_glm_blocks", BATCH, D)
+  ws = torch.zeros(int(L.query("sgmc_glm_workspace_floats", BATCH, D, blocks)), device=dev)
+  eps = step_sizes(iters)
+  s = L.stream()
+
+  def one(t):
+    e = float(eps[t])
+    L.call("sgmc_sghmc_resample", L.ptr(p), None, L.ptr(table), nb, L.ptr(k[0]), L.ptr(k[1]), s)
+    k.reverse()
+    for _ in range(LEAPFROG):
+      L.call("sgmc_sghmc_position", L.ptr(theta), L.ptr(p), None, None, FRICTION, D, e, s)
+      L.call("sgmc_draw_batch", L.ptr(dk[0]), L.ptr(dk[1]), L.ptr(idx), BATCH, N_OBS, s)
+      dk.reverse()
+      L.call("sgmc_gather_rows", L.ptr(X), N_OBS, L.ptr(idx), L.ptr(Xb), BATCH, 4 * D, 4 * D, s)
+      L.call("sgmc_gather_rows", L.ptr(y), N_OBS, L.ptr(idx), L.ptr(yb), BATCH, 4, 4, s)
+      L.call("sgmc_glm_potential", L.LIK_LOGISTIC, L.ptr(Xb), D, L.ptr(yb), None, BATCH, N_OBS,
+             D, L.ptr(theta), 1.0, 1, 1.0, 1.0, L.ptr(U), L.ptr(g), None, None, L.ptr(ws),
+             ws.numel(), blocks, s)
+      L.call("sgmc_sghmc_momentum", L.ptr(p), L.ptr(g), None, FRICTION, L.ptr(table), nb,
+             L.ptr(k[0]), L.ptr(k[1]), e, s)
+      k.reverse()
+
+  return one, theta
+
+
+def cpu_baseline(X_host, y_host, keys, dkeys, target_s=12.0):
+  from oracle import cport
+  threads = max(1, min(16, os.cpu_count() or 1))
+  C = min(CHAINS, threads)
+  eps = step_sizes(1000)
+
+  def go(iters):
+    th = np.zeros((C, D), np.float32)
+    kk = np.array(keys[:C], np.uint32).copy()
+    dd = np.array(dkeys[:C], np.uint32).copy()
+    t0 = time.perf_counter()
+    cport.sghmc(1, X_host, y_host, BATCH, th, kk, dd, eps[:iters], LEAPFROG, FRICTION,
+                threads=threads)
+    return time.perf_counter() - t0
+
+  t1 = go(1)
+  iters = int(max(1, min(200, target_s / max(t1, 1e-3))))
+  dt = go(iters)
+  return {"value": C * iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
+          "sample": f"{C} chains x {iters} iterations of the same workload "
+                    f"(oracle/cport.c SGHMC logistic, OpenMP over chains, {dt:.1f} s)"}
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=1000)
+  ap.add_argument("--warmup", type=int, default=50)
+  ap.add_argument("--no-cpu", action="store_true")
+  ap.add_argument("--no-plain", action="store_true")
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+          "(chains shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+    sys.exit(2)
+
+  import torch
+  import __graft_entry__
+  __graft_entry__.load()
+  from jax_sgmc_amd import _lib as L, fused, random as R, tree as T
+  L.require_gpu()
+
+  log("generating data")
+  X, y, _ = make_data(torch, R)
+  torch.cuda.synchronize()
+  K, W = args.steps, args.warmup
+  log(f"fused run: {CHAINS} chains, {K} timed iterations")
+  eng, fr = run_fused(torch, fused, R, X, y, K, W)
+  chain_steps = CHAINS * fr["steps"]
+  value = chain_steps / (fr["ms"] / 1e3)
+  avg_launch_s = fr["ms"] / 1e3 / fr["launches"]
+  bytes_launch = CHAINS * BYTES_PER_CHAIN_GRAD
+  achieved = bytes_launch / avg_launch_s / 1e9
+  out = {
+      "metric": METRIC, "value": value, "unit": "chain-steps/s", "n_gpus": 1,
+      "steps": fr["steps"], "warmup": W, "ms_per_step": fr["ms"] / fr["steps"],
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+      "data": "synthetic (jax-compatible Threefry draws from PRNGKey(1); random-init chains)",
+      "config": {"workload": "SGHMC logistic regression, N=1e6, D=512, batch 4096, "
+                             "64 chains, 5 leapfrog steps, friction 0.5",
+                 "chains": CHAINS, "N": N_OBS, "D": D, "batch": BATCH,
+                 "leapfrog_steps": LEAPFROG, "friction": FRICTION,
+                 "step_size": f"polynomial {EPS_FIRST} -> {EPS_LAST}",
+                 "grid_blocks": fr["nblocks"], "graph_iterations": 10,
+                 "target_bytes_per_block": fr["tuned"]["target_bytes"]},
+      "gradient_evals_per_s": value * LEAPFROG,
+  }
+  traffic = None
+  tp = os.path.join(ROOT, "profiles", "traffic_glm_fused_sghmc_logistic.json")
+  if os.path.exists(tp):
+    with open(tp) as f:
+      traffic = json.load(f).get("hbm_bytes_per_launch")
+  out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "glm_fused_kernel<3> (sgmc_glm_fused_step)",
+                     "bytes_per_launch": bytes_launch, "avg_launch_us": avg_launch_s * 1e6}
+
+  keys, dkeys = chain_keys(R, CHAINS)
+  if not args.no_plain:
+    log("plain single-chain path")
+    one, th_plain = run_plain(torch, L, R, T, X, y, 64, tuple(keys[0]), tuple(dkeys[0]))
+    one(0)
+    torch.cuda.synchronize()
+    it = 40
+    t0 = time.perf_counter()
+    for t in range(1, 1 + it):
+      one(t)
+    torch.cuda.synchronize()
+    plain_rate = it / (time.perf_counter() - t0)
+    # same chain through both paths: chain 0 of a fresh fused table vs the plain path
+    eng2 = fused.FusedGLM([dict(X=X, y=y, D=D, n=BATCH, theta0=np.zeros(D, np.float32),
+                                key=tuple(keys[0]), dkey=tuple(dkeys[0]))], "logistic",
+                          "sghmc", step_sizes(64), steps=LEAPFROG, friction=FRICTION)
+    one2, th2 = run_plain(torch, L, R, T, X, y, 64, tuple(keys[0]), tuple(dkeys[0]))
+    for t in range(8):
+      one2(t)
+    eng2.run(8)
+    torch.cuda.synchronize()
+    a, b = eng2.theta(0).cpu().numpy(), th2.cpu().numpy()
+    rel = float(np.max(np.abs(a - b)) / max(1e-30, float(np.max(np.abs(b)))))
+    out["plain"] = {"value": plain_rate, "unit": "chain-steps/s", "chains": 1,
+                    "path": "per gradient: draw_batch, 2x gather_rows, glm_potential, "
+                            "sghmc_position, sghmc_momentum (eager launches)",
+                    "same_chain_max_rel_diff_after_8_iterations": rel}
+    out["fused_over_plain"] = value / plain_rate
+
+  if not args.no_cpu:
+    log("CPU baseline (oracle C port)")
+    X_host = X.cpu().numpy()
+    y_host = y.cpu().numpy()
+    out["cpu_baseline"] = cpu_baseline(X_host, y_host, keys, dkeys)
+    out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+  else:
+    out["cpu_baseline"] = None
+  print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+  main()

@@ -165,7 +165,8 @@ int sgmc_obabo_b(float* p, const float* grad, int64_t P, float eps, void* stream
  * eps_table[t] with t = *iter (a device counter the launch advances after the last
  * step of an iteration), so a captured graph of launches replays without host input.
  * save_slot (nullable, int32[table_len]): where save_slot[t] >= 0 the chain's
- * post-step theta is written to samples[save_slot[t]][D]. */
+ * post-step theta is written to samples[save_slot[t]][D] and the potential of the
+ * step (the value the integrator state stores) to samples_U[save_slot[t]]. */
 typedef struct sgmc_glm_chain {
   const float* X;      /* [N, ldx] rows of this chain's dataset */
   const float* y;      /* [N] */
@@ -180,6 +181,7 @@ typedef struct sgmc_glm_chain {
   float* partial;      /* [nblocks][4 ceil(D/4) + 1] scratch */
   unsigned* counter;   /* [1] arrival counter, zero between launches */
   float* samples;      /* [slots][D] or NULL */
+  float* samples_U;    /* [slots] potential of each saved sample, or NULL */
   int64_t N;           /* observations */
   int64_t ldx;         /* row stride in floats */
   int32_t D;           /* features (<= 1024) */

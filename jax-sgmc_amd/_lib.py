@@ -35,7 +35,7 @@ class GlmChain(ctypes.Structure):
       ("X", c_void_p), ("y", c_void_p), ("theta", c_void_p), ("mom", c_void_p),
       ("v", c_void_p), ("key", c_void_p), ("dkey", c_void_p), ("U", c_void_p),
       ("var", c_void_p), ("lik", c_void_p), ("partial", c_void_p), ("counter", c_void_p),
-      ("samples", c_void_p), ("N", c_int64), ("ldx", c_int64), ("D", c_int32),
+      ("samples", c_void_p), ("samples_U", c_void_p), ("N", c_int64), ("ldx", c_int64), ("D", c_int32),
       ("n", c_int32), ("block_begin", c_int32), ("nblocks", c_int32), ("dclass", c_int32),
       ("temperature", c_float),
   ]

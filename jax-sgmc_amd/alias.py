@@ -1,0 +1,255 @@
+"""Pre-built samplers with the reference's signatures (alias.py:30-619).
+
+Each returns ``run_fn(*init_samples, init_model_state=None, iterations=1000)`` whose
+result is a list (one entry per chain) of ``{"sample_count": k, "samples":
+{"variables": ..., "likelihood" | "energy": ..., "model_state": ...}}``.
+
+Defaults reproduce the reference: every chain starts from the integrator key
+PRNGKey(0) and the device data key PRNGKey(0) (integrator.py:804,
+data/numpy_loader.py:124), so chains started from the same sample coincide. As an
+extension, ``run_fn`` accepts ``keys=`` and ``data_keys=`` (one jax key per chain)
+and ``fused=`` (None: automatic).
+
+Execution: when the potential is a recognized GLM family with a Gaussian (or zero)
+prior, the data loader is device-resident, and the integrator is SGLD (with or
+without rms_prop) or SGHMC with unit mass and scalar friction, all chains run
+together in the fused multi-chain kernel (``fused.FusedGLM``); the result is the
+same chain by chain. Otherwise the chains run one after another through the
+modular kernels (solver.mcmc with strategy 'map', as in the reference).
+"""
+from functools import partial
+
+import numpy as np
+import torch
+
+from . import adaption as AD
+from . import data as DT
+from . import integrator as IG
+from . import io as IO
+from . import likelihoods as LK
+from . import potential as PT
+from . import random as R
+from . import scheduler as SC
+from . import solver as SV
+from . import tree as T
+
+
+def _schedule(first_step_size, last_step_size, burn_in, accepted_samples, temperature=None):
+  step = SC.polynomial_step_size_first_last(first=first_step_size, last=last_step_size)
+  burn = SC.initial_burn_in(burn_in)
+  thin = SC.random_thinning(step, burn, selections=accepted_samples) \
+      if accepted_samples is not None else None
+  return SC.init_scheduler(step_size=step, burn_in=burn, thinning=thin,
+                           temperature=temperature)
+
+
+def _fused_eligible(potential_fn, data_loader, init_samples, fused):
+  if fused is False:
+    return False
+  ok = (isinstance(potential_fn, PT.StochasticPotential) and potential_fn.fast
+        and isinstance(data_loader, DT.DeviceNumpyDataLoader))
+  if ok:
+    for s in init_samples:
+      leaves, _ = T.flatten(s)
+      if len(leaves) != 1 or np.ndim(np.asarray(leaves[0].cpu() if isinstance(
+          leaves[0], torch.Tensor) else leaves[0])) != 1:
+        ok = False
+    X = data_loader.reference_data.get(potential_fn.likelihood.x)
+    if X is None or X.dim() != 2 or X.shape[1] > 1024 or X.stride(0) % 4:
+      ok = False
+  if fused and not ok:
+    raise ValueError("fused=True needs a GLM family potential, a DeviceNumpyDataLoader and "
+                     "single-leaf 1-D samples")
+  return ok
+
+
+def _key_words(k):
+  if k is None:
+    return (0, 0)
+  if isinstance(k, torch.Tensor):
+    return tuple(int(v) for v in R.key_words(k))
+  return tuple(int(v) for v in np.asarray(k, np.uint32))
+
+
+def _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, sched, mode,
+               keys, data_keys, steps=1, friction=1.0, alpha=0.9, lmbd=1e-5, to_numpy=True,
+               result_key="likelihood"):
+  init_fn = sched[0]
+  st, _ = init_fn(iterations)
+  eps, temp, keep = init_fn.tables(st)
+  slots = np.full(iterations, -1, np.int32)
+  kept = np.nonzero(keep)[0]
+  slots[kept] = np.arange(kept.size, dtype=np.int32)
+  lik = potential_fn.likelihood
+  X = data_loader.reference_data[lik.x]
+  y = data_loader.reference_data[lik.y].to(torch.float32).contiguous()
+  has_prior, pscale = PT._prior_spec(potential_fn.prior)
+  chains = []
+  for c, s in enumerate(init_samples):
+    leaf = T.flatten(s)[0][0]
+    th0 = leaf.detach().cpu().numpy() if isinstance(leaf, torch.Tensor) else np.asarray(leaf)
+    chains.append(dict(X=X, y=y, D=X.shape[1], n=batch_size, theta0=th0.astype(np.float32),
+                       key=_key_words(keys[c] if keys else None),
+                       dkey=_key_words(data_keys[c] if data_keys else None),
+                       temperature=float(temp[0]) if temp.size else 1.0))
+  from . import fused as FU
+  eng = FU.FusedGLM(chains, lik.family, mode, eps, steps=steps, friction=friction, alpha=alpha,
+                    lmbd=lmbd, sigma=lik.sigma, prior_scale=pscale, has_prior=bool(has_prior),
+                    pot_temperature=potential_fn.temperature, save_slots=slots,
+                    n_slots=max(1, kept.size))
+  eng.run(iterations)
+  torch.cuda.synchronize()
+  out = []
+  for c, s in enumerate(init_samples):
+    tr = eng.sample_trace(c)[:kept.size]
+    Us = eng.sample_potentials(c)[:kept.size]
+    _, td = T.flatten(s)
+    var = T.unflatten(td, [tr.clone()])
+    val = -Us if result_key == "likelihood" else Us
+    samples = {"variables": var, result_key: val.clone(), "model_state": None}
+    if to_numpy:
+      samples = T.tree_map(lambda x: x.cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                           samples)
+    out.append({"sample_count": int(kept.size), "samples": samples})
+  return out
+
+
+def sgld(potential_fn, data_loader, cache_size=512, batch_size=32, first_step_size=0.05,
+         last_step_size=0.001, burn_in=0, accepted_samples=1000, rms_prop=False, alpha=0.9,
+         lmbd=1e-5, save_to_numpy=True, progress_bar=True):
+  """alias.py:30-120."""
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  adapt = AD.rms_prop() if rms_prop else None
+  integ = IG.langevin_diffusion(potential_fn, random_data, adaption=adapt)
+  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  solver = SV.sgmc(integ)
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
+             data_keys=None, fused=None):
+    if init_model_state is None and _fused_eligible(potential_fn, data_loader, init_samples,
+                                                    fused):
+      return _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations,
+                        sched, "sgld_rms" if rms_prop else "sgld", keys, data_keys,
+                        alpha=alpha, lmbd=lmbd, to_numpy=save_to_numpy)
+    states = []
+    for c, s in enumerate(init_samples):
+      kw = dict(adaption_kwargs={"alpha": alpha, "lmbd": lmbd}, init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      if data_keys:
+        kw["batch_kwargs"] = {"key": data_keys[c]}
+      states.append(solver[0](s, **kw))
+    return mcmc(*states, iterations=iterations)
+
+  return run_fn
+
+
+def sghmc(potential_fn, data_loader, cache_size=512, batch_size=32, integration_steps=10,
+          friction=1.0, mass=None, first_step_size=0.05, last_step_size=0.001, burn_in=0,
+          accepted_samples=1000, adapt_noise_model=False, diagonal_noise=True,
+          save_to_numpy=True, progress_bar=True):
+  """alias.py:451-540."""
+  if adapt_noise_model:
+    raise NotImplementedError("adapt_noise_model=True (fisher_information) is not provided "
+                              "by this build")
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  integ = IG.friction_leapfrog(potential_fn, random_data, friction=friction, const_mass=mass,
+                               steps=integration_steps)
+  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  solver = SV.sgmc(integ)
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
+             data_keys=None, fused=None):
+    scalar_fric = np.ndim(friction) == 0 and not isinstance(friction, (dict, list, tuple))
+    if (init_model_state is None and mass is None and scalar_fric and
+        _fused_eligible(potential_fn, data_loader, init_samples, fused)):
+      return _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                        "sghmc", keys, data_keys, steps=integration_steps,
+                        friction=float(friction), to_numpy=save_to_numpy,
+                        result_key="energy")
+    states = []
+    for c, s in enumerate(init_samples):
+      kw = dict(init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      if data_keys:
+        kw["batch_kwargs"] = {"key": data_keys[c]}
+      states.append(solver[0](s, **kw))
+    return mcmc(*states, iterations=iterations)
+
+  return run_fn
+
+
+def obabo(potential_fn, data_loader, cache_size=512, batch_size=32, integration_steps=10,
+          friction=1.0, mass=None, first_step_size=0.05, last_step_size=0.001, burn_in=0,
+          accepted_samples=1000, save_to_numpy=True, progress_bar=True):
+  """alias.py:542-619."""
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  integ = IG.obabo(potential_fn=potential_fn, batch_fn=random_data, steps=integration_steps,
+                   friction=friction, const_mass=mass)
+  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  solver = SV.sgmc(integ)
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
+             data_keys=None):
+    states = []
+    for c, s in enumerate(init_samples):
+      kw = dict(init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      if data_keys:
+        kw["batch_kwargs"] = {"key": data_keys[c]}
+      states.append(solver[0](s, **kw))
+    return mcmc(*states, iterations=iterations)
+
+  return run_fn
+
+
+def re_sgld(potential_fn, data_loader, cache_size=512, batch_size=32, temperature=1000.0,
+            first_step_size=0.05, last_step_size=0.001, burn_in=0, accepted_samples=100,
+            save_to_numpy=True, progress_bar=True):
+  """alias.py:122-208: a normal and a hot chain exchanging states."""
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  integ = IG.langevin_diffusion(potential_fn, random_data)
+  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples,
+                    temperature=SC.constant_temperature(1.0))
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  solver = SV.parallel_tempering(integ)
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000):
+    states = [solver[0](a, b, init_model_state=init_model_state) for a, b in init_samples]
+    return mcmc(*states, iterations=iterations,
+                schedulers=[{"temperature": {"tau": temperature}}])
+
+  return run_fn
+
+
+def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=512,
+          batch_size=32, integration_steps=10, friction_coefficient=1.0, first_step_size=0.001,
+          last_step_size=0.001, adaptive_step_size=False, stabilization_constant=10,
+          decay_constant=0.75, speed_constant=0.05, target_acceptance_rate=0.25, burn_in=0,
+          accepted_samples=None, mass=None, save_to_numpy=True, progress_bar=True):
+  """alias.py:330-449: OBABO proposals with a full-data Metropolis-Hastings test."""
+  if adaptive_step_size:
+    raise NotImplementedError("adaptive_step_size is not provided by this build")
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  full_map = DT.full_reference_data(data_loader, cache_size, batch_size)
+  integ = IG.obabo(stochastic_potential_fn, random_data, integration_steps,
+                   friction_coefficient, mass)
+  solver = SV.sggmc(integ, full_potential_fn, full_map)
+  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000):
+    states = [solver[0](s, init_model_state=init_model_state) for s in init_samples]
+    return mcmc(*states, iterations=iterations)
+
+  return run_fn

@@ -532,7 +532,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   }
   vs = block_sum<kNT>(vs, lds_small);
   if (threadIdx.x == 0) {
-    ch.U[0] = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
+    const float Uc = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
+    ch.U[0] = Uc;
+    if (slot >= 0 && last_step && ch.samples_U) ch.samples_U[slot] = Uc;
     if (ch.var) ch.var[0] = vs / (float)ch.n;
     store_key(ch.key, knext);
     store_key(ch.dkey, dnext);

@@ -1,0 +1,276 @@
+"""Fused multi-chain SG-MCMC for generalized-linear likelihoods.
+
+Every chain of a chain table advances by one gradient evaluation per kernel launch
+(``sgmc_glm_fused_step``): index draw, row gather, GLM forward/backward, reduction
+and the integrator update with in-register noise, for all chains at once. This
+replaces solver.mcmc's sequential per-chain loop (solver.py:162-171, strategy
+'map') and its jit(lax.scan) over iterations (solver.py:152-160): the step size of
+iteration t is read on the device from a schedule table, the iteration counter lives
+on the device, so a captured HIP graph of one or more iterations replays without
+host input.
+
+Chains may differ in dataset, N, D (<= 1024), minibatch size and temperature
+(tempering ladders); they share the family, the prior and the integrator. Per-chain
+results equal the same chain run alone (chains are independent; every chain owns
+its keys and its reduction order is fixed by its own block count).
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+MODES = {"sgld": _lib.MODE_SGLD, "sgld_rms": _lib.MODE_SGLD_RMS, "sghmc": _lib.MODE_SGHMC}
+FAMILIES = {"linear_gaussian": _lib.LIK_LINEAR_GAUSSIAN, "logistic": _lib.LIK_LOGISTIC}
+NT = 256
+
+
+def _align4(x):
+  return (x + 3) // 4 * 4
+
+
+def blocks_for(n, D, target_bytes):
+  """Grid blocks for one chain: about target_bytes of rows per block, at least 32
+  rows per block."""
+  rows_bytes = n * 4 * _align4(D)
+  nb = int(round(rows_bytes / max(1, target_bytes)))
+  return max(1, min(nb, (n + 31) // 32))
+
+
+class FusedGLM:
+  """Chain table for ``sgmc_glm_fused_step``.
+
+  chains: list of dicts with keys X (device float32 [N, ldx], ldx % 4 == 0, zero
+  padding beyond D), y (device float32 [N]), D, n, theta0 (array-like [D]), key and
+  dkey (two uint32 words each), temperature (SGLD noise temperature, default 1).
+  """
+
+  def __init__(self, chains, family, mode, eps_table, steps=1, friction=1.0, alpha=0.9,
+               lmbd=1e-5, sigma=1.0, prior_scale=1.0, has_prior=True, pot_temperature=1.0,
+               inv_mass=None, sqrt_mass=None, save_slots=None, n_slots=0, target_bytes=None,
+               device="cuda"):
+    _lib.require_gpu()
+    self.device = torch.device(device)
+    self.family = FAMILIES[family] if isinstance(family, str) else int(family)
+    self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    self.steps = int(steps) if self.mode == _lib.MODE_SGHMC else 1
+    self.friction = float(friction)
+    self.alpha, self.lmbd = float(alpha), float(lmbd)
+    self.sigma, self.prior_scale = float(sigma), float(prior_scale)
+    self.has_prior = 1 if has_prior else 0
+    self.pot_temperature = float(pot_temperature)
+    self.spec = chains
+    self.C = len(chains)
+    eps = np.ascontiguousarray(np.asarray(eps_table, np.float32).reshape(-1))
+    self.table_len = eps.size
+    self.eps_table = torch.as_tensor(eps, device=self.device)
+    self.save_slot = None
+    self.n_slots = int(n_slots)
+    if save_slots is not None:
+      ss = np.ascontiguousarray(np.asarray(save_slots, np.int32).reshape(-1))
+      assert ss.size == self.table_len
+      self.save_slot = torch.as_tensor(ss, device=self.device)
+      self.n_slots = max(self.n_slots, int(ss.max()) + 1 if ss.size else 0)
+    self.inv_mass = inv_mass
+    self.sqrt_mass = sqrt_mass
+    Ds = [int(c["D"]) for c in chains]
+    for c in chains:
+      if not 1 <= int(c["D"]) <= 1024:
+        raise ValueError("fused GLM path supports 1 <= D <= 1024")
+      if c["X"].stride(0) % 4 or c["X"].shape[1] < _align4(int(c["D"])):
+        raise ValueError("X rows must be padded to a multiple of 4 floats")
+      if int(c["n"]) < 1 or int(c["n"]) > c["X"].shape[0] and False:
+        raise ValueError("bad minibatch size")
+    dcl = {_lib.query("sgmc_glm_dclass", d) for d in Ds}
+    self.uniform_dclass = dcl.pop() if len(dcl) == 1 else -1
+    total_bytes = sum(int(c["n"]) * 4 * _align4(int(c["D"])) for c in chains)
+    self.target_bytes = target_bytes or max(64 * 1024, total_bytes // 2048)
+    self._allocate()
+    self._build_table()
+
+  # -------------------------------------------------------------- storage ----
+  def _allocate(self):
+    dev = self.device
+    ch = self.spec
+    self.D = np.array([int(c["D"]) for c in ch])
+    self.n = np.array([int(c["n"]) for c in ch])
+    Da = np.array([_align4(d) for d in self.D])
+    self.poff = np.concatenate([[0], np.cumsum(Da)]).astype(np.int64)
+    P = int(self.poff[-1])
+    self.theta_flat = torch.zeros(P, device=dev)
+    self.mom_flat = torch.zeros(P, device=dev) if self.mode == _lib.MODE_SGHMC else None
+    self.v_flat = torch.ones(P, device=dev) if self.mode == _lib.MODE_SGLD_RMS else None
+    for i, c in enumerate(ch):
+      self.theta_flat[self.poff[i]:self.poff[i] + self.D[i]] = torch.as_tensor(
+          np.asarray(c["theta0"], np.float32).reshape(-1), device=dev)
+    keys = np.array([np.asarray(c.get("key", (0, 0)), np.uint32) for c in ch])
+    dkeys = np.array([np.asarray(c.get("dkey", (0, 0)), np.uint32) for c in ch])
+    self.keys = torch.as_tensor(keys.view(np.int32).copy(), device=dev)
+    self.dkeys = torch.as_tensor(dkeys.view(np.int32).copy(), device=dev)
+    self.U = torch.zeros(self.C, device=dev)
+    self.var = torch.zeros(self.C, device=dev)
+    self.loff = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+    self.lik = torch.zeros(int(self.loff[-1]), device=dev)
+    self.counters = torch.zeros(self.C, dtype=torch.int32, device=dev)
+    self.iter = torch.zeros(1, dtype=torch.int32, device=dev)
+    self.done = torch.zeros(1, dtype=torch.int32, device=dev)
+    self.samples = None
+    self.samples_U = None
+    if self.n_slots > 0:
+      self.samples = torch.zeros(self.n_slots * P, device=dev)
+      self.samples_U = torch.zeros(self.C * self.n_slots, device=dev)
+
+  def _build_table(self):
+    nbs = [blocks_for(int(n), int(d), self.target_bytes) for n, d in zip(self.n, self.D)]
+    self.nblocks_chain = np.array(nbs)
+    self.block_begin = np.concatenate([[0], np.cumsum(nbs)]).astype(np.int64)
+    self.nblocks = int(self.block_begin[-1])
+    D4 = (self.D + 3) // 4
+    stride = 4 * D4 + 1
+    pof = np.concatenate([[0], np.cumsum(self.nblocks_chain * stride)]).astype(np.int64)
+    self.partial = torch.zeros(int(pof[-1]), device=self.device)
+    arr = (_lib.GlmChain * self.C)()
+    fs = 4
+    P = int(self.poff[-1])
+    for i, c in enumerate(self.spec):
+      e = arr[i]
+      e.X = c["X"].data_ptr()
+      e.y = c["y"].data_ptr()
+      e.theta = self.theta_flat.data_ptr() + fs * int(self.poff[i])
+      e.mom = self.mom_flat.data_ptr() + fs * int(self.poff[i]) if self.mom_flat is not None else None
+      e.v = self.v_flat.data_ptr() + fs * int(self.poff[i]) if self.v_flat is not None else None
+      e.key = self.keys.data_ptr() + 8 * i
+      e.dkey = self.dkeys.data_ptr() + 8 * i
+      e.U = self.U.data_ptr() + fs * i
+      e.var = self.var.data_ptr() + fs * i
+      e.lik = self.lik.data_ptr() + fs * int(self.loff[i])
+      e.partial = self.partial.data_ptr() + fs * int(pof[i])
+      e.counter = self.counters.data_ptr() + 4 * i
+      e.samples = (self.samples.data_ptr() + fs * int(self.poff[i]) * self.n_slots
+                   if self.samples is not None else None)
+      e.samples_U = (self.samples_U.data_ptr() + fs * i * self.n_slots
+                     if self.samples_U is not None else None)
+      e.N = int(c["X"].shape[0]) if "N" not in c else int(c["N"])
+      e.ldx = int(c["X"].stride(0))
+      e.D = int(self.D[i])
+      e.n = int(self.n[i])
+      e.block_begin = int(self.block_begin[i])
+      e.nblocks = int(self.nblocks_chain[i])
+      e.dclass = _lib.query("sgmc_glm_dclass", int(self.D[i]))
+      e.temperature = float(c.get("temperature", 1.0))
+    raw = bytes(arr)
+    self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+    bc = np.repeat(np.arange(self.C, dtype=np.int32), self.nblocks_chain)
+    self.block_chain = torch.as_tensor(bc, device=self.device)
+    self._arr = arr  # keep the host copy alive for inspection
+
+  # ------------------------------------------------------------ launching ----
+  def _launch(self, step):
+    _lib.call("sgmc_glm_fused_step", _lib.ptr(self.table), self.C, _lib.ptr(self.block_chain),
+              self.nblocks, self.uniform_dclass, self.family, self.sigma, self.has_prior,
+              self.prior_scale, self.pot_temperature, self.mode, step, self.steps,
+              self.friction, self.alpha, self.lmbd, _lib.ptr(self.inv_mass),
+              _lib.ptr(self.sqrt_mass), _lib.ptr(self.eps_table), self.table_len,
+              _lib.ptr(self.save_slot), _lib.ptr(self.iter), _lib.ptr(self.done), _lib.stream())
+
+  def iteration(self):
+    """Advances every chain by one reference iteration (steps launches)."""
+    for s in range(self.steps):
+      self._launch(s)
+
+  def capture(self, iterations_per_graph=1):
+    """Captures ``iterations_per_graph`` iterations into a HIP graph."""
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+      with torch.cuda.graph(g, stream=s):
+        for _ in range(iterations_per_graph):
+          self.iteration()
+    torch.cuda.current_stream().wait_stream(s)
+    self._graph = g
+    self._graph_iters = iterations_per_graph
+    return g
+
+  def run(self, iterations, graph_iterations=0):
+    """Runs ``iterations`` iterations (eagerly, or by replaying a captured graph of
+    ``graph_iterations`` iterations; the remainder runs eagerly)."""
+    if graph_iterations and iterations >= graph_iterations:
+      g = getattr(self, "_graph", None)
+      if g is None or self._graph_iters != graph_iterations:
+        # capturing records the launches without running them
+        g = self.capture(graph_iterations)
+      reps, rest = divmod(iterations, graph_iterations)
+      for _ in range(reps):
+        g.replay()
+      for _ in range(rest):
+        self.iteration()
+    else:
+      for _ in range(iterations):
+        self.iteration()
+
+  # ------------------------------------------------------------- results -----
+  def theta(self, i=None):
+    if i is None:
+      return [self.theta(j) for j in range(self.C)]
+    return self.theta_flat[self.poff[i]:self.poff[i] + self.D[i]]
+
+  def momentum(self, i):
+    return self.mom_flat[self.poff[i]:self.poff[i] + self.D[i]]
+
+  def sample_trace(self, i):
+    """[n_slots, D] samples saved for chain i."""
+    P = int(self.poff[-1])
+    base = int(self.poff[i]) * self.n_slots
+    return self.samples[base:base + self.n_slots * int(self.D[i])].reshape(self.n_slots,
+                                                                          int(self.D[i]))
+
+  def sample_potentials(self, i):
+    """[n_slots] potential stored with each saved sample of chain i."""
+    return self.samples_U[i * self.n_slots:(i + 1) * self.n_slots]
+
+  def iteration_count(self):
+    return int(self.iter.item())
+
+  # ------------------------------------------------------------ autotuner ----
+  def autotune(self, candidates=None, reps=3):
+    """Picks the bytes-per-block target with the shortest launch time; restores the
+    chain state afterwards."""
+    if candidates is None:
+      base = self.target_bytes
+      candidates = sorted({max(16 * 1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4)})
+    snap = self._snapshot()
+    best = None
+    for tb in candidates:
+      self.target_bytes = tb
+      self._build_table()
+      self.iteration()
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      for _ in range(reps):
+        self.iteration()
+      torch.cuda.synchronize()
+      dt = (time.perf_counter() - t0) / reps
+      if best is None or dt < best[0]:
+        best = (dt, tb)
+    self.target_bytes = best[1]
+    self._build_table()
+    self._restore(snap)
+    self.tuned = {"target_bytes": best[1], "seconds_per_iteration": best[0],
+                  "candidates": candidates}
+    return self.tuned
+
+  def _snapshot(self):
+    t = {k: getattr(self, k).clone() for k in ("theta_flat", "keys", "dkeys", "iter", "U", "var")}
+    for k in ("mom_flat", "v_flat", "samples", "samples_U"):
+      v = getattr(self, k)
+      t[k] = v.clone() if v is not None else None
+    return t
+
+  def _restore(self, snap):
+    for k, v in snap.items():
+      if v is not None:
+        getattr(self, k).copy_(v)
+    torch.cuda.synchronize()

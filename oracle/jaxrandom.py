@@ -208,7 +208,8 @@ def erfinv_f32(x):
   for a, b in zip(_ERFINV_LT5[1:], _ERFINV_GE5[1:]):
     p = np.where(lt, a, b).astype(np.float32) + p * w
   res = (p * x).astype(np.float32)
-  edge = (x * F32(np.inf)).astype(np.float32)
+  with np.errstate(invalid="ignore"):
+    edge = (x * F32(np.inf)).astype(np.float32)
   return np.where(np.abs(x) == F32(1.0), edge, res).astype(np.float32)
 
 
