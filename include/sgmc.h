@@ -203,6 +203,53 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                         const float* sqrt_mass, const float* eps_table, int table_len,
                         const int32_t* save_slot, int32_t* iter, unsigned* done, void* stream);
 
+/* ------------------------------------------------------------ MLP (MFMA) ----
+ * Bayesian MLP classifier x -> relu(x W1 + b1) -> relu(. W2 + b2) -> . W3 + b3 with a
+ * 10-class softmax cross-entropy likelihood: the potential and its gradient
+ * (potential.py:159-214 under value_and_grad) by a hand-written backward pass on bf16
+ * MFMA GEMMs with fp32 accumulation, and the langevin / rms_prop update
+ * (integrator.py:859-922, adaption.py:254-291) fused into the weight-gradient GEMMs.
+ * Parameters per chain: float32 [P] in sorted-key order W1, W2, W3, b1, b2, b3, and a
+ * bf16 shadow of the same layout (sgmc_mlp_shadow initializes it). One call runs one
+ * iteration for C chains (update != 0), or writes U and the gradient without updating
+ * (update == 0, grad [C][P], the backward of a torch.autograd.Function). Keys, data
+ * keys, the iteration counter and arrival counters live on the device as for the
+ * fused GLM step. Shapes: n, D_in, H1 multiples of 64; H1, H2 multiples of 128. */
+typedef struct sgmc_mlp_state {
+  const float* X;          /* [N][ldx] inputs */
+  const int32_t* labels;   /* [N] class labels 0..9 */
+  int64_t N, ldx;
+  int32_t D_in, H1, H2, n, C;
+  float* theta;            /* [C][P] */
+  float* v;                /* [C][P] rms_prop second moment, or NULL */
+  uint16_t* shadow;        /* [C][P] bf16 copy of theta */
+  int64_t P;
+  uint32_t* key;           /* [C][2] */
+  uint32_t* dkey;          /* [C][2] */
+  void* work;              /* sgmc_mlp_workspace_bytes(C, n, D_in, H1, H2) bytes */
+  int64_t work_bytes;
+  float* U;                /* [C] */
+  float* grad;             /* [C][P] for update == 0, else NULL */
+  float* samples;          /* [C][n_slots][P] or NULL */
+  float* samples_U;        /* [C][n_slots] or NULL */
+  int32_t n_slots;
+  const float* eps_table;  /* [table_len] */
+  const int32_t* save_slot;/* [table_len] or NULL */
+  int32_t* iter;           /* [1] */
+  unsigned* counters;      /* [C] zeroed once */
+  unsigned* done;          /* [1] zeroed once */
+  int32_t table_len;
+  int32_t rms;
+  float alpha, lmbd, temperature, noise_temperature;
+  int32_t has_prior;
+  float prior_scale;
+} sgmc_mlp_state;
+
+int64_t sgmc_mlp_param_count(int D_in, int H1, int H2);
+int64_t sgmc_mlp_workspace_bytes(int C, int n, int D_in, int H1, int H2);
+int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* stream);
+int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,21 @@ class GlmChain(ctypes.Structure):
   ]
 
 
+class MlpState(ctypes.Structure):
+  """Mirror of ``sgmc_mlp_state`` (include/sgmc.h)."""
+  _fields_ = [
+      ("X", c_void_p), ("labels", c_void_p), ("N", c_int64), ("ldx", c_int64),
+      ("D_in", c_int32), ("H1", c_int32), ("H2", c_int32), ("n", c_int32), ("C", c_int32),
+      ("theta", c_void_p), ("v", c_void_p), ("shadow", c_void_p), ("P", c_int64),
+      ("key", c_void_p), ("dkey", c_void_p), ("work", c_void_p), ("work_bytes", c_int64),
+      ("U", c_void_p), ("grad", c_void_p), ("samples", c_void_p), ("samples_U", c_void_p),
+      ("n_slots", c_int32), ("eps_table", c_void_p), ("save_slot", c_void_p),
+      ("iter", c_void_p), ("counters", c_void_p), ("done", c_void_p), ("table_len", c_int32),
+      ("rms", c_int32), ("alpha", c_float), ("lmbd", c_float), ("temperature", c_float),
+      ("noise_temperature", c_float), ("has_prior", c_int32), ("prior_scale", c_float),
+  ]
+
+
 _SIGNATURES = {
     "sgmc_last_error": (ctypes.c_char_p, []),
     "sgmc_abi_version": (c_int, []),
@@ -82,6 +97,10 @@ _SIGNATURES = {
                                     c_int, c_float, c_float, c_int, c_int, c_int, c_float,
                                     c_float, c_float, c_void_p, c_void_p, c_void_p, c_int,
                                     c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sgmc_mlp_param_count": (c_int64, [c_int, c_int, c_int]),
+    "sgmc_mlp_workspace_bytes": (c_int64, [c_int] * 5),
+    "sgmc_mlp_shadow": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "sgmc_mlp_step": (c_int, [ctypes.POINTER(MlpState), c_int, c_void_p]),
 }
 
 # Entry points added after ABI version 1 register themselves here.

@@ -75,10 +75,14 @@ struct RowSource {
   int draw;            // nonzero: in-kernel randint with (k1, k2)
   Key k1, k2;
   uint32_t n, span, mult;
+  int64_t rows;        // rows addressable through X (bounds guard)
+  // Row index of minibatch row j, or -1 if it falls outside [0, rows).
   __device__ __forceinline__ int64_t row(int j) const {
-    if (draw) return (int64_t)draw_index(k1, k2, n, (uint32_t)j, span, mult);
-    if (idx) return (int64_t)idx[j];
-    return (int64_t)j;
+    int64_t r;
+    if (draw) r = (int64_t)draw_index(k1, k2, n, (uint32_t)j, span, mult);
+    else if (idx) r = (int64_t)idx[j];
+    else r = (int64_t)j;
+    return (r >= 0 && r < rows) ? r : -1;
   }
 };
 
@@ -97,11 +101,11 @@ __device__ __forceinline__ void accumulate_rows(const float* __restrict__ X, int
   const int q = lane % LPR;
   for (int cb = r0; cb < r1; cb += kWave) {
     const int nrows = min(kWave, r1 - cb);
-    int64_t my_row = 0;
+    int64_t my_row = -1;
     float my_y = 0.0f;
     if (lane < nrows) {
       my_row = src.row(cb + lane);
-      my_y = y[my_row];
+      if (my_row >= 0) my_y = y[my_row];
     }
     for (int g = 0; g < nrows; g += RPW * kUnroll) {
       float4 xv[kUnroll][CPL];
@@ -110,11 +114,11 @@ __device__ __forceinline__ void accumulate_rows(const float* __restrict__ X, int
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const int rl = g + u * RPW + seg;
-        ok[u] = rl < nrows;
-        const int src_lane = ok[u] ? rl : 0;
+        const int src_lane = rl < nrows ? rl : 0;
         const int64_t rid = __shfl((long long)my_row, src_lane, kWave);
+        ok[u] = rl < nrows && rid >= 0;
         yv[u] = __shfl(my_y, src_lane, kWave);
-        const float4* rp = reinterpret_cast<const float4*>(X + rid * ldx);
+        const float4* rp = reinterpret_cast<const float4*>(X + (ok[u] ? rid : 0) * ldx);
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
           const int c4 = q + LPR * k;
@@ -231,7 +235,7 @@ struct PotArgs {
 
 template <int LPR, int CPL>
 __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
-  __shared__ float lds_cols[kWaves * kMaxD];
+  __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
   __shared__ int flag;
   float4 th[CPL];
@@ -242,7 +246,7 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
   float lik_acc = 0.0f;
   int r0, r1;
   wave_rows(a.n, gridDim.x, blockIdx.x, r0, r1);
-  RowSource src{a.idx, 0, Key{0, 0}, Key{0, 0}, 0, 0, 0};
+  RowSource src{a.idx, 0, Key{0, 0}, Key{0, 0}, 0, 0, 0, a.idx ? a.N : (int64_t)a.n};
   accumulate_rows<LPR, CPL>(a.X, a.ldx, a.y, a.D4, src, r0, r1, th, a.L, acc, lik_acc,
                             a.lik_scratch);
   const int stride = 4 * a.D4 + 1;
@@ -381,7 +385,7 @@ template <int LPR, int CPL>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_chain& ch,
                                            int bchain, float* lds_cols, float* lds_small,
                                            int* flag) {
-  const int it = min(a.iter[0], a.table_len - 1);
+  const int it = max(0, min(a.iter[0], a.table_len - 1));
   const float eps = a.eps_table[it];
   const int D = ch.D;
   const int D4 = (D + 3) >> 2;
@@ -440,6 +444,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   src.n = (uint32_t)ch.n;
   src.span = (uint32_t)ch.N;
   src.mult = randint_mult(src.span);
+  src.rows = ch.N;
 
   float4 acc[CPL];
 #pragma unroll
@@ -550,12 +555,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
 
 template <int DCLASS>
 __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
-  __shared__ float lds_cols[kWaves * kMaxD];
+  __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
   __shared__ int flag;
   const int chain = a.block_chain[blockIdx.x];
+  if (chain < 0 || chain >= a.nchains) return;
   const sgmc_glm_chain& ch = a.chains[chain];
   const int bchain = blockIdx.x - ch.block_begin;
+  if (bchain < 0 || bchain >= ch.nblocks) return;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   switch (dc) {
     case 0: fused_body<4, 1>(a, ch, bchain, lds_cols, lds_small, &flag); break;

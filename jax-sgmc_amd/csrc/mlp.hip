@@ -1,0 +1,647 @@
+// Bayesian MLP classifier likelihood on MFMA (bf16 operands, fp32 accumulation), with
+// a hand-written backward pass and the SGLD / rms_prop update fused into the weight-
+// gradient GEMM epilogue. Model (the likelihood a user writes in the reference as
+// likelihood(sample, obs), potential.py:128-137):
+//   h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), logits = h2 W3 + b3,
+//   lik = log_softmax(logits)[label]
+// Parameters are one float32 vector per chain in tree_flatten order of
+// {"W1","W2","W3","b1","b2","b3"} (sorted keys): W1 [D_in][H1], W2 [H1][H2],
+// W3 [H2][10], b1, b2, b3. A bf16 shadow of the parameters (same layout) feeds the
+// GEMMs and is rewritten by every update.
+//
+// One step for C chains (grid z = chain), potential U = (-N mean(lik) - log p)/T with
+// a Gaussian prior, gradient by back-propagation:
+//   gather   Xb = bf16(X[idx]) (index draw in-kernel, data/numpy_loader.py:132-134)
+//   G1       H1 = relu(Xb W1 + b1)
+//   G2       H2 = relu(H1 W2 + b2)
+//   head     logits, softmax, lik, dL = c (onehot - p), dA2 = (dL W3^T) * (H2 > 0)
+//   G3       dA1 = (dA2 W2^T) * (H1 > 0)
+//   G4       gW2 = H1^T dA2  -> update W2 in the epilogue
+//   G5       gW1 = Xb^T dA1  -> update W1 in the epilogue
+//   small    W3, b1, b2, b3 gradients and updates; U; key and data-key advance
+// c = -N / (n T) carries the potential's scaling into the backward pass.
+//
+// GEMM tiling (gfx950, v_mfma_f32_32x32x16_bf16): a block computes 64 rows x 128
+// columns; its 64 rows are two bands of 32 rows M/2 apart, so each lane's two
+// accumulators hold the element pairs (i, i + size/2) that one Threefry block of
+// random_bits serves (integrator.py:119-135 noise). Four wavefronts own 32 columns
+// each. Operands are staged through LDS with k contiguous (padded rows); a source that
+// is m- or n-contiguous is transposed while being written to LDS.
+#include <math.h>
+
+#include "sgmc_common.h"
+#include "sgmc_device.h"
+
+namespace sgmc {
+namespace mlp {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int GT = 256;
+constexpr int BM = 64;
+constexpr int BN = 128;
+constexpr int BK = 32;
+constexpr int LDK = BK + 8;  // padded LDS pitch in bf16 elements
+constexpr int NCLS = 10;
+constexpr int NLEAF = 6;     // W1, W2, W3, b1, b2, b3
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+enum Epi { EPI_BIAS_RELU = 0, EPI_MASK = 1, EPI_UPDATE = 2, EPI_GRAD = 3 };
+
+// Per-chain state and step parameters shared by the kernels.
+struct Step {
+  const float* X;          // dataset [N][ldx] float32
+  const int32_t* labels;   // [N]
+  int64_t N, ldx;
+  int D_in, H1, H2, n, C;
+  float* theta;            // [C][P]
+  float* v;                // [C][P] or null (rms_prop)
+  uint16_t* shadow;        // [C][P] bf16
+  int64_t P;
+  int64_t off[NLEAF];      // leaf offsets
+  int64_t size[NLEAF];     // leaf sizes
+  uint32_t* key;           // [C][2]
+  uint32_t* dkey;          // [C][2]
+  uint16_t* Xb;            // [C][n][D_in]
+  int32_t* lab;            // [C][n]
+  uint16_t* h1;            // [C][n][H1]
+  uint16_t* h2;            // [C][n][H2]
+  uint16_t* da2;           // [C][n][H2]
+  uint16_t* da1;           // [C][n][H1]
+  float* dL;               // [C][n][10]
+  float* lik;              // [C][n]
+  float* prior_part;       // [C][nparts]
+  int nparts;
+  float* U;                // [C]
+  float* grad;             // [C][P] (gradient-only mode) or null
+  float* samples;          // [C][n_slots][P] or null
+  float* samples_U;        // [C][n_slots] or null
+  int n_slots;
+  const float* eps_table;
+  const int32_t* save_slot;
+  int32_t* iter;
+  unsigned* counters;      // [C] arrival counters of the small kernel
+  unsigned* done;
+  int table_len;
+  int rms;
+  int update;              // 0: gradient only (autograd backward)
+  float alpha, lmbd, temperature, noise_temperature;
+  float prior_inv_var, prior_log_norm;
+  int has_prior;
+};
+
+__device__ __forceinline__ int cur_iter(const Step& s) {
+  return max(0, min(s.iter[0], s.table_len - 1));
+}
+
+// SGLD / rms_prop update of one parameter (integrator.py:882-912, adaption.py:254-291).
+__device__ __forceinline__ void sgld_param(const Step& s, float* th, float* v, uint16_t* sh,
+                                           int64_t i, float gsum, float xi, float eps,
+                                           float noise_scale, float& lp) {
+  const float w = th[i];
+  float g = gsum;
+  if (s.has_prior) {
+    g = g + w * s.prior_inv_var;
+    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
+  }
+  g = g / s.temperature;
+  const float sg = (-eps) * g;
+  const float sn = noise_scale * xi;
+  float wn;
+  if (s.rms) {
+    const float vn = s.alpha * v[i] + (1.0f - s.alpha) * (g * g);
+    v[i] = vn;
+    const float G = 1.0f / (s.lmbd + sqrtf(vn));
+    wn = w + ((0.0f + G * sg) + sqrtf(G) * sn);
+  } else {
+    wn = w + (sg + sn);
+  }
+  th[i] = wn;
+  sh[i] = f2bf(wn);
+}
+
+// Gradient-only variant: grad = (gsum + prior) / T, no update.
+__device__ __forceinline__ void grad_param(const Step& s, const float* th, float* gout,
+                                           int64_t i, float gsum, float& lp) {
+  const float w = th[i];
+  float g = gsum;
+  if (s.has_prior) {
+    g = g + w * s.prior_inv_var;
+    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
+  }
+  gout[i] = g / s.temperature;
+}
+
+__device__ __forceinline__ Key leaf_key(const Step& s, int chain, int leaf) {
+  Key next, sub;
+  split2(load_key(s.key + 2 * chain), next, sub);
+  return split_key(sub, NLEAF, (uint32_t)leaf);
+}
+
+// ------------------------------------------------------------------ gather ----
+// Xb[c][j] = bf16(X[idx_j]), lab[c][j] = labels[idx_j]; one wavefront per row.
+__global__ void __launch_bounds__(GT) gather_kernel(Step s) {
+  const int c = blockIdx.y;
+  const int j = blockIdx.x * (GT / kWave) + threadIdx.x / kWave;
+  if (j >= s.n) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  Key next, sub, k1, k2;
+  split2(load_key(s.dkey + 2 * c), next, sub);
+  split2(sub, k1, k2);
+  const uint32_t span = (uint32_t)s.N;
+  int64_t r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, randint_mult(span));
+  if (r < 0 || r >= s.N) r = 0;
+  const float* src = s.X + r * s.ldx;
+  uint16_t* dst = s.Xb + ((int64_t)c * s.n + j) * s.D_in;
+  for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
+    const float4 x = *reinterpret_cast<const float4*>(src + f);
+    const uint32_t lo = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
+    const uint32_t hi = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
+    *reinterpret_cast<uint2*>(dst + f) = make_uint2(lo, hi);
+  }
+  if (lane == 0) s.lab[(int64_t)c * s.n + j] = s.labels[r];
+}
+
+// ------------------------------------------------------------------ GEMM ------
+struct Gemm {
+  const uint16_t* A;
+  int64_t lda, sA;   // sA: chain stride in elements
+  const uint16_t* B;
+  int64_t ldb, sB;
+  int M, N, K;
+  uint16_t* out;     // EPI_BIAS_RELU / EPI_MASK
+  int64_t ldo, sO;
+  const float* bias; // chain-strided by sBias
+  int64_t sBias;
+  const uint16_t* mask;
+  int64_t ldm, sM;
+  int leaf;          // EPI_UPDATE / EPI_GRAD: parameter leaf of the [M][N] weight
+  int part0;         // first prior-partial slot of this GEMM
+};
+
+template <bool A_K, bool B_K>
+__device__ __forceinline__ void load_tiles(const Gemm& g, int c, int bm, int bn, int k0,
+                                           uint4 (&ra)[1], uint4 (&rb)[2]) {
+  const int t = threadIdx.x;
+  const uint16_t* A = g.A + (int64_t)c * g.sA;
+  const uint16_t* B = g.B + (int64_t)c * g.sB;
+  const int half = g.M / 2;
+  if (A_K) {
+    const int r = t >> 2, k8 = (t & 3) * 8;
+    const int m = (r < 32) ? bm * 32 + r : half + bm * 32 + (r - 32);
+    ra[0] = *reinterpret_cast<const uint4*>(A + (int64_t)m * g.lda + k0 + k8);
+  } else {
+    const int k = t >> 3, r8 = (t & 7) * 8;
+    const int m = (r8 < 32) ? bm * 32 + r8 : half + bm * 32 + (r8 - 32);
+    ra[0] = *reinterpret_cast<const uint4*>(A + (int64_t)(k0 + k) * g.lda + m);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int u = t + h * GT;
+    if (B_K) {
+      const int col = u >> 2, k8 = (u & 3) * 8;
+      rb[h] = *reinterpret_cast<const uint4*>(B + (int64_t)(bn * BN + col) * g.ldb + k0 + k8);
+    } else {
+      const int k = u >> 4, c8 = (u & 15) * 8;
+      rb[h] = *reinterpret_cast<const uint4*>(B + (int64_t)(k0 + k) * g.ldb + bn * BN + c8);
+    }
+  }
+}
+
+template <bool A_K, bool B_K>
+__device__ __forceinline__ void store_tiles(uint16_t* As, uint16_t* Bs, const uint4 (&ra)[1],
+                                            const uint4 (&rb)[2]) {
+  const int t = threadIdx.x;
+  if (A_K) {
+    const int r = t >> 2, k8 = (t & 3) * 8;
+    *reinterpret_cast<uint4*>(As + r * LDK + k8) = ra[0];
+  } else {
+    const int k = t >> 3, r8 = (t & 7) * 8;
+    const uint16_t* e = reinterpret_cast<const uint16_t*>(&ra[0]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[(r8 + i) * LDK + k] = e[i];
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int u = t + h * GT;
+    if (B_K) {
+      const int col = u >> 2, k8 = (u & 3) * 8;
+      *reinterpret_cast<uint4*>(Bs + col * LDK + k8) = rb[h];
+    } else {
+      const int k = u >> 4, c8 = (u & 15) * 8;
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&rb[h]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[(c8 + i) * LDK + k] = e[i];
+    }
+  }
+}
+
+template <bool A_K, bool B_K, int EPI>
+__global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDK];
+  __shared__ float red[GT / kWave];
+  const int bm = blockIdx.x, bn = blockIdx.y, c = blockIdx.z;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  f32x16 acc0, acc1;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    acc0[j] = 0.f;
+    acc1[j] = 0.f;
+  }
+  uint4 ra[1], rb[2];
+  load_tiles<A_K, B_K>(g, c, bm, bn, 0, ra, rb);
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    __syncthreads();
+    store_tiles<A_K, B_K>(As, Bs, ra, rb);
+    __syncthreads();
+    if (k0 + BK < g.K) load_tiles<A_K, B_K>(g, c, bm, bn, k0 + BK, ra, rb);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int ko = kk + 8 * (lane >> 5);
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As + (lane & 31) * LDK + ko);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As + (32 + (lane & 31)) * LDK + ko);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + (w * 32 + (lane & 31)) * LDK + ko);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc1, 0, 0, 0);
+    }
+  }
+
+  const int half = g.M / 2;
+  const int n = bn * BN + w * 32 + (lane & 31);
+  if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
+    uint16_t* out = g.out + (int64_t)c * g.sO;
+    const float bias = (EPI == EPI_BIAS_RELU) ? g.bias[(int64_t)c * g.sBias + n] : 0.f;
+#pragma unroll
+    for (int band = 0; band < 2; ++band) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int m = (band == 0 ? bm * 32 : half + bm * 32) + r;
+        const float a = band == 0 ? acc0[j] : acc1[j];
+        float vv;
+        if (EPI == EPI_BIAS_RELU) {
+          vv = fmaxf(a + bias, 0.0f);
+        } else {
+          const uint16_t mk = g.mask[(int64_t)c * g.sM + (int64_t)m * g.ldm + n];
+          vv = (mk != 0 && !(mk & 0x8000u)) ? a : 0.0f;
+        }
+        out[(int64_t)m * g.ldo + n] = f2bf(vv);
+      }
+    }
+  } else {
+    // Parameter epilogue: element (m, n) of the [M][N] leaf at flat i = m N + n, its
+    // noise pair i + M N / 2 sits in acc1 at the same lane and register.
+    const int it = cur_iter(s);
+    const float eps = s.eps_table[it];
+    const float noise_scale = sqrtf((2.0f * s.noise_temperature) * eps);
+    const int64_t base = (int64_t)c * s.P + s.off[g.leaf];
+    float* th = s.theta + base;
+    float* vv = s.v ? s.v + base : nullptr;
+    uint16_t* sh = s.shadow + base;
+    const uint32_t S = (uint32_t)((int64_t)g.M * g.N);
+    const Key lk = leaf_key(s, c, g.leaf);
+    float lp = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+      const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+      const int64_t i1 = i0 + (int64_t)half * g.N;
+      if (EPI == EPI_UPDATE) {
+        uint32_t w0, w1;
+        bits_pair(lk, S, (uint32_t)i0, w0, w1);
+        sgld_param(s, th, vv, sh, i0, acc0[j], bits_to_normal(w0), eps, noise_scale, lp);
+        sgld_param(s, th, vv, sh, i1, acc1[j], bits_to_normal(w1), eps, noise_scale, lp);
+      } else {
+        float* go = s.grad + base;
+        grad_param(s, th, go, i0, acc0[j], lp);
+        grad_param(s, th, go, i1, acc1[j], lp);
+      }
+    }
+    const float tot = block_sum<GT>(lp, red);
+    if (threadIdx.x == 0)
+      s.prior_part[(int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn] = tot;
+  }
+}
+
+// ------------------------------------------------------------------ head ------
+// One wavefront per row: logits = h2 W3 + b3; p = softmax; lik = logits[label] - lse;
+// dL = c (onehot - p); dA2 = (dL W3^T) * (h2 > 0).
+__global__ void __launch_bounds__(GT) head_kernel(Step s) {
+  const int c = blockIdx.y;
+  const int j = blockIdx.x * (GT / kWave) + threadIdx.x / kWave;
+  if (j >= s.n) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const float* th = s.theta + (int64_t)c * s.P;
+  const float* W3 = th + s.off[2];
+  const float* b3 = th + s.off[5];
+  const uint16_t* h2 = s.h2 + ((int64_t)c * s.n + j) * s.H2;
+  float z[NCLS];
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) z[k] = 0.f;
+  for (int a = lane; a < s.H2; a += kWave) {
+    const float h = bf2f(h2[a]);
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) z[k] += h * W3[(int64_t)a * NCLS + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) z[k] = wave_sum(z[k]) + b3[k];
+  float mx = z[0];
+#pragma unroll
+  for (int k = 1; k < NCLS; ++k) mx = fmaxf(mx, z[k]);
+  float se = 0.f;
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) se += expf(z[k] - mx);
+  const float lse = mx + logf(se);
+  int lab = s.lab[(int64_t)c * s.n + j];
+  lab = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
+  const float cscale = ((float)(-s.N) / (float)s.n) / s.temperature;
+  float dl[NCLS];
+  float zl = 0.f;
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) {
+    const float p = expf(z[k] - lse);
+    dl[k] = cscale * ((k == lab ? 1.0f : 0.0f) - p);
+    if (k == lab) zl = z[k];
+  }
+  if (lane == 0) {
+    s.lik[(int64_t)c * s.n + j] = zl - lse;
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) s.dL[((int64_t)c * s.n + j) * NCLS + k] = dl[k];
+  }
+  uint16_t* da2 = s.da2 + ((int64_t)c * s.n + j) * s.H2;
+  for (int a = lane; a < s.H2; a += kWave) {
+    float d = 0.f;
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) d += dl[k] * W3[(int64_t)a * NCLS + k];
+    const uint16_t hb = h2[a];
+    da2[a] = f2bf((hb != 0 && !(hb & 0x8000u)) ? d : 0.0f);
+  }
+}
+
+// ----------------------------------------------------------- small params -----
+// W3, b1, b2, b3: gradient by a loop over the n rows, update, prior partial; the last
+// block of a chain forms U and advances the chain's keys (and the iteration counter
+// once every chain is done).
+__global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
+  __shared__ float red[GT / kWave];
+  __shared__ int flag;
+  const int c = blockIdx.y;
+  const int64_t nW3 = (int64_t)s.H2 * NCLS;
+  const int64_t total = nW3 + s.H1 + s.H2 + NCLS;
+  const int64_t p = (int64_t)blockIdx.x * GT + threadIdx.x;
+  const int it = cur_iter(s);
+  const float eps = s.eps_table[it];
+  const float noise_scale = sqrtf((2.0f * s.noise_temperature) * eps);
+  float lp = 0.f;
+  if (p < total) {
+    int leaf;
+    int64_t i;
+    float gsum = 0.f;
+    const int64_t rowb = (int64_t)c * s.n;
+    if (p < nW3) {
+      leaf = 2;
+      i = p;
+      const int a = (int)(p / NCLS), k = (int)(p % NCLS);
+      for (int r = 0; r < s.n; ++r)
+        gsum += bf2f(s.h2[(rowb + r) * s.H2 + a]) * s.dL[(rowb + r) * NCLS + k];
+    } else if (p < nW3 + s.H1) {
+      leaf = 3;
+      i = p - nW3;
+      for (int r = 0; r < s.n; ++r) gsum += bf2f(s.da1[(rowb + r) * s.H1 + i]);
+    } else if (p < nW3 + s.H1 + s.H2) {
+      leaf = 4;
+      i = p - nW3 - s.H1;
+      for (int r = 0; r < s.n; ++r) gsum += bf2f(s.da2[(rowb + r) * s.H2 + i]);
+    } else {
+      leaf = 5;
+      i = p - nW3 - s.H1 - s.H2;
+      for (int r = 0; r < s.n; ++r) gsum += s.dL[(rowb + r) * NCLS + i];
+    }
+    const int64_t base = (int64_t)c * s.P + s.off[leaf];
+    if (s.update) {
+      const Key lk = leaf_key(s, c, leaf);
+      const float xi = bits_to_normal(bits_word(lk, (uint32_t)s.size[leaf], (uint32_t)i));
+      sgld_param(s, s.theta + base, s.v ? s.v + base : nullptr, s.shadow + base, i, gsum, xi,
+                 eps, noise_scale, lp);
+    } else {
+      grad_param(s, s.theta + base, s.grad + base, i, gsum, lp);
+    }
+  }
+  const float tot = block_sum<GT>(lp, red);
+  if (threadIdx.x == 0) s.prior_part[(int64_t)c * s.nparts + part0 + blockIdx.x] = tot;
+  if (!arrive_last(s.counters + c, gridDim.x, &flag)) return;
+  float lsum = 0.f;
+  for (int r = threadIdx.x; r < s.n; r += GT) lsum += s.lik[(int64_t)c * s.n + r];
+  lsum = block_sum<GT>(lsum, red);
+  float prior = 0.f;
+  for (int q = threadIdx.x; q < s.nparts; q += GT) prior += s.prior_part[(int64_t)c * s.nparts + q];
+  prior = block_sum<GT>(prior, red);
+  if (threadIdx.x == 0) {
+    const float mean = lsum / (float)s.n;
+    const float Uc = ((float)(-s.N) * mean - prior) / s.temperature;
+    s.U[c] = Uc;
+    if (s.update) {
+      const int slot = s.save_slot ? s.save_slot[it] : -1;
+      if (slot >= 0 && slot < s.n_slots && s.samples_U) s.samples_U[(int64_t)c * s.n_slots + slot] = Uc;
+      Key next, sub;
+      split2(load_key(s.key + 2 * c), next, sub);
+      store_key(s.key + 2 * c, next);
+      split2(load_key(s.dkey + 2 * c), next, sub);
+      store_key(s.dkey + 2 * c, next);
+      __threadfence();
+      const unsigned prev = atomicAdd(s.done, 1u);
+      if (prev == (unsigned)s.C - 1) {
+        atomicExch(s.done, 0u);
+        atomicAdd(s.iter, 1);
+      }
+    }
+  }
+}
+
+// Copies theta into the sample buffer at kept iterations (after the step).
+__global__ void __launch_bounds__(GT) save_kernel(Step s, int it_offset) {
+  const int c = blockIdx.y;
+  // the iteration that just completed is iter - 1
+  const int it = max(0, min(s.iter[0] - it_offset, s.table_len - 1));
+  const int slot = s.save_slot ? s.save_slot[it] : -1;
+  if (slot < 0 || slot >= s.n_slots || !s.samples) return;
+  for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.P; i += (int64_t)gridDim.x * GT)
+    s.samples[((int64_t)c * s.n_slots + slot) * s.P + i] = s.theta[(int64_t)c * s.P + i];
+}
+
+// theta -> bf16 shadow.
+__global__ void __launch_bounds__(GT) shadow_kernel(const float* th, uint16_t* sh, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < n; i += (int64_t)gridDim.x * GT)
+    sh[i] = f2bf(th[i]);
+}
+
+}  // namespace mlp
+}  // namespace sgmc
+
+using namespace sgmc;
+using namespace sgmc::mlp;
+
+namespace {
+
+struct Layout {
+  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, total;
+  int nparts, g4_blocks, g5_blocks, small_blocks;
+};
+
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+Layout layout_of(int C, int n, int D_in, int H1, int H2) {
+  Layout L;
+  int64_t o = 0;
+  L.xb = o; o = align256(o + (int64_t)C * n * D_in * 2);
+  L.lab = o; o = align256(o + (int64_t)C * n * 4);
+  L.h1 = o; o = align256(o + (int64_t)C * n * H1 * 2);
+  L.h2 = o; o = align256(o + (int64_t)C * n * H2 * 2);
+  L.da2 = o; o = align256(o + (int64_t)C * n * H2 * 2);
+  L.da1 = o; o = align256(o + (int64_t)C * n * H1 * 2);
+  L.dl = o; o = align256(o + (int64_t)C * n * NCLS * 4);
+  L.lik = o; o = align256(o + (int64_t)C * n * 4);
+  L.g4_blocks = (H1 / BM) * (H2 / BN);
+  L.g5_blocks = (D_in / BM) * (H1 / BN);
+  const int64_t small_total = (int64_t)H2 * NCLS + H1 + H2 + NCLS;
+  L.small_blocks = (int)((small_total + GT - 1) / GT);
+  L.nparts = L.g4_blocks + L.g5_blocks + L.small_blocks;
+  L.prior = o; o = align256(o + (int64_t)C * L.nparts * 4);
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t sgmc_mlp_param_count(int D_in, int H1, int H2) {
+  return (int64_t)D_in * H1 + (int64_t)H1 * H2 + (int64_t)H2 * NCLS + H1 + H2 + NCLS;
+}
+
+int64_t sgmc_mlp_workspace_bytes(int C, int n, int D_in, int H1, int H2) {
+  return layout_of(C, n, D_in, H1, H2).total;
+}
+
+int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* stream) {
+  SGMC_REQUIRE(theta && shadow && count >= 0, SGMC_E_ARG, "sgmc_mlp_shadow: bad argument");
+  if (count == 0) return SGMC_OK;
+  const int64_t blocks = std::min<int64_t>((count + GT - 1) / GT, 65535);
+  hipLaunchKernelGGL(shadow_kernel, dim3((unsigned)blocks), dim3(GT), 0, as_stream(stream),
+                     theta, shadow, count);
+  return check_launch("sgmc_mlp_shadow");
+}
+
+int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
+  SGMC_REQUIRE(st, SGMC_E_ARG, "sgmc_mlp_step: null state");
+  const sgmc_mlp_state& q = *st;
+  SGMC_REQUIRE(q.X && q.labels && q.theta && q.shadow && q.key && q.dkey && q.work && q.U &&
+                   q.eps_table && q.iter && q.counters && q.done,
+               SGMC_E_ARG, "sgmc_mlp_step: null pointer in state");
+  SGMC_REQUIRE(update || q.grad, SGMC_E_ARG, "sgmc_mlp_step: gradient mode needs grad");
+  SGMC_REQUIRE(q.C >= 1 && q.n >= 1 && q.N >= 1 && q.N <= 2147483647LL, SGMC_E_ARG,
+               "sgmc_mlp_step: bad C/n/N");
+  SGMC_REQUIRE(q.n % BM == 0 && q.D_in % BM == 0 && q.H1 % BM == 0, SGMC_E_ARG,
+               "sgmc_mlp_step: n, D_in and H1 must be multiples of %d", BM);
+  SGMC_REQUIRE(q.H1 % BN == 0 && q.H2 % BN == 0, SGMC_E_ARG,
+               "sgmc_mlp_step: H1 and H2 must be multiples of %d", BN);
+  SGMC_REQUIRE(q.ldx % 4 == 0 && q.ldx >= q.D_in && ((uintptr_t)q.X & 15) == 0, SGMC_E_ARG,
+               "sgmc_mlp_step: X rows must be 16-byte aligned with ldx %% 4 == 0");
+  SGMC_REQUIRE(q.P == sgmc_mlp_param_count(q.D_in, q.H1, q.H2), SGMC_E_ARG,
+               "sgmc_mlp_step: P does not match the layer sizes");
+  SGMC_REQUIRE(q.table_len >= 1 && q.temperature != 0.0f, SGMC_E_ARG,
+               "sgmc_mlp_step: bad schedule");
+  SGMC_REQUIRE((int64_t)q.D_in * q.H1 < 4294967295LL, SGMC_E_ARG, "sgmc_mlp_step: W1 too large");
+  const Layout L = layout_of(q.C, q.n, q.D_in, q.H1, q.H2);
+  SGMC_REQUIRE(q.work_bytes >= L.total, SGMC_E_ARG, "sgmc_mlp_step: workspace too small");
+  uint8_t* w = static_cast<uint8_t*>(q.work);
+  Step s;
+  s.X = q.X; s.labels = q.labels; s.N = q.N; s.ldx = q.ldx;
+  s.D_in = q.D_in; s.H1 = q.H1; s.H2 = q.H2; s.n = q.n; s.C = q.C;
+  s.theta = q.theta; s.v = q.rms ? q.v : nullptr; s.shadow = q.shadow; s.P = q.P;
+  const int64_t sizes[NLEAF] = {(int64_t)q.D_in * q.H1, (int64_t)q.H1 * q.H2,
+                                (int64_t)q.H2 * NCLS, q.H1, q.H2, NCLS};
+  int64_t o = 0;
+  for (int l = 0; l < NLEAF; ++l) {
+    s.off[l] = o;
+    s.size[l] = sizes[l];
+    o += sizes[l];
+  }
+  s.key = q.key; s.dkey = q.dkey;
+  s.Xb = reinterpret_cast<uint16_t*>(w + L.xb);
+  s.lab = reinterpret_cast<int32_t*>(w + L.lab);
+  s.h1 = reinterpret_cast<uint16_t*>(w + L.h1);
+  s.h2 = reinterpret_cast<uint16_t*>(w + L.h2);
+  s.da2 = reinterpret_cast<uint16_t*>(w + L.da2);
+  s.da1 = reinterpret_cast<uint16_t*>(w + L.da1);
+  s.dL = reinterpret_cast<float*>(w + L.dl);
+  s.lik = reinterpret_cast<float*>(w + L.lik);
+  s.prior_part = reinterpret_cast<float*>(w + L.prior);
+  s.nparts = L.nparts;
+  s.U = q.U; s.grad = q.grad;
+  s.samples = q.samples; s.samples_U = q.samples_U; s.n_slots = q.n_slots;
+  s.eps_table = q.eps_table; s.save_slot = q.save_slot; s.iter = q.iter;
+  s.counters = q.counters; s.done = q.done; s.table_len = q.table_len;
+  s.rms = q.rms && q.v ? 1 : 0;
+  s.update = update ? 1 : 0;
+  s.alpha = q.alpha; s.lmbd = q.lmbd; s.temperature = q.temperature;
+  s.noise_temperature = q.noise_temperature;
+  s.has_prior = q.has_prior;
+  const float ps2 = q.prior_scale * q.prior_scale;
+  s.prior_inv_var = q.has_prior ? 1.0f / ps2 : 0.0f;
+  s.prior_log_norm = q.has_prior ? logf(2.0f * (float)M_PI * ps2) : 0.0f;
+
+  hipStream_t st_ = as_stream(stream);
+  const int C = q.C, n = q.n;
+  const dim3 rows_grid((n + 3) / 4, C);
+  hipLaunchKernelGGL(gather_kernel, rows_grid, dim3(GT), 0, st_, s);
+  uint16_t* sh = q.shadow;
+  Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, sh + s.off[0], q.H1, q.P, n, q.H1, q.D_in,
+          s.h1, q.H1, (int64_t)n * q.H1, q.theta + s.off[3], q.P, nullptr, 0, 0, 0, 0};
+  hipLaunchKernelGGL((gemm_kernel<true, false, EPI_BIAS_RELU>), dim3(n / BM, q.H1 / BN, C),
+                     dim3(GT), 0, st_, g1, s);
+  Gemm g2{s.h1, q.H1, (int64_t)n * q.H1, sh + s.off[1], q.H2, q.P, n, q.H2, q.H1,
+          s.h2, q.H2, (int64_t)n * q.H2, q.theta + s.off[4], q.P, nullptr, 0, 0, 0, 0};
+  hipLaunchKernelGGL((gemm_kernel<true, false, EPI_BIAS_RELU>), dim3(n / BM, q.H2 / BN, C),
+                     dim3(GT), 0, st_, g2, s);
+  hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
+  // dA1 = (dA2 W2^T) * (H1 > 0): B[k=b][n=a] = W2[a][b], k-contiguous rows of W2.
+  Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, sh + s.off[1], q.H2, q.P, n, q.H1, q.H2,
+          s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};
+  hipLaunchKernelGGL((gemm_kernel<true, true, EPI_MASK>), dim3(n / BM, q.H1 / BN, C), dim3(GT),
+                     0, st_, g3, s);
+  // gW2 = H1^T dA2: A[m=a][k=i] = H1[i][a] (m-contiguous), B[k=i][n=b] = dA2[i][b].
+  Gemm g4{s.h1, q.H1, (int64_t)n * q.H1, s.da2, q.H2, (int64_t)n * q.H2, q.H1, q.H2, n,
+          nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 1, 0};
+  // gW1 = Xb^T dA1.
+  Gemm g5{s.Xb, q.D_in, (int64_t)n * q.D_in, s.da1, q.H1, (int64_t)n * q.H1, q.D_in, q.H1, n,
+          nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, L.g4_blocks};
+  if (update) {
+    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_UPDATE>), dim3(q.H1 / BM, q.H2 / BN, C),
+                       dim3(GT), 0, st_, g4, s);
+    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_UPDATE>), dim3(q.D_in / BM, q.H1 / BN, C),
+                       dim3(GT), 0, st_, g5, s);
+  } else {
+    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_GRAD>), dim3(q.H1 / BM, q.H2 / BN, C),
+                       dim3(GT), 0, st_, g4, s);
+    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_GRAD>), dim3(q.D_in / BM, q.H1 / BN, C),
+                       dim3(GT), 0, st_, g5, s);
+  }
+  hipLaunchKernelGGL(small_kernel, dim3(L.small_blocks, C), dim3(GT), 0, st_, s,
+                     L.g4_blocks + L.g5_blocks);
+  if (update && q.samples && q.save_slot) {
+    hipLaunchKernelGGL(save_kernel, dim3(256, C), dim3(GT), 0, st_, s, 1);
+  }
+  return check_launch("sgmc_mlp_step");
+}
+
+}  // extern "C"

@@ -77,12 +77,24 @@ class FusedGLM:
     self.sqrt_mass = sqrt_mass
     Ds = [int(c["D"]) for c in chains]
     for c in chains:
-      if not 1 <= int(c["D"]) <= 1024:
+      X, y = c["X"], c["y"]
+      D, n = int(c["D"]), int(c["n"])
+      N = int(c.get("N", X.shape[0]))
+      if not 1 <= D <= 1024:
         raise ValueError("fused GLM path supports 1 <= D <= 1024")
-      if c["X"].stride(0) % 4 or c["X"].shape[1] < _align4(int(c["D"])):
-        raise ValueError("X rows must be padded to a multiple of 4 floats")
-      if int(c["n"]) < 1 or int(c["n"]) > c["X"].shape[0] and False:
-        raise ValueError("bad minibatch size")
+      if not (X.is_cuda and y.is_cuda and X.dtype == torch.float32 and
+              y.dtype == torch.float32 and X.dim() == 2 and X.stride(1) == 1):
+        raise ValueError("X must be a float32 [N, ldx] device tensor, y float32 [N]")
+      if X.stride(0) % 4 or X.stride(0) < _align4(D) or X.data_ptr() % 16:
+        raise ValueError("X rows must be 16-byte aligned and padded to a multiple of 4 floats")
+      if not 1 <= N <= min(X.shape[0], y.numel(), 2 ** 31 - 1):
+        raise ValueError("observation count outside the dataset")
+      if not y.is_contiguous():
+        raise ValueError("y must be contiguous")
+      if n < 1:
+        raise ValueError("minibatch size must be positive")
+      if np.asarray(c["theta0"]).size != D:
+        raise ValueError("theta0 must have D entries")
     dcl = {_lib.query("sgmc_glm_dclass", d) for d in Ds}
     self.uniform_dclass = dcl.pop() if len(dcl) == 1 else -1
     total_bytes = sum(int(c["n"]) * 4 * _align4(int(c["D"])) for c in chains)

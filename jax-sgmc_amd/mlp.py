@@ -1,0 +1,214 @@
+"""Bayesian MLP classifier on MFMA: chain engine and autograd wrapper.
+
+Model: x -> relu(x W1 + b1) -> relu(. W2 + b2) -> . W3 + b3 -> 10-class softmax
+cross-entropy likelihood, Gaussian prior on every parameter. Parameters are a dict
+{"W1","W2","W3","b1","b2","b3"} whose tree_flatten (sorted-key) order is W1, W2, W3,
+b1, b2, b3; one flat float32 vector per chain in that order.
+
+``FusedMLP`` advances C chains per call of ``sgmc_mlp_step`` (index draw, gather, bf16
+MFMA forward and hand-written backward, update fused into the weight-gradient GEMMs);
+``MLPPotential`` exposes the same kernels as a torch.autograd.Function (forward: U,
+backward: the kernel's gradient).
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import tree as T
+
+NCLS = 10
+
+
+MlpState = _lib.MlpState
+
+
+def leaf_shapes(D_in, H1, H2):
+  return {"W1": (D_in, H1), "W2": (H1, H2), "W3": (H2, NCLS), "b1": (H1,), "b2": (H2,),
+          "b3": (NCLS,)}
+
+
+def param_count(D_in, H1, H2):
+  return D_in * H1 + H1 * H2 + H2 * NCLS + H1 + H2 + NCLS
+
+
+def init_params(key_seed, D_in, H1, H2, scale=None):
+  """Glorot-like random init as a flat vector (numpy, float32)."""
+  rng = np.random.default_rng(key_seed)
+  parts = []
+  for name in ("W1", "W2", "W3", "b1", "b2", "b3"):
+    shp = leaf_shapes(D_in, H1, H2)[name]
+    if name.startswith("W"):
+      s = scale if scale is not None else math.sqrt(2.0 / shp[0])
+      parts.append((s * rng.standard_normal(shp)).astype(np.float32).ravel())
+    else:
+      parts.append(np.zeros(shp, np.float32).ravel())
+  return np.concatenate(parts)
+
+
+class MLPSoftmax:
+  """likelihood(sample, obs) for one observation, torch (generic path)."""
+
+  def __init__(self, x="x", y="y"):
+    self.x, self.y = x, y
+
+  def __call__(self, sample, obs):
+    h = torch.relu(obs[self.x] @ sample["W1"] + sample["b1"])
+    h = torch.relu(h @ sample["W2"] + sample["b2"])
+    z = h @ sample["W3"] + sample["b3"]
+    return torch.log_softmax(z, -1)[obs[self.y].long()]
+
+
+class FusedMLP:
+  """C chains of SGLD (optionally rms_prop) on the MLP likelihood."""
+
+  def __init__(self, X, labels, dims, n, theta0, eps_table, keys=None, dkeys=None, rms=False,
+               alpha=0.9, lmbd=1e-5, prior_scale=1.0, has_prior=True, temperature=1.0,
+               noise_temperature=1.0, save_slots=None, device="cuda"):
+    _lib.require_gpu()
+    _lib.load()
+    dev = torch.device(device)
+    D_in, H1, H2 = dims
+    self.dims = dims
+    self.n = int(n)
+    if X.dtype != torch.float32 or X.dim() != 2 or X.stride(1) != 1 or X.stride(0) % 4:
+      raise ValueError("X must be float32 [N, ldx] with ldx % 4 == 0")
+    self.X = X
+    self.labels = labels.to(torch.int32).contiguous()
+    th = np.asarray(theta0, np.float32)
+    if th.ndim == 1:
+      th = th[None]
+    self.C = th.shape[0]
+    self.P = param_count(D_in, H1, H2)
+    assert th.shape[1] == self.P
+    self.theta = torch.as_tensor(th, device=dev).contiguous()
+    self.v = torch.ones_like(self.theta) if rms else None
+    self.shadow = torch.empty(self.theta.shape, dtype=torch.int16, device=dev)
+    _lib.call("sgmc_mlp_shadow", _lib.ptr(self.theta), _lib.ptr(self.shadow), self.theta.numel(),
+              _lib.stream())
+    kw = np.zeros((self.C, 2), np.uint32) if keys is None else np.asarray(keys, np.uint32)
+    dk = np.zeros((self.C, 2), np.uint32) if dkeys is None else np.asarray(dkeys, np.uint32)
+    self.keys = torch.as_tensor(kw.view(np.int32).copy(), device=dev)
+    self.dkeys = torch.as_tensor(dk.view(np.int32).copy(), device=dev)
+    nb = int(_lib.query("sgmc_mlp_workspace_bytes", self.C, self.n, D_in, H1, H2))
+    self.work = torch.zeros(nb, dtype=torch.uint8, device=dev)
+    self.U = torch.zeros(self.C, device=dev)
+    eps = np.ascontiguousarray(np.asarray(eps_table, np.float32).reshape(-1))
+    self.eps_table = torch.as_tensor(eps, device=dev)
+    self.table_len = eps.size
+    self.iter = torch.zeros(1, dtype=torch.int32, device=dev)
+    self.counters = torch.zeros(self.C, dtype=torch.int32, device=dev)
+    self.done = torch.zeros(1, dtype=torch.int32, device=dev)
+    self.grad = None
+    self.save_slot = None
+    self.samples = None
+    self.samples_U = None
+    self.n_slots = 0
+    if save_slots is not None:
+      ss = np.asarray(save_slots, np.int32).reshape(-1)
+      self.save_slot = torch.as_tensor(ss, device=dev)
+      self.n_slots = int(ss.max()) + 1 if ss.size and ss.max() >= 0 else 0
+      if self.n_slots:
+        self.samples = torch.zeros((self.C, self.n_slots, self.P), device=dev)
+        self.samples_U = torch.zeros((self.C, self.n_slots), device=dev)
+    self.cfg = dict(rms=1 if rms else 0, alpha=float(alpha), lmbd=float(lmbd),
+                    temperature=float(temperature), noise_temperature=float(noise_temperature),
+                    has_prior=1 if has_prior else 0, prior_scale=float(prior_scale))
+
+  def _state(self):
+    D_in, H1, H2 = self.dims
+    st = MlpState()
+    st.X, st.labels = self.X.data_ptr(), self.labels.data_ptr()
+    st.N, st.ldx = self.X.shape[0], self.X.stride(0)
+    st.D_in, st.H1, st.H2, st.n, st.C = D_in, H1, H2, self.n, self.C
+    st.theta = self.theta.data_ptr()
+    st.v = self.v.data_ptr() if self.v is not None else None
+    st.shadow = self.shadow.data_ptr()
+    st.P = self.P
+    st.key, st.dkey = self.keys.data_ptr(), self.dkeys.data_ptr()
+    st.work, st.work_bytes = self.work.data_ptr(), self.work.numel()
+    st.U = self.U.data_ptr()
+    st.grad = self.grad.data_ptr() if self.grad is not None else None
+    st.samples = self.samples.data_ptr() if self.samples is not None else None
+    st.samples_U = self.samples_U.data_ptr() if self.samples_U is not None else None
+    st.n_slots = self.n_slots
+    st.eps_table = self.eps_table.data_ptr()
+    st.save_slot = self.save_slot.data_ptr() if self.save_slot is not None else None
+    st.iter, st.counters, st.done = (self.iter.data_ptr(), self.counters.data_ptr(),
+                                     self.done.data_ptr())
+    st.table_len = self.table_len
+    for k, v in self.cfg.items():
+      setattr(st, k, v)
+    return st
+
+  def iteration(self):
+    st = self._state()
+    _lib.call("sgmc_mlp_step", ctypes.byref(st), 1, _lib.stream())
+
+  def potential_and_grad(self):
+    """U [C] and grad U [C, P] at the current parameters for the minibatch the data
+    keys select (the keys do not advance)."""
+    if self.grad is None:
+      self.grad = torch.empty_like(self.theta)
+    st = self._state()
+    _lib.call("sgmc_mlp_step", ctypes.byref(st), 0, _lib.stream())
+    return self.U.clone(), self.grad.clone()
+
+  def capture(self, iterations_per_graph=1):
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+      with torch.cuda.graph(g, stream=s):
+        for _ in range(iterations_per_graph):
+          self.iteration()
+    torch.cuda.current_stream().wait_stream(s)
+    self._graph, self._graph_iters = g, iterations_per_graph
+    return g
+
+  def run(self, iterations, graph_iterations=0):
+    if graph_iterations and iterations >= graph_iterations:
+      g = getattr(self, "_graph", None)
+      if g is None or self._graph_iters != graph_iterations:
+        g = self.capture(graph_iterations)
+      reps, rest = divmod(iterations, graph_iterations)
+      for _ in range(reps):
+        g.replay()
+      for _ in range(rest):
+        self.iteration()
+    else:
+      for _ in range(iterations):
+        self.iteration()
+
+  def params(self, c):
+    D_in, H1, H2 = self.dims
+    flat = self.theta[c]
+    out, off = {}, 0
+    for name in ("W1", "W2", "W3", "b1", "b2", "b3"):
+      shp = leaf_shapes(D_in, H1, H2)[name]
+      size = int(np.prod(shp))
+      out[name] = flat[off:off + size].reshape(shp)
+      off += size
+    return out
+
+
+class MLPPotential(torch.autograd.Function):
+  """U(theta) for every chain of a FusedMLP, differentiable: backward returns the
+  hand-written kernel's gradient (no activations kept; the kernels recompute)."""
+
+  @staticmethod
+  def forward(ctx, theta, engine):
+    with torch.no_grad():
+      engine.theta.copy_(theta)
+      _lib.call("sgmc_mlp_shadow", _lib.ptr(engine.theta), _lib.ptr(engine.shadow),
+                engine.theta.numel(), _lib.stream())
+      U, g = engine.potential_and_grad()
+    ctx.save_for_backward(g)
+    return U
+
+  @staticmethod
+  def backward(ctx, gU):
+    (g,) = ctx.saved_tensors
+    return gU[:, None] * g, None
