@@ -151,7 +151,7 @@ def cpu_baseline(X_host, y_host, keys, dkeys, target_s=12.0):
     return time.perf_counter() - t0
 
   t1 = go(1)
-  iters = int(max(1, min(200, target_s / max(t1, 1e-3))))
+  iters = int(max(1, min(5000, target_s / max(t1, 1e-3))))
   dt = go(iters)
   return {"value": C * iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
           "sample": f"{C} chains x {iters} iterations of the same workload "

@@ -89,7 +89,7 @@ int sgmc_gather_rows(const void* src, int64_t src_rows, const int32_t* idx, void
  * 16-byte aligned; D <= 1024. Outputs: U (scalar), grad [D], optional lik [n] (the
  * `likelihoods=True` output, potential.py:208-211) and var(lik) (integrator.py:880).
  * workspace: sgmc_glm_workspace_floats(n, D, nblocks) floats; nblocks from
- * sgmc_glm_blocks(n, D) or any value >= 1. */
+ * sgmc_glm_blocks(n, D) or any value >= 1 with nblocks * 4096 >= n. */
 int sgmc_glm_blocks(int n, int D);
 int64_t sgmc_glm_workspace_floats(int n, int D, int nblocks);
 int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
@@ -177,8 +177,8 @@ typedef struct sgmc_glm_chain {
   uint32_t* dkey;      /* [2] data-loader key, advanced in place */
   float* U;            /* [1] potential at the evaluation point */
   float* var;          /* [1] var(lik) of the minibatch, or NULL */
-  float* lik;          /* [n] scratch: per-row likelihoods */
-  float* partial;      /* [nblocks][4 ceil(D/4) + 1] scratch */
+  float* lik;          /* [n] per-row likelihoods of the last evaluation, or NULL */
+  float* partial;      /* [nblocks][4 ceil(D/4) + 2] scratch */
   unsigned* counter;   /* [1] arrival counter, zero between launches */
   float* samples;      /* [slots][D] or NULL */
   float* samples_U;    /* [slots] potential of each saved sample, or NULL */

@@ -28,7 +28,7 @@ namespace sgmc {
 constexpr int kNT = 256;
 constexpr int kWaves = kNT / kWave;
 constexpr int kMaxD = 1024;  // fused path: features per row
-constexpr int kUnroll = 2;   // row groups in flight per wavefront
+constexpr int kLdsRows = 4096;  // per-row likelihoods staged in LDS per block
 
 struct Lik {
   int family;
@@ -58,11 +58,24 @@ __device__ __forceinline__ void row_lik(const Lik& L, float z, float y, float& l
   }
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ float dot4(float4 a, float4 b) {
   return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
 }
 
+__device__ __forceinline__ float dot4(f4 a, float4 b) {
+  return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
+}
+
 __device__ __forceinline__ void axpy4(float4& acc, float r, float4 x) {
+  acc.x += r * x.x;
+  acc.y += r * x.y;
+  acc.z += r * x.z;
+  acc.w += r * x.w;
+}
+
+__device__ __forceinline__ void axpy4(float4& acc, float r, f4 x) {
   acc.x += r * x.x;
   acc.y += r * x.y;
   acc.z += r * x.z;
@@ -86,66 +99,183 @@ struct RowSource {
   }
 };
 
-// Accumulates sum_j r_j x_j (this lane's chunks) and sum_j lik_j over rows
-// [r0, r1) of the minibatch. th holds the lane's chunks of w.
+// Sum over the LPR lanes of a segment (segments are aligned runs of LPR lanes); every
+// lane of the segment gets the sum. Steps inside a 16-lane row use DPP (quad
+// permutes, half-row and row mirrors), the xor-16 step a ds_swizzle, the last step
+// two readlanes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF,
+                                                        false));
+}
+
+template <int LPR>
+__device__ __forceinline__ float seg_sum(float v) {
+  if (LPR >= 2) v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (LPR >= 4) v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (LPR >= 8) v = dpp_add<0x141>(v);  // row_half_mirror
+  if (LPR >= 16) v = dpp_add<0x140>(v); // row_mirror
+  if (LPR >= 32) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  if (LPR >= 64)
+    v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)) +
+        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  return v;
+}
+
+// Rows in flight per segment per pipeline stage.
+template <int CPL>
+struct Unroll {
+  static constexpr int U = CPL >= 4 ? 2 : 4;
+};
+
+// Row ids of one 64-row chunk of a wavefront's range: lane l holds the id of row cb + l
+// (-1 when outside [0, rows) or past the range end). Labels are loaded with the rows'
+// features, so nothing here has to be waited on before the next loads are issued.
+__device__ __forceinline__ int chunk_rows(const RowSource& src, int cb, int r1) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int rid = -1;
+  if (cb + lane < r1) {
+    const int64_t r = src.row(cb + lane);
+    if (r >= 0) rid = (int)r;
+  }
+  return rid;
+}
+
 template <int LPR, int CPL>
-__device__ __forceinline__ void accumulate_rows(const float* __restrict__ X, int64_t ldx,
-                                                const float* __restrict__ y, int D4,
-                                                const RowSource& src, int r0, int r1,
-                                                const float4 (&th)[CPL], const Lik& L,
-                                                float4 (&acc)[CPL], float& lik_acc,
-                                                float* __restrict__ lik_out) {
+struct RowGroup {
+  static constexpr int U = Unroll<CPL>::U;
+  f4 x[U][CPL];
+  float y[U];
+  bool ok[U];
+};
+
+// Issues the loads of the U row groups starting at group-local offset gl of a chunk.
+template <int LPR, int CPL>
+__device__ __forceinline__ void load_group(gcfloat* __restrict__ X, int64_t ldx,
+                                           gcfloat* __restrict__ y, int D4, int gl, int rid,
+                                           RowGroup<LPR, CPL>& g) {
   constexpr int RPW = kWave / LPR;
+  constexpr int U = RowGroup<LPR, CPL>::U;
   const int lane = threadIdx.x & (kWave - 1);
   const int seg = lane / LPR;
   const int q = lane % LPR;
-  for (int cb = r0; cb < r1; cb += kWave) {
-    const int nrows = min(kWave, r1 - cb);
-    int64_t my_row = -1;
-    float my_y = 0.0f;
-    if (lane < nrows) {
-      my_row = src.row(cb + lane);
-      if (my_row >= 0) my_y = y[my_row];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int sl = gl + u * RPW + seg;  // < 64
+    int r;
+    if (RPW == 1) r = __builtin_amdgcn_readlane(rid, sl);
+    else r = __shfl(rid, sl, kWave);
+    g.ok[u] = r >= 0;
+    const int64_t rc = r >= 0 ? r : 0;
+    g.y[u] = y[rc];
+    SGMC_GLOBAL const f4* rp = reinterpret_cast<SGMC_GLOBAL const f4*>(X + rc * ldx);
+    // every lane loads (clamped to a valid chunk); invalid chunks are zeroed at use
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c4 = q + LPR * k;
+      g.x[u][k] = rp[c4 < D4 ? c4 : D4 - 1];
     }
-    for (int g = 0; g < nrows; g += RPW * kUnroll) {
-      float4 xv[kUnroll][CPL];
-      float yv[kUnroll];
-      bool ok[kUnroll];
+
+  }
+}
+
+template <int LPR, int CPL>
+__device__ __forceinline__ void process_group(RowGroup<LPR, CPL>& g, int row0, int D4,
+                                              const float4 (&th)[CPL], const Lik& L,
+                                              float4 (&acc)[CPL], float& lik_acc,
+                                              float* lds_lik, int lds_row0) {
+  constexpr int RPW = kWave / LPR;
+  constexpr int U = RowGroup<LPR, CPL>::U;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int seg = lane / LPR;
+  const int q = lane % LPR;
+  // chunks past D (clamped loads) and rows outside the data contribute zero
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int rl = g + u * RPW + seg;
-        const int src_lane = rl < nrows ? rl : 0;
-        const int64_t rid = __shfl((long long)my_row, src_lane, kWave);
-        ok[u] = rl < nrows && rid >= 0;
-        yv[u] = __shfl(my_y, src_lane, kWave);
-        const float4* rp = reinterpret_cast<const float4*>(X + (ok[u] ? rid : 0) * ldx);
+  for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int k = 0; k < CPL; ++k) {
-          const int c4 = q + LPR * k;
-          xv[u][k] = (ok[u] && c4 < D4) ? rp[c4] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        float z = 0.0f;
-#pragma unroll
-        for (int k = 0; k < CPL; ++k) z += dot4(xv[u][k], th[k]);
-#pragma unroll
-        for (int m = 1; m < LPR; m <<= 1) z += __shfl_xor(z, m, kWave);
-        float lik, r;
-        row_lik(L, z, yv[u], lik, r);
-        if (!ok[u]) {
-          lik = 0.0f;
-          r = 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < CPL; ++k) axpy4(acc[k], r, xv[u][k]);
-        if (q == 0 && ok[u]) {
-          lik_acc += lik;
-          if (lik_out) lik_out[cb + g + u * RPW + seg] = lik;
-        }
-      }
+    for (int k = 0; k < CPL; ++k) {
+      const bool valid = g.ok[u] && (q + LPR * k) < D4;
+      const f4 zero = {0.f, 0.f, 0.f, 0.f};
+      g.x[u][k] = valid ? g.x[u][k] : zero;
     }
+  }
+  float z[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) t += dot4(g.x[u][k], th[k]);
+    z[u] = t;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) z[u] = seg_sum<LPR>(z[u]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    float lik, r;
+    row_lik(L, z[u], g.y[u], lik, r);
+    if (!g.ok[u]) {
+      lik = 0.0f;
+      r = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) axpy4(acc[k], r, g.x[u][k]);
+    if (q == 0 && g.ok[u]) {
+      lik_acc += lik;
+      lds_lik[row0 + u * RPW + seg - lds_row0] = lik;
+    }
+  }
+}
+
+// Accumulates sum_j r_j x_j (this lane's chunks) and sum_j lik_j over minibatch rows
+// [r0, r1). Loads run one row group ahead of the arithmetic (two register buffers), and
+// the row ids of the next 64-row chunk are drawn before the current chunk is done.
+template <int LPR, int CPL>
+__device__ __forceinline__ void accumulate_rows(gcfloat* __restrict__ X, int64_t ldx,
+                                                gcfloat* __restrict__ y, int D4,
+                                                const RowSource& src, int r0, int r1,
+                                                const float4 (&th)[CPL], const Lik& L,
+                                                float4 (&acc)[CPL], float& lik_acc,
+                                                float* lds_lik, int lds_row0) {
+  constexpr int RPW = kWave / LPR;
+  constexpr int G = RPW * RowGroup<LPR, CPL>::U;  // rows per group; divides 64
+  static_assert(kWave % G == 0, "row groups must tile a 64-row chunk");
+  if (r0 >= r1) return;
+  int cb = r0;
+  int rid = chunk_rows(src, cb, r1);
+  int rid_n = chunk_rows(src, cb + kWave, r1);
+  RowGroup<LPR, CPL> ga, gb;
+  load_group<LPR, CPL>(X, ldx, y, D4, 0, rid, ga);
+  int gl = 0;  // group offset inside the current chunk
+  // The prefetch is issued on every path (rows past the range carry id -1 and load the
+  // clamped row 0), so the number of loads in flight is the same on every path and the
+  // wait before the arithmetic only covers the group being processed.
+  for (;;) {
+    int nl = gl + G;
+    bool more = cb + nl < r1;
+    bool next = nl >= kWave;
+    load_group<LPR, CPL>(X, ldx, y, D4, next ? 0 : nl, next ? rid_n : rid, gb);
+    process_group<LPR, CPL>(ga, cb + gl, D4, th, L, acc, lik_acc, lds_lik, lds_row0);
+    if (!more) break;
+    if (next) {
+      cb += kWave;
+      nl = 0;
+      rid = rid_n;
+      rid_n = chunk_rows(src, cb + kWave, r1);
+    }
+    gl = nl;
+    nl = gl + G;
+    more = cb + nl < r1;
+    next = nl >= kWave;
+    load_group<LPR, CPL>(X, ldx, y, D4, next ? 0 : nl, next ? rid_n : rid, ga);
+    process_group<LPR, CPL>(gb, cb + gl, D4, th, L, acc, lik_acc, lds_lik, lds_row0);
+    if (!more) break;
+    if (next) {
+      cb += kWave;
+      nl = 0;
+      rid = rid_n;
+      rid_n = chunk_rows(src, cb + kWave, r1);
+    }
+    gl = nl;
   }
 }
 
@@ -153,7 +283,7 @@ __device__ __forceinline__ void accumulate_rows(const float* __restrict__ X, int
 // writes the block's column sums to out[0..4*D4) and returns the block's lik sum.
 template <int LPR, int CPL>
 __device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float lik_acc, int D4,
-                                                    float* __restrict__ out, float* lds_cols,
+                                                    gfloat* __restrict__ out, float* lds_cols,
                                                     float* lds_small) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
@@ -188,10 +318,11 @@ __device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float li
 
 // Splits rows [0, n) of a minibatch among the blocks of a chain and then among the
 // wavefronts of a block (contiguous ranges).
-__device__ __forceinline__ void wave_rows(int n, int nblocks, int b, int& r0, int& r1) {
+__device__ __forceinline__ void wave_rows(int n, int nblocks, int b, int& r0, int& r1,
+                                          int& b0, int& b1) {
   const int per_block = (n + nblocks - 1) / nblocks;
-  const int b0 = min(n, b * per_block);
-  const int b1 = min(n, b0 + per_block);
+  b0 = min(n, b * per_block);
+  b1 = min(n, b0 + per_block);
   const int per_wave = (b1 - b0 + kWaves - 1) / kWaves;
   const int w = threadIdx.x / kWave;
   r0 = min(b1, b0 + w * per_wave);
@@ -222,8 +353,7 @@ struct PotArgs {
   float* out_grad;
   float* out_lik;
   float* out_var;
-  float* partial;      // [nblocks][4*D4 + 1]
-  float* lik_scratch;  // [n]
+  float* partial;      // [nblocks][4*D4 + 2]
   unsigned* counter;
   int64_t ldx;
   int64_t N;
@@ -237,6 +367,7 @@ template <int LPR, int CPL>
 __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
   __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
+  __shared__ float lds_lik[kLdsRows];
   __shared__ int flag;
   float4 th[CPL];
   load_theta<LPR, CPL>(a.theta, a.D, th);
@@ -244,25 +375,49 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
 #pragma unroll
   for (int k = 0; k < CPL; ++k) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
   float lik_acc = 0.0f;
-  int r0, r1;
-  wave_rows(a.n, gridDim.x, blockIdx.x, r0, r1);
+  int r0, r1, b0, b1;
+  wave_rows(a.n, gridDim.x, blockIdx.x, r0, r1, b0, b1);
   RowSource src{a.idx, 0, Key{0, 0}, Key{0, 0}, 0, 0, 0, a.idx ? a.N : (int64_t)a.n};
-  accumulate_rows<LPR, CPL>(a.X, a.ldx, a.y, a.D4, src, r0, r1, th, a.L, acc, lik_acc,
-                            a.lik_scratch);
-  const int stride = 4 * a.D4 + 1;
-  float* mine = a.partial + (int64_t)blockIdx.x * stride;
+  accumulate_rows<LPR, CPL>(to_global(a.X), a.ldx, to_global(a.y), a.D4, src, r0, r1, th, a.L,
+                            acc, lik_acc, lds_lik, b0);
+  // Block record: column sums, the block's lik sum and its lik sum of squared deviations
+  // about the block mean; per-row likelihoods go straight to out_lik.
+  const int stride = 4 * a.D4 + 2;
+  gfloat* mine = to_global(a.partial) + (int64_t)blockIdx.x * stride;
   const float ls = block_reduce_write<LPR, CPL>(acc, lik_acc, a.D4, mine, lds_cols, lds_small);
-  if (threadIdx.x == 0) mine[4 * a.D4] = ls;
+  const int cnt = b1 - b0;
+  float m2 = 0.0f;
+  if (a.out_var) {
+    const float mb = cnt > 0 ? ls / (float)cnt : 0.0f;
+    float s2 = 0.0f;
+    for (int j = threadIdx.x; j < cnt; j += kNT) {
+      const float d = lds_lik[j] - mb;
+      s2 += d * d;
+    }
+    m2 = block_sum<kNT>(s2, lds_small);
+  }
+  if (threadIdx.x == 0) {
+    mine[4 * a.D4] = ls;
+    mine[4 * a.D4 + 1] = m2;
+  }
+  if (a.out_lik) {
+    gfloat* lo = to_global(a.out_lik);
+    for (int j = threadIdx.x; j < cnt; j += kNT) lo[b0 + j] = lds_lik[j];
+  }
   if (!arrive_last(a.counter, gridDim.x, &flag)) return;
 
-  // Last block: combine partials in block order, add the prior, scale.
+  // Last block: combine the block records, add the prior, scale.
+  gcfloat* part = to_global(a.partial);
+  const int nb = gridDim.x;
   const float c = (float)(-a.N) / (float)a.n;
-  float sumlik = 0.0f;
-  for (int b = 0; b < (int)gridDim.x; ++b) sumlik += a.partial[(int64_t)b * stride + 4 * a.D4];
+  float sl = 0.0f;
+  for (int b = threadIdx.x; b < nb; b += kNT) sl += part[(int64_t)b * stride + 4 * a.D4];
+  const float sumlik = block_sum<kNT>(sl, lds_small);
+  const float mean = sumlik / (float)a.n;
   float lp = 0.0f;
   for (int j = threadIdx.x; j < a.D; j += kNT) {
-    float s = 0.0f;
-    for (int b = 0; b < (int)gridDim.x; ++b) s += a.partial[(int64_t)b * stride + j];
+    float s, unused;
+    sum_strided2(part + j, part + j, stride, nb, s, unused);
     const float wj = a.theta[j];
     float g = c * s;
     if (a.pr.has_prior) {
@@ -272,18 +427,21 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
     a.out_grad[j] = g / a.temperature;
   }
   lp = block_sum<kNT>(lp, lds_small);
-  const float mean = sumlik / (float)a.n;
   if (threadIdx.x == 0) a.out_U[0] = ((float)(-a.N) * mean - lp) / a.temperature;
-  if (a.out_lik || a.out_var) {
-    float vs = 0.0f;
-    for (int j = threadIdx.x; j < a.n; j += kNT) {
-      const float l = a.lik_scratch[j];
-      if (a.out_lik) a.out_lik[j] = l;
-      const float d = l - mean;
-      vs += d * d;
+  if (a.out_var) {
+    // var(lik) = (sum_b M2_b + cnt_b (mean_b - mean)^2) / n  (pairwise/Chan combination)
+    const int per_block = (a.n + nb - 1) / nb;
+    float t = 0.0f;
+    for (int b = threadIdx.x; b < nb; b += kNT) {
+      const int lo = min(a.n, b * per_block);
+      const int k = min(a.n, lo + per_block) - lo;
+      if (k > 0) {
+        const float d = part[(int64_t)b * stride + 4 * a.D4] / (float)k - mean;
+        t += part[(int64_t)b * stride + 4 * a.D4 + 1] + (float)k * (d * d);
+      }
     }
-    vs = block_sum<kNT>(vs, lds_small);
-    if (a.out_var && threadIdx.x == 0) a.out_var[0] = vs / (float)a.n;
+    t = block_sum<kNT>(t, lds_small);
+    if (threadIdx.x == 0) a.out_var[0] = t / (float)a.n;
   }
 }
 
@@ -384,11 +542,19 @@ __device__ __forceinline__ float leaf_normal(Key leafkey, uint32_t S, uint32_t i
 template <int LPR, int CPL>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_chain& ch,
                                            int bchain, float* lds_cols, float* lds_small,
-                                           int* flag) {
+                                           float* lds_lik, int* flag) {
   const int it = max(0, min(a.iter[0], a.table_len - 1));
   const float eps = a.eps_table[it];
   const int D = ch.D;
   const int D4 = (D + 3) >> 2;
+  gfloat* g_theta = to_global(ch.theta);
+  gfloat* g_mom = to_global(ch.mom);
+  gfloat* g_v = to_global(ch.v);
+  gfloat* g_partial = to_global(ch.partial);
+  gfloat* g_lik = to_global(ch.lik);
+  gfloat* g_samples = to_global(ch.samples);
+  gcfloat* g_inv_mass = to_global(a.inv_mass);
+  gcfloat* g_sqrt_mass = to_global(a.sqrt_mass);
   const Key key0 = load_key(ch.key);
   const bool hmc = a.mode == SGMC_MODE_SGHMC;
   const bool resample = hmc && a.step == 0;
@@ -415,16 +581,16 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         const int c = 4 * (q + LPR * k) + t;
         float v = 0.0f;
         if (c < D) {
-          v = ch.theta[c];
+          v = g_theta[c];
           if (hmc) {
             float p;
             if (resample) {
               const float xi = leaf_normal(resample_leaf, (uint32_t)D, (uint32_t)c);
-              p = a.sqrt_mass ? a.sqrt_mass[c] * xi : xi;
+              p = a.sqrt_mass ? g_sqrt_mass[c] * xi : xi;
             } else {
-              p = ch.mom[c];
+              p = g_mom[c];
             }
-            const float s = a.inv_mass ? a.inv_mass[c] * p : p;
+            const float s = a.inv_mass ? g_inv_mass[c] * p : p;
             v = v + eps * s;
           }
         }
@@ -450,21 +616,56 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
 #pragma unroll
   for (int k = 0; k < CPL; ++k) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
   float lik_acc = 0.0f;
-  int r0, r1;
-  wave_rows(ch.n, ch.nblocks, bchain, r0, r1);
-  accumulate_rows<LPR, CPL>(ch.X, ch.ldx, ch.y, D4, src, r0, r1, th, a.L, acc, lik_acc,
-                            ch.lik);
-  const int stride = 4 * D4 + 1;
-  float* mine = ch.partial + (int64_t)bchain * stride;
+  int r0, r1, b0, b1;
+  wave_rows(ch.n, ch.nblocks, bchain, r0, r1, b0, b1);
+  accumulate_rows<LPR, CPL>(to_global(ch.X), ch.ldx, to_global(ch.y), D4, src, r0, r1, th, a.L,
+                            acc, lik_acc, lds_lik, b0);
+  // Block record: column sums, then the block's lik sum and its lik sum of squared
+  // deviations about the block mean (combined into var(lik) by the last block).
+  const int stride = 4 * D4 + 2;
+  gfloat* mine = g_partial + (int64_t)bchain * stride;
   const float ls = block_reduce_write<LPR, CPL>(acc, lik_acc, D4, mine, lds_cols, lds_small);
-  if (threadIdx.x == 0) mine[4 * D4] = ls;
+  const int cnt = b1 - b0;
+  float m2 = 0.0f;
+  if (ch.var) {
+    const float mb = cnt > 0 ? ls / (float)cnt : 0.0f;
+    float s2 = 0.0f;
+    for (int j = threadIdx.x; j < cnt; j += kNT) {
+      const float d = lds_lik[j] - mb;
+      s2 += d * d;
+    }
+    m2 = block_sum<kNT>(s2, lds_small);
+  }
+  if (threadIdx.x == 0) {
+    mine[4 * D4] = ls;
+    mine[4 * D4 + 1] = m2;
+  }
+  if (ch.lik)
+    for (int j = threadIdx.x; j < cnt; j += kNT) g_lik[b0 + j] = lds_lik[j];
   if (!arrive_last(ch.counter, ch.nblocks, flag)) return;
 
   // ---- last block of this chain: potential, gradient and the state update ----
+  const int nb = ch.nblocks;
   const float c = (float)(-ch.N) / (float)ch.n;
-  float sumlik = 0.0f;
-  for (int b = 0; b < ch.nblocks; ++b) sumlik += ch.partial[(int64_t)b * stride + 4 * D4];
+  float sl = 0.0f;
+  for (int b = threadIdx.x; b < nb; b += kNT) sl += g_partial[(int64_t)b * stride + 4 * D4];
+  const float sumlik = block_sum<kNT>(sl, lds_small);
   const float mean = sumlik / (float)ch.n;
+  // var(lik) = (sum_b M2_b + cnt_b (mean_b - mean)^2) / n  (pairwise/Chan combination)
+  float var_sum = 0.0f;
+  if (ch.var) {
+    const int per_block = (ch.n + nb - 1) / nb;
+    float t = 0.0f;
+    for (int b = threadIdx.x; b < nb; b += kNT) {
+      const int lo = min(ch.n, b * per_block);
+      const int k = min(ch.n, lo + per_block) - lo;
+      if (k > 0) {
+        const float d = g_partial[(int64_t)b * stride + 4 * D4] / (float)k - mean;
+        t += g_partial[(int64_t)b * stride + 4 * D4 + 1] + (float)k * (d * d);
+      }
+    }
+    var_sum = block_sum<kNT>(t, lds_small);
+  }
 
   // Noise key of this step.
   Key knext, ksub;
@@ -473,33 +674,48 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   const int slot = a.save_slot ? a.save_slot[it] : -1;
   const bool last_step = !hmc || a.step == a.steps - 1;
   const float Tn = ch.temperature;
+  const bool rms = a.mode == SGMC_MODE_SGLD_RMS;
 
   float lp = 0.0f;
   const uint32_t S = (uint32_t)D;
   const uint32_t m = (S + 1) >> 1;
   for (uint32_t jp = threadIdx.x; jp < m; jp += kNT) {
+    // Element jp and its partner jp + m share one Threefry block of the noise draw.
     uint32_t w0, w1;
     bits_pair(leaf, S, jp, w0, w1);
+    const bool two = jp + m < S;
+    const uint32_t jj[2] = {jp, two ? jp + m : jp};
+    // every load of both columns is issued before the arithmetic
+    float s[2];
+    sum_strided2(g_partial + jj[0], g_partial + jj[1], stride, nb, s[0], s[1]);
+    float wv[2], pv[2], imv[2], smv[2], vv[2];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const uint32_t j = half ? jp + m : jp;
-      if (half && j >= S) break;
-      const float xi = bits_to_normal(half ? w1 : w0);
-      float s = 0.0f;
-      for (int b = 0; b < ch.nblocks; ++b) s += ch.partial[(int64_t)b * stride + j];
-      float wj = ch.theta[j];
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t j = jj[h];
+      wv[h] = g_theta[j];
+      pv[h] = (hmc && !resample) ? g_mom[j] : 0.0f;
+      imv[h] = (hmc && a.inv_mass) ? g_inv_mass[j] : 1.0f;
+      smv[h] = (resample && a.sqrt_mass) ? g_sqrt_mass[j] : 1.0f;
+      vv[h] = rms ? g_v[j] : 0.0f;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h && !two) break;
+      const uint32_t j = jj[h];
+      const float xi = bits_to_normal(h ? w1 : w0);
+      float wj = wv[h];
       float p = 0.0f, sm = 0.0f;
       if (hmc) {
         if (resample) {
           const float xr = leaf_normal(resample_leaf, S, j);
-          p = a.sqrt_mass ? a.sqrt_mass[j] * xr : xr;
+          p = a.sqrt_mass ? smv[h] * xr : xr;
         } else {
-          p = ch.mom[j];
+          p = pv[h];
         }
-        sm = a.inv_mass ? a.inv_mass[j] * p : p;
+        sm = a.inv_mass ? imv[h] * p : p;
         wj = wj + eps * sm;  // evaluation point
       }
-      float g = c * s;
+      float g = c * s[h];
       if (a.pr.has_prior) {
         g = g + wj * a.pr.inv_var;
         lp += (a.pr.log_norm + (wj * wj) * a.pr.inv_var) / -2.0f;
@@ -509,9 +725,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       if (!hmc) {
         const float sg = (-eps) * g;
         const float sn = sqrtf((2.0f * Tn) * eps) * xi;
-        if (a.mode == SGMC_MODE_SGLD_RMS) {
-          const float vn = a.alpha * ch.v[j] + (1.0f - a.alpha) * (g * g);
-          ch.v[j] = vn;
+        if (rms) {
+          const float vn = a.alpha * vv[h] + (1.0f - a.alpha) * (g * g);
+          g_v[j] = vn;
           const float G = 1.0f / (a.lmbd + sqrtf(vn));
           wn = wj + ((0.0f + G * sg) + sqrtf(G) * sn);
         } else {
@@ -522,25 +738,19 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         float pn = p + ((-eps) * C) * sm;
         pn = pn + (-eps) * g;
         pn = pn + C * (sqrtf(2.0f * eps) * xi);
-        ch.mom[j] = pn;
+        g_mom[j] = pn;
         wn = wj;
       }
-      ch.theta[j] = wn;
-      if (slot >= 0 && last_step && ch.samples) ch.samples[(int64_t)slot * D + j] = wn;
+      g_theta[j] = wn;
+      if (slot >= 0 && last_step && ch.samples) g_samples[(int64_t)slot * D + j] = wn;
     }
   }
   lp = block_sum<kNT>(lp, lds_small);
-  float vs = 0.0f;
-  for (int j = threadIdx.x; j < ch.n; j += kNT) {
-    const float d = ch.lik[j] - mean;
-    vs += d * d;
-  }
-  vs = block_sum<kNT>(vs, lds_small);
   if (threadIdx.x == 0) {
     const float Uc = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
     ch.U[0] = Uc;
     if (slot >= 0 && last_step && ch.samples_U) ch.samples_U[slot] = Uc;
-    if (ch.var) ch.var[0] = vs / (float)ch.n;
+    if (ch.var) ch.var[0] = var_sum / (float)ch.n;
     store_key(ch.key, knext);
     store_key(ch.dkey, dnext);
     // Last chain of the launch advances the iteration counter.
@@ -557,6 +767,7 @@ template <int DCLASS>
 __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
+  __shared__ float lds_lik[kLdsRows];
   __shared__ int flag;
   const int chain = a.block_chain[blockIdx.x];
   if (chain < 0 || chain >= a.nchains) return;
@@ -565,11 +776,11 @@ __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   if (bchain < 0 || bchain >= ch.nblocks) return;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   switch (dc) {
-    case 0: fused_body<4, 1>(a, ch, bchain, lds_cols, lds_small, &flag); break;
-    case 1: fused_body<16, 1>(a, ch, bchain, lds_cols, lds_small, &flag); break;
-    case 2: fused_body<64, 1>(a, ch, bchain, lds_cols, lds_small, &flag); break;
-    case 3: fused_body<64, 2>(a, ch, bchain, lds_cols, lds_small, &flag); break;
-    default: fused_body<64, 4>(a, ch, bchain, lds_cols, lds_small, &flag); break;
+    case 0: fused_body<4, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 1: fused_body<16, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 2: fused_body<64, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 3: fused_body<64, 2>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    default: fused_body<64, 4>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
   }
 }
 
@@ -607,7 +818,8 @@ extern "C" {
 
 int64_t sgmc_glm_workspace_floats(int n, int D, int nblocks) {
   const int D4 = (D + 3) / 4;
-  return (int64_t)nblocks * (4 * D4 + 1) + (int64_t)n + 4;
+  (void)n;
+  return (int64_t)nblocks * (4 * D4 + 2) + 4;
 }
 
 int sgmc_glm_blocks(int n, int D) {
@@ -633,7 +845,8 @@ int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
                (long long)ldx);
   SGMC_REQUIRE(((uintptr_t)X & 15) == 0, SGMC_E_ARG, "sgmc_glm_potential: X not 16B aligned");
   SGMC_REQUIRE(n >= 1 && N >= 1, SGMC_E_ARG, "sgmc_glm_potential: empty batch");
-  SGMC_REQUIRE(nblocks >= 1, SGMC_E_ARG, "sgmc_glm_potential: nblocks < 1");
+  SGMC_REQUIRE(nblocks >= 1 && (int64_t)nblocks * kLdsRows >= n, SGMC_E_ARG,
+               "sgmc_glm_potential: nblocks must be >= 1 and >= n / %d", kLdsRows);
   SGMC_REQUIRE(workspace_floats >= sgmc_glm_workspace_floats(n, D, nblocks), SGMC_E_ARG,
                "sgmc_glm_potential: workspace too small");
   SGMC_REQUIRE(temperature != 0.0f, SGMC_E_ARG, "sgmc_glm_potential: temperature is zero");
@@ -649,8 +862,7 @@ int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
   a.D = D;
   a.D4 = (D + 3) / 4;
   a.partial = workspace;
-  a.lik_scratch = workspace + (int64_t)nblocks * (4 * a.D4 + 1);
-  a.counter = reinterpret_cast<unsigned*>(a.lik_scratch + n);
+  a.counter = reinterpret_cast<unsigned*>(workspace + (int64_t)nblocks * (4 * a.D4 + 2));
   a.ldx = ldx;
   a.N = N;
   a.n = n;

@@ -23,6 +23,21 @@ namespace sgmc {
 
 constexpr int kWave = 64;
 
+// Global-address-space pointers. Pointers read from device memory (a chain table)
+// are generic to the compiler and become flat loads, which also count against the
+// LDS/scalar wait counter and so serialize with every LDS or DPP wait; casting them
+// to address space 1 yields global loads.
+#define SGMC_GLOBAL __attribute__((address_space(1)))
+typedef SGMC_GLOBAL float gfloat;
+typedef SGMC_GLOBAL const float gcfloat;
+typedef SGMC_GLOBAL uint32_t guint;
+typedef SGMC_GLOBAL unsigned gunsigned;
+
+template <class T>
+__device__ __forceinline__ SGMC_GLOBAL T* to_global(T* p) {
+  return (SGMC_GLOBAL T*)p;
+}
+
 struct Key {
   uint32_t k0, k1;
 };
@@ -285,6 +300,31 @@ __device__ __forceinline__ double block_sum_d(double v, double* lds) {
 #pragma unroll
   for (int i = 0; i < NT / kWave; ++i) t += lds[i];
   return t;
+}
+
+// s0 = sum over b in [0, nb) of p0[b * stride], s1 likewise for p1, each added in block
+// order. The loads go out eight records at a time (missing records read as +0, which
+// leaves the sums unchanged), so one memory latency covers eight records.
+__device__ __forceinline__ void sum_strided2(const SGMC_GLOBAL float* p0,
+                                             const SGMC_GLOBAL float* p1, int64_t stride,
+                                             int nb, float& s0, float& s1) {
+  float a0 = 0.0f, a1 = 0.0f;
+  for (int b = 0; b < nb; b += 8) {
+    float x0[8], x1[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const bool in = b + t < nb;
+      x0[t] = in ? p0[(int64_t)(b + t) * stride] : 0.0f;
+      x1[t] = in ? p1[(int64_t)(b + t) * stride] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      a0 += x0[t];
+      a1 += x1[t];
+    }
+  }
+  s0 = a0;
+  s1 = a1;
 }
 
 // Arrival counter for "last block finishes" reductions: every block calls this after

@@ -45,12 +45,13 @@ class FusedGLM:
   chains: list of dicts with keys X (device float32 [N, ldx], ldx % 4 == 0, zero
   padding beyond D), y (device float32 [N]), D, n, theta0 (array-like [D]), key and
   dkey (two uint32 words each), temperature (SGLD noise temperature, default 1).
+  keep_lik: also keep the per-row likelihoods of the last evaluation in ``lik``.
   """
 
   def __init__(self, chains, family, mode, eps_table, steps=1, friction=1.0, alpha=0.9,
                lmbd=1e-5, sigma=1.0, prior_scale=1.0, has_prior=True, pot_temperature=1.0,
                inv_mass=None, sqrt_mass=None, save_slots=None, n_slots=0, target_bytes=None,
-               device="cuda"):
+               keep_lik=False, device="cuda"):
     _lib.require_gpu()
     self.device = torch.device(device)
     self.family = FAMILIES[family] if isinstance(family, str) else int(family)
@@ -99,6 +100,7 @@ class FusedGLM:
     self.uniform_dclass = dcl.pop() if len(dcl) == 1 else -1
     total_bytes = sum(int(c["n"]) * 4 * _align4(int(c["D"])) for c in chains)
     self.target_bytes = target_bytes or max(64 * 1024, total_bytes // 2048)
+    self.keep_lik = bool(keep_lik)
     self._allocate()
     self._build_table()
 
@@ -124,7 +126,7 @@ class FusedGLM:
     self.U = torch.zeros(self.C, device=dev)
     self.var = torch.zeros(self.C, device=dev)
     self.loff = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
-    self.lik = torch.zeros(int(self.loff[-1]), device=dev)
+    self.lik = torch.zeros(int(self.loff[-1]), device=dev) if self.keep_lik else None
     self.counters = torch.zeros(self.C, dtype=torch.int32, device=dev)
     self.iter = torch.zeros(1, dtype=torch.int32, device=dev)
     self.done = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -140,7 +142,7 @@ class FusedGLM:
     self.block_begin = np.concatenate([[0], np.cumsum(nbs)]).astype(np.int64)
     self.nblocks = int(self.block_begin[-1])
     D4 = (self.D + 3) // 4
-    stride = 4 * D4 + 1
+    stride = 4 * D4 + 2
     pof = np.concatenate([[0], np.cumsum(self.nblocks_chain * stride)]).astype(np.int64)
     self.partial = torch.zeros(int(pof[-1]), device=self.device)
     arr = (_lib.GlmChain * self.C)()
@@ -157,7 +159,7 @@ class FusedGLM:
       e.dkey = self.dkeys.data_ptr() + 8 * i
       e.U = self.U.data_ptr() + fs * i
       e.var = self.var.data_ptr() + fs * i
-      e.lik = self.lik.data_ptr() + fs * int(self.loff[i])
+      e.lik = (self.lik.data_ptr() + fs * int(self.loff[i])) if self.lik is not None else None
       e.partial = self.partial.data_ptr() + fs * int(pof[i])
       e.counter = self.counters.data_ptr() + 4 * i
       e.samples = (self.samples.data_ptr() + fs * int(self.poff[i]) * self.n_slots
