@@ -311,7 +311,7 @@ __device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float li
     float s = 0.0f;
 #pragma unroll
     for (int ww = 0; ww < kWaves; ++ww) s += lds_cols[ww * 4 * D4 + c];
-    out[c] = s;
+    publish(out + c, s);
   }
   return lsum;
 }
@@ -397,8 +397,8 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
     m2 = block_sum<kNT>(s2, lds_small);
   }
   if (threadIdx.x == 0) {
-    mine[4 * a.D4] = ls;
-    mine[4 * a.D4 + 1] = m2;
+    publish(mine + 4 * a.D4, ls);
+    publish(mine + 4 * a.D4 + 1, m2);
   }
   if (a.out_lik) {
     gfloat* lo = to_global(a.out_lik);
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
   const int nb = gridDim.x;
   const float c = (float)(-a.N) / (float)a.n;
   float sl = 0.0f;
-  for (int b = threadIdx.x; b < nb; b += kNT) sl += part[(int64_t)b * stride + 4 * a.D4];
+  for (int b = threadIdx.x; b < nb; b += kNT) sl += consume(part + (int64_t)b * stride + 4 * a.D4);
   const float sumlik = block_sum<kNT>(sl, lds_small);
   const float mean = sumlik / (float)a.n;
   float lp = 0.0f;
@@ -436,8 +436,8 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
       const int lo = min(a.n, b * per_block);
       const int k = min(a.n, lo + per_block) - lo;
       if (k > 0) {
-        const float d = part[(int64_t)b * stride + 4 * a.D4] / (float)k - mean;
-        t += part[(int64_t)b * stride + 4 * a.D4 + 1] + (float)k * (d * d);
+        const float d = consume(part + (int64_t)b * stride + 4 * a.D4) / (float)k - mean;
+        t += consume(part + (int64_t)b * stride + 4 * a.D4 + 1) + (float)k * (d * d);
       }
     }
     t = block_sum<kNT>(t, lds_small);
@@ -492,10 +492,10 @@ __global__ void __launch_bounds__(kNT) glm_full_kernel(FullArgs a) {
     if (q == 0) acc += (double)lik;
   }
   const double bs = block_sum_d<kNT>(acc, lds_d);
-  if (threadIdx.x == 0) a.partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) publish(a.partial + blockIdx.x, bs);
   if (!arrive_last(a.counter, gridDim.x, &flag)) return;
   double tot = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT) tot += a.partial[b];
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT) tot += consume(a.partial + b);
   tot = block_sum_d<kNT>(tot, lds_d);
   double lp = 0.0;
   if (a.pr.has_prior) {
@@ -637,8 +637,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     m2 = block_sum<kNT>(s2, lds_small);
   }
   if (threadIdx.x == 0) {
-    mine[4 * D4] = ls;
-    mine[4 * D4 + 1] = m2;
+    publish(mine + 4 * D4, ls);
+    publish(mine + 4 * D4 + 1, m2);
   }
   if (ch.lik)
     for (int j = threadIdx.x; j < cnt; j += kNT) g_lik[b0 + j] = lds_lik[j];
@@ -648,7 +648,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   const int nb = ch.nblocks;
   const float c = (float)(-ch.N) / (float)ch.n;
   float sl = 0.0f;
-  for (int b = threadIdx.x; b < nb; b += kNT) sl += g_partial[(int64_t)b * stride + 4 * D4];
+  for (int b = threadIdx.x; b < nb; b += kNT) sl += consume(g_partial + (int64_t)b * stride + 4 * D4);
   const float sumlik = block_sum<kNT>(sl, lds_small);
   const float mean = sumlik / (float)ch.n;
   // var(lik) = (sum_b M2_b + cnt_b (mean_b - mean)^2) / n  (pairwise/Chan combination)
@@ -660,8 +660,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       const int lo = min(ch.n, b * per_block);
       const int k = min(ch.n, lo + per_block) - lo;
       if (k > 0) {
-        const float d = g_partial[(int64_t)b * stride + 4 * D4] / (float)k - mean;
-        t += g_partial[(int64_t)b * stride + 4 * D4 + 1] + (float)k * (d * d);
+        const float d = consume(g_partial + (int64_t)b * stride + 4 * D4) / (float)k - mean;
+        t += consume(g_partial + (int64_t)b * stride + 4 * D4 + 1) + (float)k * (d * d);
       }
     }
     var_sum = block_sum<kNT>(t, lds_small);
@@ -753,8 +753,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     if (ch.var) ch.var[0] = var_sum / (float)ch.n;
     store_key(ch.key, knext);
     store_key(ch.dkey, dnext);
-    // Last chain of the launch advances the iteration counter.
-    __threadfence();
+    // Last chain of the launch advances the iteration counter (read by the next launch).
     const unsigned prev = atomicAdd(a.done, 1u);
     if (prev == (unsigned)a.nchains - 1) {
       atomicExch(a.done, 0u);

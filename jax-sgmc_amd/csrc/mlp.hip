@@ -437,13 +437,14 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
     }
   }
   const float tot = block_sum<GT>(lp, red);
-  if (threadIdx.x == 0) s.prior_part[(int64_t)c * s.nparts + part0 + blockIdx.x] = tot;
+  if (threadIdx.x == 0) publish(s.prior_part + (int64_t)c * s.nparts + part0 + blockIdx.x, tot);
   if (!arrive_last(s.counters + c, gridDim.x, &flag)) return;
   float lsum = 0.f;
   for (int r = threadIdx.x; r < s.n; r += GT) lsum += s.lik[(int64_t)c * s.n + r];
   lsum = block_sum<GT>(lsum, red);
   float prior = 0.f;
-  for (int q = threadIdx.x; q < s.nparts; q += GT) prior += s.prior_part[(int64_t)c * s.nparts + q];
+  for (int q = threadIdx.x; q < s.nparts; q += GT)
+    prior += consume(s.prior_part + (int64_t)c * s.nparts + q);
   prior = block_sum<GT>(prior, red);
   if (threadIdx.x == 0) {
     const float mean = lsum / (float)s.n;
@@ -457,7 +458,7 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
       store_key(s.key + 2 * c, next);
       split2(load_key(s.dkey + 2 * c), next, sub);
       store_key(s.dkey + 2 * c, next);
-      __threadfence();
+      // the last chain advances the iteration counter (read by the next launch)
       const unsigned prev = atomicAdd(s.done, 1u);
       if (prev == (unsigned)s.C - 1) {
         atomicExch(s.done, 0u);

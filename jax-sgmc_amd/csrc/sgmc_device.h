@@ -302,9 +302,31 @@ __device__ __forceinline__ double block_sum_d(double v, double* lds) {
   return t;
 }
 
+// ---- block-to-block hand-off inside one launch ----
+// Values that blocks write for the last-arriving block of the same launch go through
+// agent-scope relaxed atomic stores and loads (publish / consume). The L2 keeps such
+// accesses coherent across the XCDs by itself (the stores write through, the loads do
+// not hit lines another XCD may have changed), so the hand-off needs none of the
+// whole-cache L2 writeback and invalidate that a device-scope fence performs.
+__device__ __forceinline__ void publish(SGMC_GLOBAL float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish(float* p, float v) { publish(to_global(p), v); }
+__device__ __forceinline__ float consume(const SGMC_GLOBAL float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float consume(const float* p) { return consume(to_global(p)); }
+__device__ __forceinline__ void publish(double* p, double v) {
+  __hip_atomic_store(to_global(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double consume(const double* p) {
+  return __hip_atomic_load(to_global(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // s0 = sum over b in [0, nb) of p0[b * stride], s1 likewise for p1, each added in block
-// order. The loads go out eight records at a time (missing records read as +0, which
-// leaves the sums unchanged), so one memory latency covers eight records.
+// order, from published records. The loads go out eight records at a time (missing
+// records read as +0, which leaves the sums unchanged), so one memory latency covers
+// eight records.
 __device__ __forceinline__ void sum_strided2(const SGMC_GLOBAL float* p0,
                                              const SGMC_GLOBAL float* p1, int64_t stride,
                                              int nb, float& s0, float& s1) {
@@ -314,8 +336,8 @@ __device__ __forceinline__ void sum_strided2(const SGMC_GLOBAL float* p0,
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const bool in = b + t < nb;
-      x0[t] = in ? p0[(int64_t)(b + t) * stride] : 0.0f;
-      x1[t] = in ? p1[(int64_t)(b + t) * stride] : 0.0f;
+      x0[t] = in ? consume(p0 + (int64_t)(b + t) * stride) : 0.0f;
+      x1[t] = in ? consume(p1 + (int64_t)(b + t) * stride) : 0.0f;
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -328,22 +350,25 @@ __device__ __forceinline__ void sum_strided2(const SGMC_GLOBAL float* p0,
 }
 
 // Arrival counter for "last block finishes" reductions: every block calls this after
-// writing its partial results; exactly one (the last to arrive) gets true and sees
-// all partials. The last block resets the counter for the next launch.
+// publishing its partial results; exactly one (the last to arrive) gets true and can
+// consume all of them. The wait makes every published store of the block complete
+// before the block's arrival is counted. The last block resets the counter for the
+// next launch.
 __device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned expected,
                                             int* lds_flag) {
-  __threadfence();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(counter, 1u);
+    const unsigned prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const bool last = (prev == expected - 1);
-    if (last) atomicExch(counter, 0u);
+    if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *lds_flag = last ? 1 : 0;
   }
   __syncthreads();
-  const bool last = *lds_flag != 0;
-  if (last) __threadfence();
-  return last;
+  return *lds_flag != 0;
 }
 
 }  // namespace sgmc

@@ -168,10 +168,10 @@ __global__ void __launch_bounds__(kNT) obabo_o_kernel(float* __restrict__ p,
     ke += q * (im * q);
   });
   const float tot = block_sum<kNT>(ke, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+  if (threadIdx.x == 0) publish(partial + blockIdx.x, tot);
   if (arrive_last(counter, gridDim.x, &flag)) {
     float s = 0.0f;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT) s += partial[b];
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT) s += consume(partial + b);
     s = block_sum<kNT>(s, lds);
     if (threadIdx.x == 0) ke_out[0] = ke_in[0] + 0.5f * s;
   }

@@ -171,6 +171,41 @@ def test_glm_potential(family, D, n, gathered):
     assert ok, f"var rel err {err}"
 
 
+def test_glm_potential_block_handoff_repeatable():
+  """Many blocks hand their partial records to the last block of the launch; every
+  repetition (with unrelated device writes in between) gives bit-identical results that
+  match the oracle."""
+  print("glm_potential handoff", flush=True)
+  N, D, n = 20000, 512, 4096
+  X, y = _glm_data(N, D, L.LIK_LOGISTIC)
+  rng = np.random.default_rng(7)
+  theta = (0.3 * rng.standard_normal(D)).astype(np.float32)
+  idx = rng.integers(0, N, n).astype(np.int32)
+  Xd, ldx = _padded(X)
+  yd = torch.as_tensor(y, device=DEV)
+  idx_d = torch.as_tensor(idx, device=DEV)
+  th = torch.as_tensor(theta, device=DEV)
+  blocks = 1024
+  ws = torch.zeros(int(L.query("sgmc_glm_workspace_floats", n, D, blocks)), device=DEV)
+  U = torch.empty(1, device=DEV)
+  g = torch.empty(D, device=DEV)
+  var = torch.empty(1, device=DEV)
+  junk = torch.empty(1 << 22, device=DEV)
+  outs = []
+  for r in range(24):
+    junk.fill_(float(r))
+    L.call("sgmc_glm_potential", L.LIK_LOGISTIC, L.ptr(Xd), ldx, L.ptr(yd), L.ptr(idx_d), n, N,
+           D, L.ptr(th), 1.0, 1, 1.0, 1.0, L.ptr(U), L.ptr(g), None, L.ptr(var), L.ptr(ws),
+           ws.numel(), blocks, L.stream())
+    outs.append(torch.cat([U, g, var]).clone())
+  for o in outs[1:]:
+    assert torch.equal(o, outs[0])
+  Ur, gr, likr = om.stochastic_potential(om.Family("logistic", X, y), theta, idx, 1.0, 1.0)
+  assert close(U.item(), Ur)[0]
+  assert close(g.cpu().numpy(), gr)[0]
+  assert close(var.item(), np.var(likr), rtol=1e-4)[0]
+
+
 @pytest.mark.parametrize("family", [L.LIK_LINEAR_GAUSSIAN, L.LIK_LOGISTIC])
 @pytest.mark.parametrize("N,D", [(1, 3), (1000, 32), (12345, 512)])
 def test_glm_full_potential_f64(family, N, D):
