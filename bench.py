@@ -14,6 +14,7 @@ both compute the same chain, the roofline of the dominant kernel, and the oracle
 port timed on the host cores as the CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+       python bench.py --copy-calibration   (rocprofv3 counter scaling run)
 """
 import argparse
 import json
@@ -158,6 +159,189 @@ def cpu_baseline(X_host, y_host, keys, dkeys, target_s=12.0):
                     f"(oracle/cport.c SGHMC logistic, OpenMP over chains, {dt:.1f} s)"}
 
 
+# ---------------------------------------------------------------------------------
+# Secondary workloads (python bench.py --workload ...): not the headline line.
+
+LADDER_PROBLEMS, LADDER_RUNGS, LADDER_SWAP_EVERY, LADDER_BATCH = 32, 8, 50, 512
+LADDER_DIMS = (16, 64, 256, 1024)
+MLP_DIMS, MLP_N, MLP_BATCH, MLP_CHAINS = (3072, 512, 512), 50_000, 256, 16
+
+
+def ladder_problems(torch, R):
+  """32 logistic problems: N_k = round(exp(U[ln 2e3, ln 2e5])), D_k cycling 16, 64,
+  256, 1024, X ~ N(0,1)/sqrt(D), w* ~ N(0,1), y ~ Bernoulli(sigmoid(X w*))."""
+  ku, kp = R.split(R.PRNGKey(3), 2)
+  u = R.uniform(ku, (LADDER_PROBLEMS,)).double().cpu().numpy()
+  Ns = np.round(np.exp(np.log(2e3) + u * (np.log(2e5) - np.log(2e3)))).astype(np.int64)
+  pk = R.split(kp, LADDER_PROBLEMS)
+  out = []
+  for k in range(LADDER_PROBLEMS):
+    Dk = LADDER_DIMS[k % len(LADDER_DIMS)]
+    kx, kw, ky = R.split(pk[k].clone(), 3)
+    X = R.normal(kx, (int(Ns[k]) * Dk,)).reshape(int(Ns[k]), Dk).mul_(1.0 / np.sqrt(Dk))
+    w = R.normal(kw, (Dk,))
+    y = (R.uniform(ky, (int(Ns[k]),)) < torch.sigmoid(X @ w)).to(torch.float32)
+    out.append((X, y))
+  return out
+
+
+def run_ladder(args):
+  import torch
+  import __graft_entry__
+  __graft_entry__.load()
+  from jax_sgmc_amd import _lib as L, fused, random as R, scheduler
+  L.require_gpu()
+  log("ladder data")
+  probs = ladder_problems(torch, R)
+  C = LADDER_PROBLEMS * LADDER_RUNGS
+  temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
+  keys = R.key_words(R.split(R.PRNGKey(0), C))
+  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
+  npairs = LADDER_PROBLEMS * (LADDER_RUNGS - 1)
+  skeys = R.key_words(R.split(R.PRNGKey(4), npairs))
+  chains, pairs, ptemps, parity = [], [], [], []
+  for k, (X, y) in enumerate(probs):
+    for j in range(LADDER_RUNGS):
+      c = k * LADDER_RUNGS + j
+      chains.append(dict(X=X, y=y, D=X.shape[1], n=LADDER_BATCH,
+                         theta0=np.zeros(X.shape[1], np.float32), key=tuple(keys[c]),
+                         dkey=tuple(dkeys[c]), temperature=temps[j], save=False))
+      if j + 1 < LADDER_RUNGS:
+        pairs.append((c, c + 1))
+        ptemps.append((temps[j], temps[j + 1]))
+        parity.append(j % 2)
+  K, W = args.steps, args.warmup
+  G = LADDER_SWAP_EVERY
+  total = K + W + 2 * G
+  eps = scheduler.polynomial_table(1e-3, 1e-4, total)
+  eng = fused.FusedGLM(chains, "logistic", "sgld", eps, prior_scale=1.0)
+  tuned = eng.autotune(reps=3)
+  log(f"autotune: {tuned}")
+  ex = fused.ReplicaExchange(eng, pairs, ptemps, keys=[tuple(k) for k in skeys], swap_every=G,
+                             parity=parity)
+  g = eng.capture(G)
+  for _ in range(max(1, W // G)):
+    g.replay()
+  torch.cuda.synchronize()
+  reps = max(1, K // G)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    g.replay()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1)
+  iters = reps * G
+  value = C * iters / (ms / 1e3)
+  bytes_it = sum(LADDER_RUNGS * (LADDER_BATCH * (4 * X.shape[1] + 8) + 8 * X.shape[1])
+                 for X, _ in probs)
+  achieved = bytes_it / (ms / 1e3 / iters) / 1e9
+  out = {"metric": "chain-steps/sec, reSGLD ladders (fused, 1 MI355X)", "value": value,
+         "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
+         "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "fp32",
+         "data": "synthetic (jax-compatible Threefry draws from PRNGKey(3))",
+         "config": {"workload": "reSGLD: 32 logistic problems x 8 rungs (T_j = 20^(j/7)), "
+                                "swap every 50 steps", "chains": C, "batch": LADDER_BATCH,
+                    "N": [int(X.shape[0]) for X, _ in probs],
+                    "D": [int(X.shape[1]) for X, _ in probs],
+                    "step_size": "polynomial 1e-3 -> 1e-4",
+                    "target_bytes_per_block": tuned["target_bytes"], "grid_blocks": eng.nblocks},
+         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                      "kernel": "glm_fused_kernel<-1> + swap_kernel per iteration",
+                      "bytes_per_iteration": bytes_it},
+         "exchanges": int(ex.swaps.sum().item()), "tests": int(ex.step.sum().item())}
+  print(json.dumps(out), flush=True)
+
+
+def mlp_data(torch, R, M):
+  """Inputs U[0,1]^3072 [50000]; labels = argmax of a fixed random teacher MLP."""
+  D_in, H1, H2 = MLP_DIMS
+  kx, kt = R.split(R.PRNGKey(5), 2)
+  X = R.uniform(kx, (MLP_N * D_in,)).reshape(MLP_N, D_in)
+  t = torch.as_tensor(M.init_params(99, D_in, H1, H2), device=X.device)
+  W1, W2, W3, b1, b2, b3 = torch.split(t, [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10])
+  h = torch.relu(X @ W1.view(D_in, H1) + b1)
+  h = torch.relu(h @ W2.view(H1, H2) + b2)
+  labels = torch.argmax(h @ W3.view(H2, 10) + b3, dim=1).to(torch.int32)
+  return X, labels
+
+
+def run_mlp(args):
+  import torch
+  import __graft_entry__
+  __graft_entry__.load()
+  from jax_sgmc_amd import _lib as L, mlp as M, random as R, scheduler
+  L.require_gpu()
+  D_in, H1, H2 = MLP_DIMS
+  X, labels = mlp_data(torch, R, M)
+  C = MLP_CHAINS
+  th0 = np.stack([M.init_params(10 + c, D_in, H1, H2) for c in range(C)])
+  keys = R.key_words(R.split(R.PRNGKey(0), C))
+  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
+  K, W = args.steps, args.warmup
+  G = 10
+  eps = scheduler.polynomial_table(1e-4, 1e-5, K + W + 2 * G)
+  eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0, eps, keys=keys, dkeys=dkeys, rms=True,
+                   prior_scale=1.0)
+  g = eng.capture(G)
+  for _ in range(max(1, W // G)):
+    g.replay()
+  torch.cuda.synchronize()
+  reps = max(1, K // G)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    g.replay()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1)
+  iters = reps * G
+  value = C * iters / (ms / 1e3)
+  macs = D_in * H1 + H1 * H2 + H2 * 10
+  flops = 2 * MLP_BATCH * macs * 2 + 2 * MLP_BATCH * (H1 * H2 + H2 * 10)
+  achieved = C * flops / (ms / 1e3 / iters) / 1e12
+  out = {"metric": "chain-steps/sec, pSGLD Bayesian MLP (fused bf16 MFMA, 1 MI355X)",
+         "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
+         "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "bf16",
+         "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
+         "config": {"workload": "pSGLD (rms_prop 0.9, 1e-5) MLP 3072-512-512-10, N=50000, "
+                                "batch 256, 16 chains", "chains": C, "P": eng.P,
+                    "batch": MLP_BATCH, "N": MLP_N},
+         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0,
+                      "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+                      "kernel": "all kernels of sgmc_mlp_step (GEMM flops per iteration)",
+                      "flops_per_chain_step": flops}}
+  if not args.no_plain:
+    log("plain fp32 torch path (generic potential, one chain)")
+    from jax_sgmc_amd import alias, data as DT, likelihoods as LK, potential as PT
+    loader = DT.DeviceNumpyDataLoader(x=X, y=labels)
+    pot = PT.minibatch_potential(LK.GaussianPrior(1.0), M.MLPSoftmax(), strategy="vmap")
+    run = alias.sgld(pot, loader, cache_size=1, batch_size=MLP_BATCH, rms_prop=True,
+                     first_step_size=1e-4, last_step_size=1e-5, accepted_samples=1,
+                     save_to_numpy=False)
+    p0 = {k: torch.as_tensor(v, device=X.device) for k, v in
+          zip(("W1", "W2", "W3", "b1", "b2", "b3"),
+              np.split(th0[0], np.cumsum([D_in * H1, H1 * H2, H2 * 10, H1, H2])))}
+    p0 = {k: v.reshape(M.leaf_shapes(D_in, H1, H2)[k]) for k, v in p0.items()}
+    run(p0, iterations=3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = 20
+    run(p0, iterations=its)
+    torch.cuda.synchronize()
+    plain = its / (time.perf_counter() - t0)
+    out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
+                    "path": "generic torch fp32 potential (vmap + autograd) with the "
+                            "alias.sgld rms_prop loop"}
+    out["fused_over_plain"] = value / plain
+  print(json.dumps(out), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -165,13 +349,32 @@ def main():
   ap.add_argument("--warmup", type=int, default=50)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--copy-calibration", action="store_true",
+                  help="only copy a 512 MiB device buffer 5 times (the known-size kernel "
+                       "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+  ap.add_argument("--workload", default="sghmc-logistic",
+                  choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp"])
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
           "(chains shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
     sys.exit(2)
 
+  if args.workload == "resgld-ladder":
+    return run_ladder(args)
+  if args.workload == "psgld-mlp":
+    return run_mlp(args)
   import torch
+  if args.copy_calibration:
+    n = 128 << 20
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+      b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "launches": 5}), flush=True)
+    return
   import __graft_entry__
   __graft_entry__.load()
   from jax_sgmc_amd import _lib as L, fused, random as R, tree as T

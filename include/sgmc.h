@@ -203,6 +203,39 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                         const float* sqrt_mass, const float* eps_table, int table_len,
                         const int32_t* save_slot, int32_t* iter, unsigned* done, void* stream);
 
+/* ---------------------------------------------------- replica exchange ----
+ * parallel_tempering's swap step (solver.py:264-293, alias.py:122-207) for pairs of
+ * chains of a fused chain table, run after the chains' update of iteration
+ * it = iter[0] - 1 (iter as advanced by sgmc_glm_fused_step). For each tested pair:
+ *   step += 1; ssq = (1 - 1/step) ssq + (1/step) var(lo);
+ *   temps = 1/t_lo - 1/t_hi; log_s = temps ((U(lo) - U(hi)) - temps ssq / F);
+ *   (key, sub) = split(key); the pair is kept when log(uniform(sub)) < log_s and
+ *   otherwise the two chains exchange their states (theta, momentum, rms_prop state,
+ *   integrator and data-loader keys, U, var); the slot temperatures stay.
+ * A pair is tested after iteration it when (it + 1) % swap_every == 0 and its parity
+ * is -1 or equals ((it + 1) / swap_every - 1) % 2 (alternating even / odd rungs of a
+ * ladder; swap_every = 1 with parity -1 is the reference's two-chain solver). Tested
+ * pairs of one call must not share a chain. After the test, a pair with samples !=
+ * NULL saves the lo chain's theta and U at save_slot[it] when that is >= 0 (the
+ * reference saves the normal chain after the exchange). pairs, save_slot and iter are
+ * device memory; lo and hi must have the same D. */
+typedef struct sgmc_swap_pair {
+  int32_t lo, hi;      /* chain indices: lo runs the colder schedule */
+  float t_lo, t_hi;    /* schedule temperatures of the two slots */
+  int32_t parity;      /* -1, 0 or 1 (see above) */
+  int32_t reserved;
+  float* ssq;          /* [1] running estimate of var(lik) of the lo chain, init 0 */
+  int32_t* step;       /* [1] tests so far, init 0 */
+  uint32_t* key;       /* [2] swap key, advanced in place */
+  int32_t* swaps;      /* [1] exchanges so far, or NULL */
+  float* samples;      /* [slots][D] saved lo-chain samples, or NULL */
+  float* samples_U;    /* [slots] their potentials, or NULL */
+} sgmc_swap_pair;
+
+int sgmc_resgld_swap(const sgmc_glm_chain* chains, int nchains, const sgmc_swap_pair* pairs,
+                     int npairs, float F, int swap_every, const int32_t* save_slot,
+                     int table_len, const int32_t* iter, void* stream);
+
 /* ------------------------------------------------------------ MLP (MFMA) ----
  * Bayesian MLP classifier x -> relu(x W1 + b1) -> relu(. W2 + b2) -> . W3 + b3 with a
  * 10-class softmax cross-entropy likelihood: the potential and its gradient
@@ -243,12 +276,44 @@ typedef struct sgmc_mlp_state {
   float alpha, lmbd, temperature, noise_temperature;
   int32_t has_prior;
   float prior_scale;
+  /* OBABO (obabo.integrate, integrator.py:203-338, identity mass): mode 1 runs the
+   * first half of one step (O: momentum refresh with xi(s1); B with the gradient at
+   * theta; A), mode 2 the second half (B with the gradient at the moved theta; O with
+   * xi(s2)); each half draws its own minibatch. mode 0 is the SGLD / rms_prop step. */
+  int32_t mode;
+  float* mom;              /* [C][P] momentum (modes 1, 2) */
+  double* ke;              /* [C][2] kinetic energy accumulated at start / end */
+  float* U_half;           /* [C] potential of the first evaluation of the step */
+  const float* decay_table;  /* [table_len] sqrt(exp(-friction eps)) */
+  const float* noise_table;  /* [table_len] sqrt((1 - exp(-friction eps)) T) */
+  /* SGGMC Metropolis-Hastings (solver.py:500-567), see sgmc_mlp_mh */
+  double* U_full;          /* [C] full-data potential of the current state */
+  double* U_prop;          /* [C] full-data potential of the proposal */
+  float* theta_old;        /* [C][P] state the proposal started from */
+  float* mom_old;          /* [C][P] */
+  float* U_old;            /* [C] its potential (ObaboState.potential) */
+  uint32_t* mh_key;        /* [C][2] the solver's key */
+  float* mh_stats;         /* [C][n_slots][3] acceptance ratio, step size, KE_end - KE_start
+                              at the saved iterations, or NULL */
+  int32_t* accepted;       /* [C] accepted proposals, or NULL */
 } sgmc_mlp_state;
 
 int64_t sgmc_mlp_param_count(int D_in, int H1, int H2);
 int64_t sgmc_mlp_workspace_bytes(int C, int n, int D_in, int H1, int H2);
 int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* stream);
 int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream);
+/* full_potential (potential.py:219-293) of every chain's current theta: the
+ * likelihood of all N rows, n rows per pass, summed in float64 with the float64 log
+ * prior: out[c] = (-sum_i lik_i - log p(theta_c)) / temperature. */
+int sgmc_mlp_full_potential(const sgmc_mlp_state* st, double* out, void* stream);
+/* sggmc's MH step after an OBABO proposal (solver.py:500-567), U_prop holding the
+ * proposal's full potential: log a = min(0, -(U_prop - U_full + KE_end - KE_start) / T);
+ * (mh_key, sub) = split(mh_key); accepted iff log(uniform(sub)) < log a. A rejected
+ * proposal restores theta, momentum and the step potential from the *_old buffers
+ * (keys and data keys keep their advanced values); then *_old take the chain's state,
+ * the kinetic energies reset to 0, the sample is saved at save_slot[it], and the
+ * iteration counter advances. */
+int sgmc_mlp_mh(const sgmc_mlp_state* st, void* stream);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,15 @@ class GlmChain(ctypes.Structure):
   ]
 
 
+class SwapPair(ctypes.Structure):
+  """Mirror of ``sgmc_swap_pair`` (include/sgmc.h)."""
+  _fields_ = [
+      ("lo", c_int32), ("hi", c_int32), ("t_lo", c_float), ("t_hi", c_float),
+      ("parity", c_int32), ("reserved", c_int32), ("ssq", c_void_p), ("step", c_void_p),
+      ("key", c_void_p), ("swaps", c_void_p), ("samples", c_void_p), ("samples_U", c_void_p),
+  ]
+
+
 class MlpState(ctypes.Structure):
   """Mirror of ``sgmc_mlp_state`` (include/sgmc.h)."""
   _fields_ = [
@@ -53,6 +62,11 @@ class MlpState(ctypes.Structure):
       ("iter", c_void_p), ("counters", c_void_p), ("done", c_void_p), ("table_len", c_int32),
       ("rms", c_int32), ("alpha", c_float), ("lmbd", c_float), ("temperature", c_float),
       ("noise_temperature", c_float), ("has_prior", c_int32), ("prior_scale", c_float),
+      ("mode", c_int32), ("mom", c_void_p), ("ke", c_void_p), ("U_half", c_void_p),
+      ("decay_table", c_void_p), ("noise_table", c_void_p), ("U_full", c_void_p),
+      ("U_prop", c_void_p), ("theta_old", c_void_p), ("mom_old", c_void_p),
+      ("U_old", c_void_p), ("mh_key", c_void_p), ("mh_stats", c_void_p),
+      ("accepted", c_void_p),
   ]
 
 
@@ -97,6 +111,10 @@ _SIGNATURES = {
                                     c_int, c_float, c_float, c_int, c_int, c_int, c_float,
                                     c_float, c_float, c_void_p, c_void_p, c_void_p, c_int,
                                     c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sgmc_resgld_swap": (c_int, [c_void_p, c_int, c_void_p, c_int, c_float, c_int, c_void_p,
+                                 c_int, c_void_p, c_void_p]),
+    "sgmc_mlp_full_potential": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sgmc_mlp_mh": (c_int, [c_void_p, c_void_p]),
     "sgmc_mlp_param_count": (c_int64, [c_int, c_int, c_int]),
     "sgmc_mlp_workspace_bytes": (c_int64, [c_int] * 5),
     "sgmc_mlp_shadow": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),

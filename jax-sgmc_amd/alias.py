@@ -114,6 +114,59 @@ def _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, 
   return out
 
 
+def _run_fused_resgld(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                      tau, keys, data_keys, to_numpy=True):
+  """alias.re_sgld on the fused engine: chain 2c runs the normal schedule of pair c,
+  chain 2c + 1 the hot one (temperature tau); after every iteration the swap kernel
+  tests each pair and saves the normal slot (solver.py:244-293)."""
+  init_fn = sched[0]
+  st, _ = init_fn(iterations)
+  eps, temp, keep = init_fn.tables(st)
+  slots = np.full(iterations, -1, np.int32)
+  kept = np.nonzero(keep)[0]
+  slots[kept] = np.arange(kept.size, dtype=np.int32)
+  lik = potential_fn.likelihood
+  X = data_loader.reference_data[lik.x]
+  y = data_loader.reference_data[lik.y].to(torch.float32).contiguous()
+  has_prior, pscale = PT._prior_spec(potential_fn.prior)
+  t_normal = float(temp[0]) if temp.size else 1.0
+  dev = X.device
+  chains, pairs, temps, swap_keys = [], [], [], []
+  for c, (normal, hot) in enumerate(init_samples):
+    key = keys[c] if keys else None
+    key = R.key_from_words(_key_words(key), device=dev)
+    ks = R.key_words(R.split(key, 3))  # (swap key, normal key, hot key), solver.py:254
+    dk = _key_words(data_keys[c] if data_keys else None)
+    for s, k, t in ((normal, ks[1], t_normal), (hot, ks[2], float(tau))):
+      leaf = T.flatten(s)[0][0]
+      th0 = leaf.detach().cpu().numpy() if isinstance(leaf, torch.Tensor) else np.asarray(leaf)
+      chains.append(dict(X=X, y=y, D=X.shape[1], n=batch_size,
+                         theta0=np.asarray(th0, np.float32).reshape(-1), key=tuple(k),
+                         dkey=dk, temperature=t, save=False))
+    pairs.append((2 * c, 2 * c + 1))
+    temps.append((t_normal, float(tau)))
+    swap_keys.append(tuple(ks[0]))
+  from . import fused as FU
+  eng = FU.FusedGLM(chains, lik.family, "sgld", eps, sigma=lik.sigma, prior_scale=pscale,
+                    has_prior=bool(has_prior), pot_temperature=potential_fn.temperature,
+                    save_slots=slots, n_slots=max(1, kept.size))
+  ex = FU.ReplicaExchange(eng, pairs, temps, keys=swap_keys, save=[True] * len(pairs))
+  eng.run(iterations)
+  torch.cuda.synchronize()
+  out = []
+  for c, (normal, _) in enumerate(init_samples):
+    tr = ex.sample_trace(c)[:kept.size]
+    Us = ex.sample_potentials(c)[:kept.size]
+    _, td = T.flatten(normal)
+    samples = {"variables": T.unflatten(td, [tr.clone()]), "likelihood": (-Us).clone(),
+               "model_state": None}
+    if to_numpy:
+      samples = T.tree_map(lambda x: x.cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                           samples)
+    out.append({"sample_count": int(kept.size), "samples": samples})
+  return out
+
+
 def sgld(potential_fn, data_loader, cache_size=512, batch_size=32, first_step_size=0.05,
          last_step_size=0.001, burn_in=0, accepted_samples=1000, rms_prop=False, alpha=0.9,
          lmbd=1e-5, save_to_numpy=True, progress_bar=True):
@@ -223,8 +276,21 @@ def re_sgld(potential_fn, data_loader, cache_size=512, batch_size=32, temperatur
   solver = SV.parallel_tempering(integ)
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
-  def run_fn(*init_samples, init_model_state=None, iterations=1000):
-    states = [solver[0](a, b, init_model_state=init_model_state) for a, b in init_samples]
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
+             data_keys=None, fused=None):
+    flat = [s for pair in init_samples for s in pair]
+    if init_model_state is None and _fused_eligible(potential_fn, data_loader, flat, fused):
+      return _run_fused_resgld(potential_fn, data_loader, batch_size, init_samples,
+                               iterations, sched, temperature, keys, data_keys,
+                               to_numpy=save_to_numpy)
+    states = []
+    for c, (a, b) in enumerate(init_samples):
+      kw = dict(init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      if data_keys:
+        kw["batch_kwargs"] = {"key": data_keys[c]}
+      states.append(solver[0](a, b, **kw))
     return mcmc(*states, iterations=iterations,
                 schedulers=[{"temperature": {"tau": temperature}}])
 

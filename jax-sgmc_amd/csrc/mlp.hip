@@ -95,7 +95,20 @@ struct Step {
   float alpha, lmbd, temperature, noise_temperature;
   float prior_inv_var, prior_log_norm;
   int has_prior;
+  // OBABO (update != 0, mode != MLP_SGLD)
+  int mode;
+  float* mom;              // [C][P] momentum
+  float* ke_part;          // [C][nparts] kinetic-energy partials of one evaluation
+  double* ke;              // [C][2] accumulated kinetic energy at start / end
+  float* U_half;           // [C] potential of the step's first evaluation
+  const float* decay_table;  // [table_len] sqrt(exp(-friction eps))
+  const float* noise_table;  // [table_len] sqrt((1 - exp(-friction eps)) T)
+  // full-data forward pass (likelihoods of rows row0 .. row0 + n - 1 only)
+  int full;
+  int64_t row0;
 };
+
+enum MlpMode { MLP_SGLD = 0, MLP_OBABO_BA = 1, MLP_OBABO_BO = 2 };
 
 __device__ __forceinline__ int cur_iter(const Step& s) {
   return max(0, min(s.iter[0], s.table_len - 1));
@@ -139,10 +152,74 @@ __device__ __forceinline__ void grad_param(const Step& s, const float* th, float
   gout[i] = g / s.temperature;
 }
 
+// Per-kernel step coefficients of the current iteration.
+struct Coef {
+  float eps, noise_scale, decay, noise;
+};
+
+__device__ __forceinline__ Coef coef_of(const Step& s) {
+  const int it = cur_iter(s);
+  Coef k;
+  k.eps = s.eps_table[it];
+  k.noise_scale = sqrtf((2.0f * s.noise_temperature) * k.eps);
+  k.decay = s.decay_table ? s.decay_table[it] : 0.0f;
+  k.noise = s.noise_table ? s.noise_table[it] : 0.0f;
+  return k;
+}
+
+// OBABO halves for one parameter with the identity mass (integrator.py:181-273):
+//   BA: p1 = sqrt(pi) p + sqrt((1 - pi) T) xi(s1); KE_start += p1 p1 / 2;
+//       p2 = (-eps/2) g(theta) + p1; theta' = theta + eps p2
+//   BO: p3 = (-eps/2) g(theta') + p2; KE_end += p3 p3 / 2; p' = sqrt(pi) p3 + ... xi(s2)
+// with g the potential's gradient at the evaluation point (prior and temperature in).
+__device__ __forceinline__ void obabo_param(const Step& s, int64_t base, int64_t i, float gsum,
+                                            float xi, const Coef& k, float& lp, float& ke) {
+  float* th = s.theta + base;
+  float* mom = s.mom + base;
+  const float w = th[i];
+  float g = gsum;
+  if (s.has_prior) {
+    g = g + w * s.prior_inv_var;
+    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
+  }
+  g = g / s.temperature;
+  const float half = 0.5f * k.eps;
+  if (s.mode == MLP_OBABO_BA) {
+    const float p1 = k.decay * mom[i] + k.noise * xi;
+    ke += p1 * p1;
+    const float p2 = (-half) * g + p1;
+    mom[i] = p2;
+    const float wn = w + k.eps * p2;
+    th[i] = wn;
+    s.shadow[base + i] = f2bf(wn);
+  } else {
+    const float p3 = (-half) * g + mom[i];
+    ke += p3 * p3;
+    mom[i] = k.decay * p3 + k.noise * xi;
+  }
+}
+
+__device__ __forceinline__ void update_param(const Step& s, int64_t base, int64_t i, float gsum,
+                                             float xi, const Coef& k, float& lp, float& ke) {
+  if (s.mode == MLP_SGLD)
+    sgld_param(s, s.theta + base, s.v ? s.v + base : nullptr, s.shadow + base, i, gsum, xi,
+               k.eps, k.noise_scale, lp);
+  else
+    obabo_param(s, base, i, gsum, xi, k, lp, ke);
+}
+
+// Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
+// (key, sub) = split(key) for SGLD; s1 / s2 of (key, s1, s2) = split(key, 3) for the
+// two momentum refreshes of an OBABO step.
 __device__ __forceinline__ Key leaf_key(const Step& s, int chain, int leaf) {
-  Key next, sub;
-  split2(load_key(s.key + 2 * chain), next, sub);
-  return split_key(sub, NLEAF, (uint32_t)leaf);
+  const Key k = load_key(s.key + 2 * chain);
+  Key a, b, c;
+  if (s.mode == MLP_SGLD) {
+    split2(k, a, b);
+    return split_key(b, NLEAF, (uint32_t)leaf);
+  }
+  split3(k, a, b, c);
+  return split_key(s.mode == MLP_OBABO_BA ? b : c, NLEAF, (uint32_t)leaf);
 }
 
 // ------------------------------------------------------------------ gather ----
@@ -152,21 +229,30 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
   const int j = blockIdx.x * (GT / kWave) + threadIdx.x / kWave;
   if (j >= s.n) return;
   const int lane = threadIdx.x & (kWave - 1);
-  Key next, sub, k1, k2;
-  split2(load_key(s.dkey + 2 * c), next, sub);
-  split2(sub, k1, k2);
-  const uint32_t span = (uint32_t)s.N;
-  int64_t r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, randint_mult(span));
-  if (r < 0 || r >= s.N) r = 0;
+  int64_t r;
+  bool valid = true;
+  if (s.full) {  // rows row0 + j of the full-data pass; rows past N are zero
+    r = s.row0 + j;
+    valid = r < s.N;
+    if (!valid) r = 0;
+  } else {
+    Key next, sub, k1, k2;
+    split2(load_key(s.dkey + 2 * c), next, sub);
+    split2(sub, k1, k2);
+    const uint32_t span = (uint32_t)s.N;
+    r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, randint_mult(span));
+    if (r < 0 || r >= s.N) r = 0;
+  }
   const float* src = s.X + r * s.ldx;
   uint16_t* dst = s.Xb + ((int64_t)c * s.n + j) * s.D_in;
   for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
-    const float4 x = *reinterpret_cast<const float4*>(src + f);
+    float4 x = *reinterpret_cast<const float4*>(src + f);
+    if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t lo = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
     const uint32_t hi = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
     *reinterpret_cast<uint2*>(dst + f) = make_uint2(lo, hi);
   }
-  if (lane == 0) s.lab[(int64_t)c * s.n + j] = s.labels[r];
+  if (lane == 0) s.lab[(int64_t)c * s.n + j] = valid ? s.labels[r] : -1;
 }
 
 // ------------------------------------------------------------------ GEMM ------
@@ -300,16 +386,12 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
   } else {
     // Parameter epilogue: element (m, n) of the [M][N] leaf at flat i = m N + n, its
     // noise pair i + M N / 2 sits in acc1 at the same lane and register.
-    const int it = cur_iter(s);
-    const float eps = s.eps_table[it];
-    const float noise_scale = sqrtf((2.0f * s.noise_temperature) * eps);
+    const Coef k = coef_of(s);
     const int64_t base = (int64_t)c * s.P + s.off[g.leaf];
-    float* th = s.theta + base;
-    float* vv = s.v ? s.v + base : nullptr;
-    uint16_t* sh = s.shadow + base;
+    const float* th = s.theta + base;
     const uint32_t S = (uint32_t)((int64_t)g.M * g.N);
     const Key lk = leaf_key(s, c, g.leaf);
-    float lp = 0.0f;
+    float lp = 0.0f, ke = 0.0f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
@@ -318,17 +400,21 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       if (EPI == EPI_UPDATE) {
         uint32_t w0, w1;
         bits_pair(lk, S, (uint32_t)i0, w0, w1);
-        sgld_param(s, th, vv, sh, i0, acc0[j], bits_to_normal(w0), eps, noise_scale, lp);
-        sgld_param(s, th, vv, sh, i1, acc1[j], bits_to_normal(w1), eps, noise_scale, lp);
+        update_param(s, base, i0, acc0[j], bits_to_normal(w0), k, lp, ke);
+        update_param(s, base, i1, acc1[j], bits_to_normal(w1), k, lp, ke);
       } else {
         float* go = s.grad + base;
         grad_param(s, th, go, i0, acc0[j], lp);
         grad_param(s, th, go, i1, acc1[j], lp);
       }
     }
+    const int64_t slot = (int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn;
     const float tot = block_sum<GT>(lp, red);
-    if (threadIdx.x == 0)
-      s.prior_part[(int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn] = tot;
+    if (threadIdx.x == 0) s.prior_part[slot] = tot;
+    if (EPI == EPI_UPDATE && s.mode != MLP_SGLD) {
+      const float kt = block_sum<GT>(ke, red);
+      if (threadIdx.x == 0) s.ke_part[slot] = kt;
+    }
   }
 }
 
@@ -362,6 +448,15 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
   for (int k = 0; k < NCLS; ++k) se += expf(z[k] - mx);
   const float lse = mx + logf(se);
   int lab = s.lab[(int64_t)c * s.n + j];
+  if (s.full) {  // likelihood only; padding rows (label -1) contribute 0
+    const int lb = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
+    float zl = z[0];
+#pragma unroll
+    for (int k = 1; k < NCLS; ++k)
+      if (k == lb) zl = z[k];
+    if (lane == 0) s.lik[(int64_t)c * s.n + j] = lab < 0 ? 0.0f : zl - lse;
+    return;
+  }
   lab = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
   const float cscale = ((float)(-s.N) / (float)s.n) / s.temperature;
   float dl[NCLS];
@@ -393,15 +488,15 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
 // once every chain is done).
 __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
   __shared__ float red[GT / kWave];
+  __shared__ double red_d[GT / kWave];
   __shared__ int flag;
   const int c = blockIdx.y;
   const int64_t nW3 = (int64_t)s.H2 * NCLS;
   const int64_t total = nW3 + s.H1 + s.H2 + NCLS;
   const int64_t p = (int64_t)blockIdx.x * GT + threadIdx.x;
   const int it = cur_iter(s);
-  const float eps = s.eps_table[it];
-  const float noise_scale = sqrtf((2.0f * s.noise_temperature) * eps);
-  float lp = 0.f;
+  const Coef cf = coef_of(s);
+  float lp = 0.f, ke = 0.f;
   if (p < total) {
     int leaf;
     int64_t i;
@@ -430,15 +525,26 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
     if (s.update) {
       const Key lk = leaf_key(s, c, leaf);
       const float xi = bits_to_normal(bits_word(lk, (uint32_t)s.size[leaf], (uint32_t)i));
-      sgld_param(s, s.theta + base, s.v ? s.v + base : nullptr, s.shadow + base, i, gsum, xi,
-                 eps, noise_scale, lp);
+      update_param(s, base, i, gsum, xi, cf, lp, ke);
     } else {
       grad_param(s, s.theta + base, s.grad + base, i, gsum, lp);
     }
   }
+  const bool obabo = s.update && s.mode != MLP_SGLD;
+  const int64_t slot = (int64_t)c * s.nparts + part0 + blockIdx.x;
   const float tot = block_sum<GT>(lp, red);
-  if (threadIdx.x == 0) publish(s.prior_part + (int64_t)c * s.nparts + part0 + blockIdx.x, tot);
+  if (threadIdx.x == 0) publish(s.prior_part + slot, tot);
+  if (obabo) {
+    const float kt = block_sum<GT>(ke, red);
+    if (threadIdx.x == 0) publish(s.ke_part + slot, kt);
+  }
   if (!arrive_last(s.counters + c, gridDim.x, &flag)) return;
+  double kin = 0.0;
+  if (obabo) {
+    for (int q = threadIdx.x; q < s.nparts; q += GT)
+      kin += (double)consume(s.ke_part + (int64_t)c * s.nparts + q);
+    kin = block_sum_d<GT>(kin, red_d);
+  }
   float lsum = 0.f;
   for (int r = threadIdx.x; r < s.n; r += GT) lsum += s.lik[(int64_t)c * s.n + r];
   lsum = block_sum<GT>(lsum, red);
@@ -449,10 +555,28 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
   if (threadIdx.x == 0) {
     const float mean = lsum / (float)s.n;
     const float Uc = ((float)(-s.N) * mean - prior) / s.temperature;
-    s.U[c] = Uc;
-    if (s.update) {
-      const int slot = s.save_slot ? s.save_slot[it] : -1;
-      if (slot >= 0 && slot < s.n_slots && s.samples_U) s.samples_U[(int64_t)c * s.n_slots + slot] = Uc;
+    if (obabo) {
+      // ObaboState bookkeeping (integrator.py:203-273): the step's potential is the mean
+      // of its two evaluations; the key advances once per step, the data key per draw.
+      Key next, sub;
+      split2(load_key(s.dkey + 2 * c), next, sub);
+      store_key(s.dkey + 2 * c, next);
+      if (s.mode == MLP_OBABO_BA) {
+        s.U_half[c] = Uc;
+        s.ke[2 * c] += 0.5 * kin;
+      } else {
+        s.U[c] = 0.5f * (s.U_half[c] + Uc);
+        s.ke[2 * c + 1] += 0.5 * kin;
+        Key a, b, cc;
+        split3(load_key(s.key + 2 * c), a, b, cc);
+        store_key(s.key + 2 * c, a);
+      }
+    } else {
+      s.U[c] = Uc;
+    }
+    if (s.update && !obabo) {
+      const int ss = s.save_slot ? s.save_slot[it] : -1;
+      if (ss >= 0 && ss < s.n_slots && s.samples_U) s.samples_U[(int64_t)c * s.n_slots + ss] = Uc;
       Key next, sub;
       split2(load_key(s.key + 2 * c), next, sub);
       store_key(s.key + 2 * c, next);
@@ -485,6 +609,115 @@ __global__ void __launch_bounds__(GT) shadow_kernel(const float* th, uint16_t* s
     sh[i] = f2bf(th[i]);
 }
 
+// ------------------------------------------------------ full-data potential ----
+__global__ void __launch_bounds__(GT) zero_d_kernel(double* a, int n) {
+  const int i = blockIdx.x * GT + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// acc[c] += sum of the pass's per-row likelihoods in float64 (fixed order).
+__global__ void __launch_bounds__(GT) full_acc_kernel(Step s, double* acc) {
+  __shared__ double red[GT / kWave];
+  const int c = blockIdx.x;
+  double t = 0.0;
+  for (int r = threadIdx.x; r < s.n; r += GT) t += (double)s.lik[(int64_t)c * s.n + r];
+  t = block_sum_d<GT>(t, red);
+  if (threadIdx.x == 0) acc[c] += t;
+}
+
+// out[c] = (-acc[c] - log p(theta_c)) / T with the float64 Gaussian log prior.
+__global__ void __launch_bounds__(GT) full_final_kernel(Step s, const double* acc, double* out) {
+  __shared__ double red[GT / kWave];
+  const int c = blockIdx.x;
+  double lp = 0.0;
+  if (s.has_prior) {
+    const double iv = (double)s.prior_inv_var, ln = (double)s.prior_log_norm;
+    const float* th = s.theta + (int64_t)c * s.P;
+    for (int64_t i = threadIdx.x; i < s.P; i += GT) {
+      const double w = (double)th[i];
+      lp += (ln + w * w * iv) / -2.0;
+    }
+  }
+  lp = block_sum_d<GT>(lp, red);
+  if (threadIdx.x == 0) out[c] = (-acc[c] - lp) / (double)s.temperature;
+}
+
+// ----------------------------------------------------------------- SGGMC MH ----
+struct MH {
+  double* U_full;
+  const double* U_prop;
+  float* theta_old;
+  float* mom_old;
+  float* U_old;
+  uint32_t* key;
+  float* stats;
+  int32_t* accepted;
+};
+
+// Every block of chain c evaluates the same decision from inputs that only the chain's
+// last-arriving block rewrites, applies it to its slice of the parameters, and saves
+// the slice at a kept iteration.
+__global__ void __launch_bounds__(GT) mh_kernel(Step s, MH m) {
+  __shared__ int flag;
+  const int c = blockIdx.y;
+  const int it = cur_iter(s);
+  const float T = s.noise_temperature;
+  const double dE = (m.U_prop[c] - m.U_full[c]) + (s.ke[2 * c + 1] - s.ke[2 * c]);
+  double log_alpha = -1.0 / (double)T * dE;
+  log_alpha = log_alpha <= 0.0 ? log_alpha : 0.0;
+  Key next, sub;
+  split2(load_key(m.key + 2 * c), next, sub);
+  const float u = bits_to_uniform(bits_word(sub, 1u, 0u), 0.0f, 1.0f);
+  const bool accept = (double)logf_cephes(u) < log_alpha;
+  const int slot = s.save_slot ? s.save_slot[it] : -1;
+  const bool save = s.samples && slot >= 0 && slot < s.n_slots;
+  float* th = s.theta + (int64_t)c * s.P;
+  float* mo = s.mom + (int64_t)c * s.P;
+  float* tho = m.theta_old + (int64_t)c * s.P;
+  float* moo = m.mom_old + (int64_t)c * s.P;
+  uint16_t* sh = s.shadow + (int64_t)c * s.P;
+  float* smp = save ? s.samples + ((int64_t)c * s.n_slots + slot) * s.P : nullptr;
+  for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.P;
+       i += (int64_t)gridDim.x * GT) {
+    float w;
+    if (accept) {
+      w = th[i];
+      tho[i] = w;
+      moo[i] = mo[i];
+    } else {
+      w = tho[i];
+      th[i] = w;
+      mo[i] = moo[i];
+      sh[i] = f2bf(w);
+    }
+    if (save) smp[i] = w;
+  }
+  if (!arrive_last(s.counters + c, gridDim.x, &flag)) return;
+  if (threadIdx.x == 0) {
+    const float U = accept ? s.U[c] : m.U_old[c];
+    s.U[c] = U;
+    m.U_old[c] = U;
+    if (accept) m.U_full[c] = m.U_prop[c];
+    const double dk = s.ke[2 * c + 1] - s.ke[2 * c];
+    s.ke[2 * c] = 0.0;
+    s.ke[2 * c + 1] = 0.0;
+    store_key(m.key + 2 * c, next);
+    if (m.accepted && accept) m.accepted[c] += 1;
+    if (save && s.samples_U) s.samples_U[(int64_t)c * s.n_slots + slot] = U;
+    if (save && m.stats) {
+      float* st = m.stats + ((int64_t)c * s.n_slots + slot) * 3;
+      st[0] = (float)exp(log_alpha);
+      st[1] = s.eps_table[it];
+      st[2] = (float)dk;
+    }
+    const unsigned prev = atomicAdd(s.done, 1u);
+    if (prev == (unsigned)s.C - 1) {
+      atomicExch(s.done, 0u);
+      atomicAdd(s.iter, 1);
+    }
+  }
+}
+
 }  // namespace mlp
 }  // namespace sgmc
 
@@ -494,7 +727,7 @@ using namespace sgmc::mlp;
 namespace {
 
 struct Layout {
-  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, total;
+  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, ke, facc, total;
   int nparts, g4_blocks, g5_blocks, small_blocks;
 };
 
@@ -517,6 +750,8 @@ Layout layout_of(int C, int n, int D_in, int H1, int H2) {
   L.small_blocks = (int)((small_total + GT - 1) / GT);
   L.nparts = L.g4_blocks + L.g5_blocks + L.small_blocks;
   L.prior = o; o = align256(o + (int64_t)C * L.nparts * 4);
+  L.ke = o; o = align256(o + (int64_t)C * L.nparts * 4);
+  L.facc = o; o = align256(o + (int64_t)C * 8);
   L.total = o;
   return L;
 }
@@ -542,30 +777,38 @@ int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* s
   return check_launch("sgmc_mlp_shadow");
 }
 
-int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
-  SGMC_REQUIRE(st, SGMC_E_ARG, "sgmc_mlp_step: null state");
+}  // extern "C"
+
+namespace {
+
+// Validates the state and fills the kernels' view of it.
+int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
+  SGMC_REQUIRE(st, SGMC_E_ARG, "sgmc_mlp: null state");
   const sgmc_mlp_state& q = *st;
   SGMC_REQUIRE(q.X && q.labels && q.theta && q.shadow && q.key && q.dkey && q.work && q.U &&
                    q.eps_table && q.iter && q.counters && q.done,
-               SGMC_E_ARG, "sgmc_mlp_step: null pointer in state");
+               SGMC_E_ARG, "sgmc_mlp: null pointer in state");
   SGMC_REQUIRE(update || q.grad, SGMC_E_ARG, "sgmc_mlp_step: gradient mode needs grad");
   SGMC_REQUIRE(q.C >= 1 && q.n >= 1 && q.N >= 1 && q.N <= 2147483647LL, SGMC_E_ARG,
-               "sgmc_mlp_step: bad C/n/N");
+               "sgmc_mlp: bad C/n/N");
   SGMC_REQUIRE(q.n % BM == 0 && q.D_in % BM == 0 && q.H1 % BM == 0, SGMC_E_ARG,
-               "sgmc_mlp_step: n, D_in and H1 must be multiples of %d", BM);
+               "sgmc_mlp: n, D_in and H1 must be multiples of %d", BM);
   SGMC_REQUIRE(q.H1 % BN == 0 && q.H2 % BN == 0, SGMC_E_ARG,
-               "sgmc_mlp_step: H1 and H2 must be multiples of %d", BN);
+               "sgmc_mlp: H1 and H2 must be multiples of %d", BN);
   SGMC_REQUIRE(q.ldx % 4 == 0 && q.ldx >= q.D_in && ((uintptr_t)q.X & 15) == 0, SGMC_E_ARG,
-               "sgmc_mlp_step: X rows must be 16-byte aligned with ldx %% 4 == 0");
+               "sgmc_mlp: X rows must be 16-byte aligned with ldx %% 4 == 0");
   SGMC_REQUIRE(q.P == sgmc_mlp_param_count(q.D_in, q.H1, q.H2), SGMC_E_ARG,
-               "sgmc_mlp_step: P does not match the layer sizes");
+               "sgmc_mlp: P does not match the layer sizes");
   SGMC_REQUIRE(q.table_len >= 1 && q.temperature != 0.0f, SGMC_E_ARG,
-               "sgmc_mlp_step: bad schedule");
-  SGMC_REQUIRE((int64_t)q.D_in * q.H1 < 4294967295LL, SGMC_E_ARG, "sgmc_mlp_step: W1 too large");
-  const Layout L = layout_of(q.C, q.n, q.D_in, q.H1, q.H2);
-  SGMC_REQUIRE(q.work_bytes >= L.total, SGMC_E_ARG, "sgmc_mlp_step: workspace too small");
+               "sgmc_mlp: bad schedule");
+  SGMC_REQUIRE((int64_t)q.D_in * q.H1 < 4294967295LL, SGMC_E_ARG, "sgmc_mlp: W1 too large");
+  SGMC_REQUIRE(q.mode >= MLP_SGLD && q.mode <= MLP_OBABO_BO, SGMC_E_ARG, "sgmc_mlp: bad mode");
+  if (update && q.mode != MLP_SGLD)
+    SGMC_REQUIRE(q.mom && q.ke && q.U_half && q.decay_table && q.noise_table, SGMC_E_ARG,
+                 "sgmc_mlp_step: OBABO needs mom, ke, U_half and the decay / noise tables");
+  L = layout_of(q.C, q.n, q.D_in, q.H1, q.H2);
+  SGMC_REQUIRE(q.work_bytes >= L.total, SGMC_E_ARG, "sgmc_mlp: workspace too small");
   uint8_t* w = static_cast<uint8_t*>(q.work);
-  Step s;
   s.X = q.X; s.labels = q.labels; s.N = q.N; s.ldx = q.ldx;
   s.D_in = q.D_in; s.H1 = q.H1; s.H2 = q.H2; s.n = q.n; s.C = q.C;
   s.theta = q.theta; s.v = q.rms ? q.v : nullptr; s.shadow = q.shadow; s.P = q.P;
@@ -587,6 +830,7 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   s.dL = reinterpret_cast<float*>(w + L.dl);
   s.lik = reinterpret_cast<float*>(w + L.lik);
   s.prior_part = reinterpret_cast<float*>(w + L.prior);
+  s.ke_part = reinterpret_cast<float*>(w + L.ke);
   s.nparts = L.nparts;
   s.U = q.U; s.grad = q.grad;
   s.samples = q.samples; s.samples_U = q.samples_U; s.n_slots = q.n_slots;
@@ -600,8 +844,16 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   const float ps2 = q.prior_scale * q.prior_scale;
   s.prior_inv_var = q.has_prior ? 1.0f / ps2 : 0.0f;
   s.prior_log_norm = q.has_prior ? logf(2.0f * (float)M_PI * ps2) : 0.0f;
+  s.mode = update ? q.mode : MLP_SGLD;
+  s.mom = q.mom; s.ke = q.ke; s.U_half = q.U_half;
+  s.decay_table = q.decay_table; s.noise_table = q.noise_table;
+  s.full = 0;
+  s.row0 = 0;
+  return SGMC_OK;
+}
 
-  hipStream_t st_ = as_stream(stream);
+// gather, forward GEMMs and the head (likelihood, and its backward unless s.full).
+void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   const int C = q.C, n = q.n;
   const dim3 rows_grid((n + 3) / 4, C);
   hipLaunchKernelGGL(gather_kernel, rows_grid, dim3(GT), 0, st_, s);
@@ -615,6 +867,22 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   hipLaunchKernelGGL((gemm_kernel<true, false, EPI_BIAS_RELU>), dim3(n / BM, q.H2 / BN, C),
                      dim3(GT), 0, st_, g2, s);
   hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
+  Layout L;
+  Step s;
+  const int rc = make_step(st, update, L, s);
+  if (rc != SGMC_OK) return rc;
+  const sgmc_mlp_state& q = *st;
+  hipStream_t st_ = as_stream(stream);
+  const int C = q.C, n = q.n;
+  launch_forward(s, q, st_);
+  uint16_t* sh = q.shadow;
   // dA1 = (dA2 W2^T) * (H1 > 0): B[k=b][n=a] = W2[a][b], k-contiguous rows of W2.
   Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, sh + s.off[1], q.H2, q.P, n, q.H1, q.H2,
           s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};
@@ -639,10 +907,49 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   }
   hipLaunchKernelGGL(small_kernel, dim3(L.small_blocks, C), dim3(GT), 0, st_, s,
                      L.g4_blocks + L.g5_blocks);
-  if (update && q.samples && q.save_slot) {
+  if (update && s.mode == MLP_SGLD && q.samples && q.save_slot) {
     hipLaunchKernelGGL(save_kernel, dim3(256, C), dim3(GT), 0, st_, s, 1);
   }
   return check_launch("sgmc_mlp_step");
+}
+
+int sgmc_mlp_full_potential(const sgmc_mlp_state* st, double* out, void* stream) {
+  Layout L;
+  Step s;
+  const int rc = make_step(st, 1, L, s);
+  if (rc != SGMC_OK) return rc;
+  SGMC_REQUIRE(out, SGMC_E_ARG, "sgmc_mlp_full_potential: null output");
+  const sgmc_mlp_state& q = *st;
+  hipStream_t st_ = as_stream(stream);
+  double* acc = reinterpret_cast<double*>(static_cast<uint8_t*>(q.work) + L.facc);
+  hipLaunchKernelGGL(zero_d_kernel, dim3((q.C + GT - 1) / GT), dim3(GT), 0, st_, acc, q.C);
+  s.full = 1;
+  for (int64_t r0 = 0; r0 < q.N; r0 += q.n) {
+    s.row0 = r0;
+    launch_forward(s, q, st_);
+    hipLaunchKernelGGL(full_acc_kernel, dim3(q.C), dim3(GT), 0, st_, s, acc);
+  }
+  hipLaunchKernelGGL(full_final_kernel, dim3(q.C), dim3(GT), 0, st_, s, acc, out);
+  return check_launch("sgmc_mlp_full_potential");
+}
+
+int sgmc_mlp_mh(const sgmc_mlp_state* st, void* stream) {
+  Layout L;
+  Step s;
+  const int rc = make_step(st, 1, L, s);
+  if (rc != SGMC_OK) return rc;
+  const sgmc_mlp_state& q = *st;
+  SGMC_REQUIRE(q.mom && q.ke && q.U_full && q.U_prop && q.theta_old && q.mom_old && q.U_old &&
+                   q.mh_key,
+               SGMC_E_ARG, "sgmc_mlp_mh: null pointer in state");
+  SGMC_REQUIRE(q.noise_temperature != 0.0f, SGMC_E_ARG, "sgmc_mlp_mh: zero temperature");
+  s.mom = q.mom;
+  s.ke = q.ke;
+  MH m{q.U_full, q.U_prop, q.theta_old, q.mom_old, q.U_old, q.mh_key, q.mh_stats, q.accepted};
+  const int64_t blocks = std::min<int64_t>((q.P + GT - 1) / GT, 1024);
+  hipLaunchKernelGGL(mh_kernel, dim3((unsigned)blocks, q.C), dim3(GT), 0, as_stream(stream), s,
+                     m);
+  return check_launch("sgmc_mlp_mh");
 }
 
 }  // extern "C"

@@ -58,12 +58,20 @@ class HostDataLoader(DataLoader):
 
 
 def _to_device_array(a, device):
-  a = np.asarray(a)
-  if a.dtype == np.float64:
-    a = a.astype(np.float32)
-  if a.dtype == np.int64:
-    a = a.astype(np.int32)
-  t = torch.as_tensor(np.ascontiguousarray(a), device=device)
+  if isinstance(a, torch.Tensor):
+    t = a.to(device)
+    if t.dtype == torch.float64:
+      t = t.to(torch.float32)
+    if t.dtype == torch.int64:
+      t = t.to(torch.int32)
+    t = t.contiguous()
+  else:
+    a = np.asarray(a)
+    if a.dtype == np.float64:
+      a = a.astype(np.float32)
+    if a.dtype == np.int64:
+      a = a.astype(np.int32)
+    t = torch.as_tensor(np.ascontiguousarray(a), device=device)
   if t.dim() == 2 and t.dtype == torch.float32 and t.shape[1] % 4:
     ld = (t.shape[1] + 3) // 4 * 4
     store = torch.zeros((t.shape[0], ld), dtype=t.dtype, device=device)

@@ -45,6 +45,8 @@ class FusedGLM:
   chains: list of dicts with keys X (device float32 [N, ldx], ldx % 4 == 0, zero
   padding beyond D), y (device float32 [N]), D, n, theta0 (array-like [D]), key and
   dkey (two uint32 words each), temperature (SGLD noise temperature, default 1).
+  Optional per-chain keys: N (observations, default X.shape[0]) and save (default
+  True; False leaves the chain out of the sample buffer).
   keep_lik: also keep the per-row likelihoods of the last evaluation in ``lik``.
   """
 
@@ -101,6 +103,7 @@ class FusedGLM:
     total_bytes = sum(int(c["n"]) * 4 * _align4(int(c["D"])) for c in chains)
     self.target_bytes = target_bytes or max(64 * 1024, total_bytes // 2048)
     self.keep_lik = bool(keep_lik)
+    self.exchange = None
     self._allocate()
     self._build_table()
 
@@ -162,10 +165,10 @@ class FusedGLM:
       e.lik = (self.lik.data_ptr() + fs * int(self.loff[i])) if self.lik is not None else None
       e.partial = self.partial.data_ptr() + fs * int(pof[i])
       e.counter = self.counters.data_ptr() + 4 * i
+      saves = self.samples is not None and c.get("save", True)
       e.samples = (self.samples.data_ptr() + fs * int(self.poff[i]) * self.n_slots
-                   if self.samples is not None else None)
-      e.samples_U = (self.samples_U.data_ptr() + fs * i * self.n_slots
-                     if self.samples_U is not None else None)
+                   if saves else None)
+      e.samples_U = (self.samples_U.data_ptr() + fs * i * self.n_slots if saves else None)
       e.N = int(c["X"].shape[0]) if "N" not in c else int(c["N"])
       e.ldx = int(c["X"].stride(0))
       e.D = int(self.D[i])
@@ -189,10 +192,16 @@ class FusedGLM:
               _lib.ptr(self.sqrt_mass), _lib.ptr(self.eps_table), self.table_len,
               _lib.ptr(self.save_slot), _lib.ptr(self.iter), _lib.ptr(self.done), _lib.stream())
 
-  def iteration(self):
-    """Advances every chain by one reference iteration (steps launches)."""
+  def _launch_steps(self):
     for s in range(self.steps):
       self._launch(s)
+
+  def iteration(self):
+    """Advances every chain by one reference iteration (steps launches), followed by
+    the replica-exchange step when one is attached."""
+    self._launch_steps()
+    if self.exchange is not None:
+      self.exchange.launch()
 
   def capture(self, iterations_per_graph=1):
     """Captures ``iterations_per_graph`` iterations into a HIP graph."""
@@ -260,11 +269,11 @@ class FusedGLM:
     for tb in candidates:
       self.target_bytes = tb
       self._build_table()
-      self.iteration()
+      self._launch_steps()
       torch.cuda.synchronize()
       t0 = time.perf_counter()
       for _ in range(reps):
-        self.iteration()
+        self._launch_steps()
       torch.cuda.synchronize()
       dt = (time.perf_counter() - t0) / reps
       if best is None or dt < best[0]:
@@ -288,3 +297,92 @@ class FusedGLM:
       if v is not None:
         getattr(self, k).copy_(v)
     torch.cuda.synchronize()
+
+
+class ReplicaExchange:
+  """parallel_tempering's swap step (solver.py:264-293) between pairs of chains of a
+  FusedGLM, launched after every iteration (``sgmc_resgld_swap``).
+
+  pairs: [(lo, hi)] chain indices, lo on the colder schedule; temperatures: [(t_lo,
+  t_hi)] the slot temperatures; keys: one swap key (two uint32 words) per pair;
+  swap_every: iterations between tests; parity: per pair -1 (tested every round) or
+  0 / 1 (even / odd rounds of a ladder); save: per pair, keep the lo chain's samples at
+  the engine's save slots (taken after the exchange, as the reference saves the normal
+  chain). Pairs tested in one round must not share a chain.
+  """
+
+  def __init__(self, engine, pairs, temperatures, keys=None, swap_every=1, parity=None,
+               save=None, F=1.0, ssq_init=0.0):
+    dev = engine.device
+    P = len(pairs)
+    parity = [-1] * P if parity is None else [int(q) for q in parity]
+    save = [False] * P if save is None else [bool(v) for v in save]
+    if swap_every < 1:
+      raise ValueError("swap_every must be >= 1")
+    for (lo, hi), q in zip(pairs, parity):
+      if not (0 <= lo < engine.C and 0 <= hi < engine.C and lo != hi):
+        raise ValueError("pair chains out of range")
+      if engine.D[lo] != engine.D[hi]:
+        raise ValueError("paired chains must have the same D")
+      if q not in (-1, 0, 1):
+        raise ValueError("parity must be -1, 0 or 1")
+    for r in (0, 1):  # chains tested together in an even / odd round are distinct
+      used = [c for pr, q in zip(pairs, parity) if q in (-1, r) for c in pr]
+      if len(used) != len(set(used)):
+        raise ValueError("pairs tested in the same round share a chain")
+    for p, ((lo, _), sv) in enumerate(zip(pairs, save)):
+      if sv and any(lo in pr for j, pr in enumerate(pairs) if j != p):
+        raise ValueError("a saving pair's lo chain must not belong to another pair")
+    self.engine = engine
+    self.P = P
+    self.F = float(F)
+    self.swap_every = int(swap_every)
+    self.ssq = torch.full((P,), float(ssq_init), device=dev)
+    self.step = torch.zeros(P, dtype=torch.int32, device=dev)
+    kw = np.zeros((P, 2), np.uint32) if keys is None else np.asarray(keys, np.uint32)
+    self.keys = torch.as_tensor(kw.reshape(P, 2).view(np.int32).copy(), device=dev)
+    self.swaps = torch.zeros(P, dtype=torch.int32, device=dev)
+    self.save = save
+    self.n_slots = engine.n_slots
+    nsave = sum(save)
+    self.samples = None
+    self.samples_U = None
+    self.save_index = {}
+    if nsave and self.n_slots:
+      D = [int(engine.D[lo]) for (lo, _), sv in zip(pairs, save) if sv]
+      self.soff = np.concatenate([[0], np.cumsum(D)]).astype(np.int64) * self.n_slots
+      self.samples = torch.zeros(int(self.soff[-1]), device=dev)
+      self.samples_U = torch.zeros(nsave * self.n_slots, device=dev)
+    arr = (_lib.SwapPair * P)()
+    k = 0
+    for p, ((lo, hi), (tl, th), q, sv) in enumerate(zip(pairs, temperatures, parity, save)):
+      e = arr[p]
+      e.lo, e.hi, e.t_lo, e.t_hi, e.parity = int(lo), int(hi), float(tl), float(th), q
+      e.ssq = self.ssq.data_ptr() + 4 * p
+      e.step = self.step.data_ptr() + 4 * p
+      e.key = self.keys.data_ptr() + 8 * p
+      e.swaps = self.swaps.data_ptr() + 4 * p
+      if sv and self.samples is not None:
+        e.samples = self.samples.data_ptr() + 4 * int(self.soff[k])
+        e.samples_U = self.samples_U.data_ptr() + 4 * k * self.n_slots
+        self.save_index[p] = k
+        k += 1
+    self.table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+    self.pairs = list(pairs)
+    engine.exchange = self
+
+  def launch(self):
+    eng = self.engine
+    _lib.call("sgmc_resgld_swap", _lib.ptr(eng.table), eng.C, _lib.ptr(self.table), self.P,
+              self.F, self.swap_every, _lib.ptr(eng.save_slot), eng.table_len,
+              _lib.ptr(eng.iter), _lib.stream())
+
+  def sample_trace(self, p):
+    """[n_slots, D] samples of the lo chain of pair p (saved after the exchange)."""
+    k = self.save_index[p]
+    D = int(self.engine.D[self.pairs[p][0]])
+    return self.samples[int(self.soff[k]):int(self.soff[k + 1])].reshape(self.n_slots, D)
+
+  def sample_potentials(self, p):
+    k = self.save_index[p]
+    return self.samples_U[k * self.n_slots:(k + 1) * self.n_slots]

@@ -62,11 +62,17 @@ class MLPSoftmax:
 
 
 class FusedMLP:
-  """C chains of SGLD (optionally rms_prop) on the MLP likelihood."""
+  """C chains on the MLP likelihood: SGLD (optionally rms_prop), OBABO
+  (``integrator="obabo"``: ``steps`` OBABO steps per iteration, identity mass), or
+  SGGMC (``integrator="sggmc"``: OBABO proposals with the full-data FP64 MH test).
+
+  For SGGMC the keys are the solver's (split into the integrator key and the MH key
+  as solver.py:478-487 does)."""
 
   def __init__(self, X, labels, dims, n, theta0, eps_table, keys=None, dkeys=None, rms=False,
                alpha=0.9, lmbd=1e-5, prior_scale=1.0, has_prior=True, temperature=1.0,
-               noise_temperature=1.0, save_slots=None, device="cuda"):
+               noise_temperature=1.0, save_slots=None, device="cuda", integrator="sgld",
+               friction=1.0, steps=1):
     _lib.require_gpu()
     _lib.load()
     dev = torch.device(device)
@@ -116,6 +122,41 @@ class FusedMLP:
     self.cfg = dict(rms=1 if rms else 0, alpha=float(alpha), lmbd=float(lmbd),
                     temperature=float(temperature), noise_temperature=float(noise_temperature),
                     has_prior=1 if has_prior else 0, prior_scale=float(prior_scale))
+    if integrator not in ("sgld", "obabo", "sggmc"):
+      raise ValueError("integrator must be 'sgld', 'obabo' or 'sggmc'")
+    self.integrator = integrator
+    self.steps = int(steps) if integrator != "sgld" else 1
+    self.mom = self.ke = self.U_half = self.decay = self.noise = None
+    self.U_full = self.U_prop = self.theta_old = self.mom_old = self.U_old = None
+    self.mh_key = self.mh_stats = self.accepted = None
+    if integrator != "sgld":
+      if rms:
+        raise ValueError("rms_prop applies to the SGLD integrator only")
+      f32 = np.float32
+      perm = np.exp(f32(-f32(friction)) * eps).astype(f32)
+      self.decay = torch.as_tensor(np.sqrt(perm).astype(f32), device=dev)
+      self.noise = torch.as_tensor(
+          np.sqrt(((f32(1) - perm) * f32(noise_temperature)).astype(f32)).astype(f32),
+          device=dev)
+      self.mom = torch.zeros_like(self.theta)
+      self.ke = torch.zeros((self.C, 2), dtype=torch.float64, device=dev)
+      self.U_half = torch.zeros(self.C, device=dev)
+    if integrator == "sggmc":
+      # solver.py:478-487: (integrator key, MH key) = split(key)
+      from . import random as R
+      ks = np.stack([R.key_words(R.split(R.key_from_words(k, device=dev)))
+                     for k in kw.reshape(self.C, 2)])
+      self.keys.copy_(torch.as_tensor(ks[:, 0].copy().view(np.int32), device=dev))
+      self.mh_key = torch.as_tensor(ks[:, 1].copy().view(np.int32), device=dev)
+      self.U_full = torch.zeros(self.C, dtype=torch.float64, device=dev)
+      self.U_prop = torch.zeros(self.C, dtype=torch.float64, device=dev)
+      self.theta_old = self.theta.clone()
+      self.mom_old = torch.zeros_like(self.theta)
+      self.U_old = torch.zeros(self.C, device=dev)
+      self.accepted = torch.zeros(self.C, dtype=torch.int32, device=dev)
+      if self.n_slots:
+        self.mh_stats = torch.zeros((self.C, self.n_slots, 3), device=dev)
+      self.full_potential(out=self.U_full)
 
   def _state(self):
     D_in, H1, H2 = self.dims
@@ -141,11 +182,57 @@ class FusedMLP:
     st.table_len = self.table_len
     for k, v in self.cfg.items():
       setattr(st, k, v)
+    opt = lambda t: t.data_ptr() if t is not None else None
+    st.mom, st.ke, st.U_half = opt(self.mom), opt(self.ke), opt(self.U_half)
+    st.decay_table, st.noise_table = opt(self.decay), opt(self.noise)
+    st.U_full, st.U_prop = opt(self.U_full), opt(self.U_prop)
+    st.theta_old, st.mom_old, st.U_old = opt(self.theta_old), opt(self.mom_old), opt(self.U_old)
+    st.mh_key, st.mh_stats, st.accepted = opt(self.mh_key), opt(self.mh_stats), opt(self.accepted)
+    st.mode = 0
     return st
 
   def iteration(self):
+    if self.integrator == "sgld":
+      st = self._state()
+      _lib.call("sgmc_mlp_step", ctypes.byref(st), 1, _lib.stream())
+      return
+    for _ in range(self.steps):
+      self.obabo_step()
+    if self.integrator == "sggmc":
+      self.full_potential(out=self.U_prop)
+      st = self._state()
+      _lib.call("sgmc_mlp_mh", ctypes.byref(st), _lib.stream())
+    else:
+      self._end_iteration()
+
+  def obabo_step(self):
+    """One OBABO step (integrator.py:203-273) for every chain: two launches of the
+    kernel sequence, each on a fresh minibatch."""
     st = self._state()
+    st.mode = 1
     _lib.call("sgmc_mlp_step", ctypes.byref(st), 1, _lib.stream())
+    st.mode = 2
+    _lib.call("sgmc_mlp_step", ctypes.byref(st), 1, _lib.stream())
+
+  def _end_iteration(self):
+    # plain OBABO through solver.sgmc: save the kept sample, advance the counter
+    if self.samples is not None:
+      it = torch.clamp(self.iter, 0, self.table_len - 1).long()
+      slot = self.save_slot[it]
+      ok = (slot >= 0)
+      sl = torch.clamp(slot, 0, self.n_slots - 1).long()
+      cur = self.samples[:, sl[0]]
+      self.samples[:, sl[0]] = torch.where(ok, self.theta, cur)
+      self.samples_U[:, sl[0]] = torch.where(ok, self.U, self.samples_U[:, sl[0]])
+    self.iter += 1
+
+  def full_potential(self, out=None):
+    """float64 full-data potential of every chain's current parameters."""
+    if out is None:
+      out = torch.empty(self.C, dtype=torch.float64, device=self.theta.device)
+    st = self._state()
+    _lib.call("sgmc_mlp_full_potential", ctypes.byref(st), _lib.ptr(out), _lib.stream())
+    return out
 
   def potential_and_grad(self):
     """U [C] and grad U [C, P] at the current parameters for the minibatch the data

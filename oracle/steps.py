@@ -234,3 +234,45 @@ def resgld_swap_decision(U_normal, U_hot, T_normal, T_hot, ssq, F, key):
   key, sub = jr.split(key)
   log_u = jr.logf_f32(jr.uniform(sub).reshape(1))[0]
   return not (log_u < log_s), key
+
+
+def resgld(pot, theta_normal, theta_hot, iterations, eps, T_hot, N, n, leaf_sizes,
+           key=None, dkey=None, T_normal=1.0, F=1.0, ssq_init=0.0, keep=None):
+  """parallel_tempering (solver.py:244-293) over langevin_diffusion as alias.re_sgld
+  builds it (alias.py:178-207): init splits key into (swap key, normal key, hot key);
+  both chains start their data loader from the same dkey; every iteration updates the
+  normal chain (temperature T_normal) and the hot chain (T_hot), refreshes ssq with step
+  size 1/step from the normal chain's var(lik), then keeps the pair when log u < log s
+  and exchanges the two states otherwise. Returns the final states, the decision of
+  each iteration and the normal chain's state after each kept iteration."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  ks = jr.split(key, 3)
+  skey = ks[0]
+  st = [dict(theta=np.asarray(theta_normal, np.float64).copy(), key=ks[1], dkey=dkey.copy(),
+             U=None, var=None),
+        dict(theta=np.asarray(theta_hot, np.float64).copy(), key=ks[2], dkey=dkey.copy(),
+             U=None, var=None)]
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  temps_slot = (F32(T_normal), F32(T_hot))
+  ssq = F32(ssq_init)
+  swaps, saved, saved_U = [], [], []
+  for t in range(iterations):
+    for i in (0, 1):
+      r = sgld(pot, st[i]["theta"], 1, eps[t:t + 1], temps_slot[i], N, n, leaf_sizes,
+               key=st[i]["key"], dkey=st[i]["dkey"])
+      st[i] = dict(theta=r["theta"][-1], key=r["key"], dkey=r["dkey"], U=F32(r["U"][-1]),
+                   var=F32(r["variance"][-1]))
+    step = t + 1
+    lr = F32(F32(1.0) / F32(step))
+    ssq = F32(F32(F32(1.0) - lr) * ssq + F32(lr * st[0]["var"]))
+    sw, skey = resgld_swap_decision(st[0]["U"], st[1]["U"], temps_slot[0], temps_slot[1],
+                                    ssq, F, skey)
+    if sw:
+      st[0], st[1] = st[1], st[0]
+    swaps.append(bool(sw))
+    if keep is None or keep[t]:
+      saved.append(st[0]["theta"].copy())
+      saved_U.append(st[0]["U"])
+  return {"normal": st[0], "hot": st[1], "swaps": np.asarray(swaps), "ssq": ssq,
+          "samples": np.asarray(saved), "samples_U": np.asarray(saved_U), "key": skey}

@@ -73,6 +73,33 @@ def test_sgld_linear_regression_matches_oracle(fused):
 
 
 @pytest.mark.parametrize("fused", [False, True])
+def test_re_sgld_matches_oracle(fused):
+  """alias.re_sgld: the normal chain's saved samples and potentials over 40 iterations
+  within 1e-5 of the oracle's parallel_tempering composition with the same keys, the
+  exchange decisions included."""
+  print("re_sgld parity fused=", fused, flush=True)
+  N, D, n, its, tau = 500, 8, 16, 40, 3.0
+  X, y = linreg(N=N, D=D, seed=5)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.LinearGaussian(0.5), strategy="vmap")
+  first, last = 2e-3, 5e-4
+  run = alias.re_sgld(pot, loader, cache_size=1, batch_size=n, first_step_size=first,
+                      last_step_size=last, burn_in=0, accepted_samples=its, temperature=tau)
+  xn = np.zeros(D, np.float32)
+  xh = np.full(D, 0.3, np.float32)
+  res = run((torch.as_tensor(xn, device=DEV), torch.as_tensor(xh, device=DEV)),
+            iterations=its, fused=fused)[0]
+  eps = SC.polynomial_table(first, last, its)
+  fam = om.Family("linear_gaussian", X, y, sigma=0.5)
+  potf = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
+  r = ost.resgld(potf, xn, xh, its, eps, tau, N, n, [D])
+  print("oracle exchanges", int(r["swaps"].sum()), "of", its, flush=True)
+  assert 0 < r["swaps"].sum() < its  # both outcomes occur in this run
+  _close(res["samples"]["variables"], r["samples"], "theta")
+  _close(-np.asarray(res["samples"]["likelihood"]), r["samples_U"], "potential")
+
+
+@pytest.mark.parametrize("fused", [False, True])
 def test_sgld_rmsprop_matches_oracle(fused):
   print("rms parity fused=", fused, flush=True)
   X, y = logreg(N=3000, D=16)

@@ -131,3 +131,71 @@ def test_fused_autotune_preserves_state():
   # block counts differ, so sums are reordered: equal to float32 rounding only
   np.testing.assert_allclose(a.theta_flat.cpu().numpy(), b.theta_flat.cpu().numpy(),
                              rtol=1e-4, atol=1e-5)
+
+
+def _ladder(R_=4, D=24, N=700, swap_every=3, iters=12, graph=0):
+  """One logistic problem, R_ rungs on a geometric temperature ladder, adjacent pairs
+  tested in alternating even / odd rounds."""
+  X, y = _data(N, D, "logistic", seed=42)
+  Xd, yd = _dev(X), torch.as_tensor(y, device=DEV)
+  temps = [20.0 ** (j / (R_ - 1)) for j in range(R_)]
+  chains = [dict(X=Xd, y=yd, D=D, n=64, theta0=np.full(D, 0.05 * j, np.float32), key=(2, j),
+                 dkey=(3, j), temperature=temps[j], save=False) for j in range(R_)]
+  eng = fused.FusedGLM(chains, "logistic", "sgld", np.full(iters, 2e-3, np.float32),
+                       save_slots=np.arange(iters), n_slots=iters)
+  pairs = [(j, j + 1) for j in range(R_ - 1)]
+  ex = fused.ReplicaExchange(eng, pairs, [(temps[j], temps[j + 1]) for j in range(R_ - 1)],
+                             keys=[(5, j) for j in range(R_ - 1)], swap_every=swap_every,
+                             parity=[j % 2 for j in range(R_ - 1)],
+                             save=[j == 0 for j in range(R_ - 1)])
+  return eng, ex
+
+
+def test_exchange_ladder_rounds_and_permutation():
+  """Each exchange launch tests exactly the pairs of the round's parity, every tested
+  pair either keeps or exchanges its two states (theta, keys, U), and rung 0 is saved
+  after the exchange."""
+  print("exchange ladder", flush=True)
+  eng, ex = _ladder()
+  for it in range(12):
+    eng._launch_steps()
+    th = torch.stack(eng.theta()).clone()
+    keys = eng.keys.clone()
+    U = eng.U.clone()
+    step0 = ex.step.clone()
+    swaps0 = ex.swaps.clone()
+    ex.launch()
+    torch.cuda.synchronize()
+    th1 = torch.stack(eng.theta())
+    done = it + 1
+    tested = set()
+    if done % ex.swap_every == 0:
+      par = (done // ex.swap_every - 1) % 2
+      tested = {p for p in range(ex.P) if p % 2 == par}
+    for p, (lo, hi) in enumerate(ex.pairs):
+      dstep = int(ex.step[p] - step0[p])
+      assert dstep == (1 if p in tested else 0)
+      sw = int(ex.swaps[p] - swaps0[p])
+      if sw:
+        assert torch.equal(th1[lo], th[hi]) and torch.equal(th1[hi], th[lo])
+        assert torch.equal(eng.keys[lo], keys[hi]) and torch.equal(eng.U[lo], U[hi])
+    moved = {c for p in range(ex.P) if int(ex.swaps[p] - swaps0[p]) for c in ex.pairs[p]}
+    for c in range(eng.C):
+      if c not in moved:
+        assert torch.equal(th1[c], th[c])
+    np.testing.assert_array_equal(ex.sample_trace(0)[it].cpu().numpy(), th1[0].cpu().numpy())
+  # rounds after iterations 3, 6, 9, 12 alternate parity 0, 1, 0, 1: pairs 0 and 2 are
+  # tested twice, pair 1 twice
+  assert ex.step.tolist() == [2, 2, 2]
+
+
+def test_exchange_ladder_graph_equals_eager():
+  print("exchange ladder graph", flush=True)
+  a, exa = _ladder()
+  b, exb = _ladder()
+  a.run(12, graph_iterations=6)
+  b.run(12)
+  torch.cuda.synchronize()
+  assert torch.equal(a.theta_flat, b.theta_flat)
+  assert torch.equal(exa.swaps, exb.swaps) and torch.equal(exa.ssq, exb.ssq)
+  assert torch.equal(exa.samples, exb.samples)

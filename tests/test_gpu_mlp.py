@@ -113,3 +113,107 @@ def test_mlp_graph_replay_matches_eager():
   torch.cuda.synchronize()
   assert torch.equal(a.theta, b.theta)
   assert a.iter.item() == 6
+
+
+def _obabo_engine(X, lab, th, integrator, keys, dkeys, eps, steps, friction=1.0, slots=None):
+  return M.FusedMLP(torch.as_tensor(X, device=DEV), torch.as_tensor(lab, device=DEV), DIMS, 64,
+                    th, eps, keys=keys, dkeys=dkeys, prior_scale=1.0, integrator=integrator,
+                    friction=friction, steps=steps, save_slots=slots)
+
+
+def test_mlp_obabo_matches_oracle():
+  """One iteration of 3 OBABO steps (integrator.py:203-273) for 2 chains: the noise
+  words are exact, so momentum and position differ from the float64 oracle only
+  through the bf16 gradients: the difference stays within 5e-2 of the gradient's
+  whole contribution (oracle with and without gradient); the kinetic energies within
+  1e-3 and the step potential within 1e-2."""
+  print("mlp obabo", flush=True)
+  X, lab = data()
+  rng = np.random.default_rng(3)
+  th = np.stack([M.init_params(20 + c, *DIMS) for c in range(2)])
+  keys, dkeys = [(0, 11), (0, 12)], [(4, 1), (4, 2)]
+  eps = np.full(4, 2e-3, np.float32)
+  eng = _obabo_engine(X, lab, th, "obabo", keys, dkeys, eps, steps=3, friction=0.7)
+  eng.iteration()
+  torch.cuda.synchronize()
+  fam = om.Family("mlp", X, lab, dims=DIMS + (10,))
+  pot = lambda t, idx: om.stochastic_potential(fam, t, idx, 1.0)
+  zero = lambda t, idx: (0.0, np.zeros_like(t), np.zeros(len(idx)))
+  P = th.shape[1]
+  ones = np.ones(P)
+  for c in range(2):
+    args = (np.asarray(keys[c], np.uint32), np.asarray(dkeys[c], np.uint32), eps[0], 1.0,
+            0.7, 3, X.shape[0], 64, [128 * 128, 128 * 128, 1280, 128, 128, 10], ones, ones)
+    th_r, p_r, key_r, dkey_r, U_r, ks_r, ke_r = ost.obabo_integrate(
+        pot, th[c].astype(np.float64), np.zeros(P), *args)
+    th_0, p_0, *_ = ost.obabo_integrate(zero, th[c].astype(np.float64), np.zeros(P), *args)
+    p_g = eng.mom[c].cpu().numpy().astype(np.float64)
+    t_g = eng.theta[c].cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(p_g - p_r) <= 5e-2 * np.linalg.norm(p_r - p_0) + 1e-6
+    assert np.linalg.norm(t_g - th_r) <= 5e-2 * np.linalg.norm(th_r - th_0) + 1e-6
+    ks, ke = eng.ke[c].tolist()
+    assert abs(ks - ks_r) <= 1e-3 * abs(ks_r) and abs(ke - ke_r) <= 1e-3 * abs(ke_r), (ks, ks_r)
+    assert abs(eng.U[c].item() - U_r) <= 1e-2 * abs(U_r)
+    np.testing.assert_array_equal(eng.keys[c].cpu().numpy().view(np.uint32), key_r)
+    np.testing.assert_array_equal(eng.dkeys[c].cpu().numpy().view(np.uint32), dkey_r)
+
+
+def test_mlp_full_potential_f64():
+  """float64 full-data potential of the bf16 forward pass within 1e-2 of the float64
+  oracle's (every row once, N not a multiple of the pass size)."""
+  print("mlp full potential", flush=True)
+  X, lab = data(N=1000)
+  th = np.stack([M.init_params(30 + c, *DIMS) for c in range(2)])
+  eng = _obabo_engine(X, lab, th, "obabo", None, None, np.full(2, 1e-3, np.float32), steps=1)
+  out = eng.full_potential()
+  fam = om.Family("mlp", X, lab, dims=DIMS + (10,))
+  for c in range(2):
+    ref = om.full_potential(fam, th[c].astype(np.float64), 1.0, mb=64)
+    assert abs(out[c].item() - ref) <= 1e-2 * abs(ref), (out[c].item(), ref)
+
+
+def test_mlp_sggmc_mh_decision_and_revert():
+  """sggmc's MH step (solver.py:500-567) on chosen full potentials: the decision
+  follows log u < min(0, -(dU + dKE)/T) with u from the solver key, a rejection
+  restores theta, momentum and the step potential, an acceptance keeps the proposal;
+  the kinetic energies reset and the iteration counter advances."""
+  print("mlp sggmc mh", flush=True)
+  X, lab = data()
+  th = np.stack([M.init_params(40 + c, *DIMS) for c in range(2)])
+  keys = [(0, 5), (0, 6)]
+  eng = _obabo_engine(X, lab, th, "sggmc", keys, [(1, 1), (1, 2)],
+                      np.full(4, 2e-3, np.float32), steps=2, slots=np.arange(4))
+  mh_keys = eng.mh_key.cpu().numpy().view(np.uint32).copy()
+  for _ in range(eng.steps):
+    eng.obabo_step()
+  torch.cuda.synchronize()
+  th_prop = eng.theta.clone()
+  mom_prop = eng.mom.clone()
+  U_step = eng.U.clone()
+  ke = eng.ke.cpu().numpy().copy()
+  U_full = eng.U_full.cpu().numpy().copy()
+  # chain 0: a proposal far uphill (rejected for any u); chain 1: dE = -1 (accepted)
+  dKE = ke[:, 1] - ke[:, 0]
+  U_prop = np.array([U_full[0] + 1e6, U_full[1] - dKE[1] - 1.0])
+  eng.U_prop.copy_(torch.as_tensor(U_prop, device=DEV))
+  st = eng._state()
+  import ctypes
+  from jax_sgmc_amd import _lib
+  _lib.call("sgmc_mlp_mh", ctypes.byref(st), _lib.stream())
+  torch.cuda.synchronize()
+  for c in range(2):
+    _, sub = jr.split(mh_keys[c])
+    log_u = jr.logf_f32(jr.uniform(sub).reshape(1))[0]
+    log_a = min(0.0, -(U_prop[c] - U_full[c] + dKE[c]))
+    acc = bool(log_u < log_a)
+    assert acc == (c == 1)
+  assert torch.equal(eng.theta[0], torch.as_tensor(th[0], device=DEV))
+  assert torch.equal(eng.mom[0], torch.zeros_like(eng.mom[0]))
+  assert eng.U[0].item() == 0.0  # the initial ObaboState potential
+  assert torch.equal(eng.theta[1], th_prop[1]) and torch.equal(eng.mom[1], mom_prop[1])
+  assert torch.equal(eng.theta_old[1], th_prop[1])
+  assert eng.U_full[1].item() == U_prop[1] and eng.U_full[0].item() == U_full[0]
+  assert torch.all(eng.ke == 0)
+  assert eng.iter.item() == 1 and eng.accepted.tolist() == [0, 1]
+  np.testing.assert_array_equal(eng.samples[1, 0].cpu().numpy(), th_prop[1].cpu().numpy())
+  assert eng.samples_U[1, 0].item() == U_step[1].item()
