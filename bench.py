@@ -342,6 +342,108 @@ def run_mlp(args):
   print(json.dumps(out), flush=True)
 
 
+def run_sggmc_mlp(args):
+  """SGGMC on the MLP 3072-1024-1024-10: OBABO proposals of 100 steps (two minibatches
+  of 512 per step), then the float64 full-data potential over 50,000 rows and the MH
+  test; chains advance together."""
+  import torch
+  import __graft_entry__
+  __graft_entry__.load()
+  from jax_sgmc_amd import _lib as L, mlp as M, random as R, scheduler
+  L.require_gpu()
+  dims = (3072, 1024, 1024)
+  X, labels = mlp_data(torch, R, M)
+  C, n, steps = args.chains or 4, 512, 100
+  th0 = np.stack([M.init_params(10 + c, *dims) for c in range(C)])
+  keys = R.key_words(R.split(R.PRNGKey(0), C))
+  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
+  K, W = max(1, args.steps), args.warmup
+  eps = scheduler.polynomial_table(1e-4, 1e-4, K + W + 2)
+  eng = M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys, prior_scale=1.0,
+                   integrator="sggmc", friction=1.0, steps=steps)
+  g = eng.capture(1)
+  for _ in range(max(1, W)):
+    g.replay()
+  torch.cuda.synchronize()
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(K):
+    g.replay()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1)
+  value = C * K / (ms / 1e3)
+  D_in, H1, H2 = dims
+  macs = D_in * H1 + H1 * H2 + H2 * 10
+  grad_flops = 2 * n * macs * 2 + 2 * n * (H1 * H2 + H2 * 10)
+  full_flops = 2 * MLP_N * macs
+  flops = 2 * steps * grad_flops + full_flops
+  achieved = C * flops / (ms / 1e3 / K) / 1e12
+  out = {"metric": "chain-steps/sec, SGGMC Bayesian MLP (fused MFMA, 1 MI355X)",
+         "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+         "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "bf16",
+         "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
+         "config": {"workload": "SGGMC: OBABO 100 steps per MH test, MLP 3072-1024-1024-10, "
+                                "batch 512, N=50000, float64 full-data potential",
+                    "chains": C, "P": eng.P, "batch": n, "N": MLP_N, "obabo_steps": steps},
+         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0,
+                      "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+                      "kernel": "all kernels of one SGGMC iteration (GEMM flops)",
+                      "flops_per_chain_step": flops},
+         "accepted": eng.accepted.tolist()}
+  print(json.dumps(out), flush=True)
+
+
+def run_sgld_linreg(args):
+  """SGLD on Bayesian linear regression, N=10,000, D=32, batch 128, one chain, through
+  the fused engine (graph of 100 iterations)."""
+  import torch
+  import __graft_entry__
+  __graft_entry__.load()
+  from jax_sgmc_amd import _lib as L, fused, random as R, scheduler
+  L.require_gpu()
+  N, Dl, n = 10_000, 32, 128
+  kx, kw, ke = R.split(R.PRNGKey(0), 3)
+  X = R.normal(kx, (N * Dl,)).reshape(N, Dl)
+  w = R.normal(kw, (Dl,))
+  y = X @ w + 0.5 * R.normal(ke, (N,))
+  K, W, G = args.steps, args.warmup, 100
+  eps = scheduler.polynomial_table(1e-4, 1e-5, K + W + 2 * G)
+  eng = fused.FusedGLM([dict(X=X, y=y.contiguous(), D=Dl, n=n, theta0=np.zeros(Dl, np.float32))],
+                       "linear_gaussian", "sgld", eps, sigma=0.5, prior_scale=float(np.sqrt(10.0)))
+  g = eng.capture(G)
+  for _ in range(max(1, W // G)):
+    g.replay()
+  torch.cuda.synchronize()
+  reps = max(1, K // G)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    g.replay()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1)
+  iters = reps * G
+  value = iters / (ms / 1e3)
+  bytes_step = n * (4 * Dl + 8) + 8 * Dl
+  out = {"metric": "chain-steps/sec, SGLD linear regression (one chain, 1 MI355X)",
+         "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
+         "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "fp32",
+         "data": "synthetic (jax-compatible Threefry draws from PRNGKey(0))",
+         "config": {"workload": "SGLD linear regression N=10000, D=32, batch 128, 1 chain",
+                    "step_size": "polynomial 1e-4 -> 1e-5", "graph_iterations": G},
+         "roofline": {"bound": "hbm", "achieved": bytes_step / (ms / 1e3 / iters) / 1e9,
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": bytes_step / (ms / 1e3 / iters) / 1e9 / HBM_PEAK_GBS,
+                      "traffic": None, "kernel": "glm_fused_kernel (launch-latency bound)",
+                      "bytes_per_chain_step": bytes_step}}
+  print(json.dumps(out), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -353,7 +455,9 @@ def main():
                   help="only copy a 512 MiB device buffer 5 times (the known-size kernel "
                        "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
   ap.add_argument("--workload", default="sghmc-logistic",
-                  choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp"])
+                  choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp", "sggmc-mlp",
+                           "sgld-linreg"])
+  ap.add_argument("--chains", type=int, default=0, help="chains of the sggmc-mlp workload")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -364,6 +468,10 @@ def main():
     return run_ladder(args)
   if args.workload == "psgld-mlp":
     return run_mlp(args)
+  if args.workload == "sggmc-mlp":
+    return run_sggmc_mlp(args)
+  if args.workload == "sgld-linreg":
+    return run_sgld_linreg(args)
   import torch
   if args.copy_calibration:
     n = 128 << 20

@@ -58,7 +58,8 @@ class MLPSoftmax:
     h = torch.relu(obs[self.x] @ sample["W1"] + sample["b1"])
     h = torch.relu(h @ sample["W2"] + sample["b2"])
     z = h @ sample["W3"] + sample["b3"]
-    return torch.log_softmax(z, -1)[obs[self.y].long()]
+    lab = obs[self.y].long().reshape(1)
+    return torch.gather(torch.log_softmax(z, -1), -1, lab).reshape(())
 
 
 class FusedMLP:
