@@ -37,14 +37,15 @@ namespace mlp {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 256;
 constexpr int BM = 64;
 constexpr int BN = 128;
-constexpr int BK = 32;
-constexpr int LDK = BK + 8;  // padded LDS pitch in bf16 elements
 constexpr int NCLS = 10;
 constexpr int NLEAF = 6;     // W1, W2, W3, b1, b2, b3
+constexpr int kStrip = 32;   // rows per partial sum of the small leaves' gradients
+constexpr int kFinalBlocks = 64;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -53,7 +54,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-enum Epi { EPI_BIAS_RELU = 0, EPI_MASK = 1, EPI_UPDATE = 2, EPI_GRAD = 3 };
+enum Epi { EPI_BIAS_RELU = 0, EPI_MASK = 1, EPI_UPDATE = 2, EPI_GRAD = 3, EPI_PARTIAL = 4 };
 
 // Per-chain state and step parameters shared by the kernels.
 struct Step {
@@ -99,6 +100,9 @@ struct Step {
   int mode;
   float* mom;              // [C][P] momentum
   float* ke_part;          // [C][nparts] kinetic-energy partials of one evaluation
+  float* col_part;         // [C][n / kStrip][small leaves] row-strip gradient partials
+  double* fin_part;        // [C][kFinalBlocks] log-prior partials of the full pass
+  float* split_part;       // split-K partial products of the forward GEMMs
   double* ke;              // [C][2] accumulated kinetic energy at start / end
   float* U_half;           // [C] potential of the step's first evaluation
   const float* decay_table;  // [table_len] sqrt(exp(-friction eps))
@@ -270,71 +274,107 @@ struct Gemm {
   int64_t ldm, sM;
   int leaf;          // EPI_UPDATE / EPI_GRAD: parameter leaf of the [M][N] weight
   int part0;         // first prior-partial slot of this GEMM
+  int splits;        // split-K factor (grid z = chains x splits)
+  float* part;       // EPI_PARTIAL: [splits][C][M][N] float32 partial products
+  int C;
 };
 
-template <bool A_K, bool B_K>
-__device__ __forceinline__ void load_tiles(const Gemm& g, int c, int bm, int bn, int k0,
-                                           uint4 (&ra)[1], uint4 (&rb)[2]) {
-  const int t = threadIdx.x;
-  const uint16_t* A = g.A + (int64_t)c * g.sA;
-  const uint16_t* B = g.B + (int64_t)c * g.sB;
-  const int half = g.M / 2;
-  if (A_K) {
-    const int r = t >> 2, k8 = (t & 3) * 8;
-    const int m = (r < 32) ? bm * 32 + r : half + bm * 32 + (r - 32);
-    ra[0] = *reinterpret_cast<const uint4*>(A + (int64_t)m * g.lda + k0 + k8);
-  } else {
-    const int k = t >> 3, r8 = (t & 7) * 8;
-    const int m = (r8 < 32) ? bm * 32 + r8 : half + bm * 32 + (r8 - 32);
-    ra[0] = *reinterpret_cast<const uint4*>(A + (int64_t)(k0 + k) * g.lda + m);
-  }
+// ---- operand tiles in LDS ----
+// A tile: BM rows (m) x BK (k); B tile: BN rows (n) x BK. Rows are BK bf16 long, made of
+// BK/8 16-byte chunks; chunk c of row r sits at position c ^ swz(r), so that both the
+// MFMA fragment reads (16 consecutive rows, one chunk) and the stores of transposed
+// 8x8 blocks (16 row groups 8 apart, one chunk) touch 16 distinct bank groups.
+template <int BK>
+__device__ __forceinline__ int swz(int r) {
+  return (r ^ (r >> 3)) & (BK / 8 - 1);
+}
+
+template <int BK>
+__device__ __forceinline__ int lds_off(int r, int chunk) {
+  return r * BK + ((chunk ^ swz<BK>(r)) << 3);
+}
+
+// 8x8 transpose of bf16: in[i] holds 8 consecutive columns of row i; out[j] holds row
+// i = 0..7 of column j.
+__device__ __forceinline__ void transpose8x8(const u32x4 (&in)[8], u32x4 (&out)[8]) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int u = t + h * GT;
-    if (B_K) {
-      const int col = u >> 2, k8 = (u & 3) * 8;
-      rb[h] = *reinterpret_cast<const uint4*>(B + (int64_t)(bn * BN + col) * g.ldb + k0 + k8);
-    } else {
-      const int k = u >> 4, c8 = (u & 15) * 8;
-      rb[h] = *reinterpret_cast<const uint4*>(B + (int64_t)(k0 + k) * g.ldb + bn * BN + c8);
+  for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const u32x4& x = in[2 * d];
+      const u32x4& y = in[2 * d + 1];
+      const uint32_t a = (j >> 1) == 0 ? x.x : (j >> 1) == 1 ? x.y : (j >> 1) == 2 ? x.z : x.w;
+      const uint32_t b = (j >> 1) == 0 ? y.x : (j >> 1) == 1 ? y.y : (j >> 1) == 2 ? y.z : y.w;
+      const uint32_t v = (j & 1) ? ((a >> 16) | (b & 0xFFFF0000u)) : ((a & 0xFFFFu) | (b << 16));
+      if (d == 0) out[j].x = v;
+      else if (d == 1) out[j].y = v;
+      else if (d == 2) out[j].z = v;
+      else out[j].w = v;
     }
   }
 }
 
-template <bool A_K, bool B_K>
-__device__ __forceinline__ void store_tiles(uint16_t* As, uint16_t* Bs, const uint4 (&ra)[1],
-                                            const uint4 (&rb)[2]) {
-  const int t = threadIdx.x;
-  if (A_K) {
-    const int r = t >> 2, k8 = (t & 3) * 8;
-    *reinterpret_cast<uint4*>(As + r * LDK + k8) = ra[0];
-  } else {
-    const int k = t >> 3, r8 = (t & 7) * 8;
-    const uint16_t* e = reinterpret_cast<const uint16_t*>(&ra[0]);
+// Loads of one operand tile (ROWS x BK at k0) into registers and their stores to LDS.
+// KC: the source is k-contiguous (row r at src + grow(r) * ld); otherwise it is
+// row-contiguous (k-row k at src + k * ld) and each thread moves one 8x8 block,
+// transposed in registers. grow maps tile rows to source rows (A: two bands M/2 apart).
+template <int ROWS, int BK, bool KC>
+struct Tile {
+  static constexpr int CH = BK / 8;
+  static constexpr int NR = KC ? ROWS * CH / GT : 8;   // registers per thread
+  static constexpr int BLOCKS = ROWS * BK / 64;        // 8x8 blocks (transposed case)
+  u32x4 r[NR];
+
+  template <typename Grow>
+  __device__ __forceinline__ void load(const uint16_t* src, int64_t ld, int k0, Grow grow) {
+    const int t = threadIdx.x;
+    if constexpr (KC) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) As[(r8 + i) * LDK + k] = e[i];
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int u = t + h * GT;
-    if (B_K) {
-      const int col = u >> 2, k8 = (u & 3) * 8;
-      *reinterpret_cast<uint4*>(Bs + col * LDK + k8) = rb[h];
+      for (int h = 0; h < NR; ++h) {
+        const int u = t + h * GT;
+        const int row = u / CH, ch = u % CH;
+        r[h] = *reinterpret_cast<const u32x4*>(src + grow(row) * ld + k0 + ch * 8);
+      }
     } else {
-      const int k = u >> 4, c8 = (u & 15) * 8;
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&rb[h]);
+      if (!(BLOCKS >= GT || t < BLOCKS)) return;
+      const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) Bs[(c8 + i) * LDK + k] = e[i];
+      for (int i = 0; i < 8; ++i)
+        r[i] = *reinterpret_cast<const u32x4*>(src + (int64_t)(k0 + k8 + i) * ld + grow(r8));
     }
   }
-}
 
-template <bool A_K, bool B_K, int EPI>
+  __device__ __forceinline__ void store(uint16_t* lds) const {
+    const int t = threadIdx.x;
+    if constexpr (KC) {
+#pragma unroll
+      for (int h = 0; h < NR; ++h) {
+        const int u = t + h * GT;
+        const int row = u / CH, ch = u % CH;
+        *reinterpret_cast<u32x4*>(lds + lds_off<BK>(row, ch)) = r[h];
+      }
+    } else {
+      if (!(BLOCKS >= GT || t < BLOCKS)) return;
+      const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
+      u32x4 o[8];
+      transpose8x8(r, o);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        *reinterpret_cast<u32x4*>(lds + lds_off<BK>(r8 + j, k8 / 8)) = o[j];
+    }
+  }
+};
+
+template <int BK, bool A_K, bool B_K, int EPI>
 __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDK];
+  static_assert(BN == 2 * BM, "tile shapes");
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * BK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * BK];
   __shared__ float red[GT / kWave];
-  const int bm = blockIdx.x, bn = blockIdx.y, c = blockIdx.z;
+  const int bm = blockIdx.x, bn = blockIdx.y;
+  const int c = blockIdx.z / g.splits, sp = blockIdx.z % g.splits;
+  const int kspan = g.K / g.splits;
+  const int kb = sp * kspan, ke = kb + kspan;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   f32x16 acc0, acc1;
@@ -343,19 +383,31 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     acc0[j] = 0.f;
     acc1[j] = 0.f;
   }
-  uint4 ra[1], rb[2];
-  load_tiles<A_K, B_K>(g, c, bm, bn, 0, ra, rb);
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
+  const uint16_t* A = g.A + (int64_t)c * g.sA;
+  const uint16_t* B = g.B + (int64_t)c * g.sB;
+  const int hm = g.M / 2;
+  auto growA = [&](int r) -> int64_t { return r < 32 ? bm * 32 + r : hm + bm * 32 + (r - 32); };
+  auto growB = [&](int r) -> int64_t { return (int64_t)bn * BN + r; };
+  Tile<BM, BK, A_K> ta;
+  Tile<BN, BK, B_K> tb;
+  ta.load(A, g.lda, kb, growA);
+  tb.load(B, g.ldb, kb, growB);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();
-    store_tiles<A_K, B_K>(As, Bs, ra, rb);
+    ta.store(As);
+    tb.store(Bs);
     __syncthreads();
-    if (k0 + BK < g.K) load_tiles<A_K, B_K>(g, c, bm, bn, k0 + BK, ra, rb);
+    if (k0 + BK < ke) {
+      ta.load(A, g.lda, k0 + BK, growA);
+      tb.load(B, g.ldb, k0 + BK, growB);
+    }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
-      const int ko = kk + 8 * (lane >> 5);
-      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As + (lane & 31) * LDK + ko);
-      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As + (32 + (lane & 31)) * LDK + ko);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + (w * 32 + (lane & 31)) * LDK + ko);
+      const int ch = kk / 8 + (lane >> 5);
+      const int ra0 = lane & 31, ra1 = 32 + (lane & 31), rb0 = w * 32 + (lane & 31);
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As + lds_off<BK>(ra0, ch));
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As + lds_off<BK>(ra1, ch));
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + lds_off<BK>(rb0, ch));
       acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc1, 0, 0, 0);
     }
@@ -363,7 +415,18 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
 
   const int half = g.M / 2;
   const int n = bn * BN + w * 32 + (lane & 31);
-  if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
+  if (EPI == EPI_PARTIAL) {
+    float* pp = g.part + ((int64_t)sp * g.C + c) * g.M * g.N;
+#pragma unroll
+    for (int band = 0; band < 2; ++band) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int m = (band == 0 ? bm * 32 : half + bm * 32) + r;
+        pp[(int64_t)m * g.N + n] = band == 0 ? acc0[j] : acc1[j];
+      }
+    }
+  } else if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
     uint16_t* out = g.out + (int64_t)c * g.sO;
     const float bias = (EPI == EPI_BIAS_RELU) ? g.bias[(int64_t)c * g.sBias + n] : 0.f;
 #pragma unroll
@@ -416,6 +479,41 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       if (threadIdx.x == 0) s.ke_part[slot] = kt;
     }
   }
+}
+
+// Split-K finish: out = epilogue(sum over splits of the partial products), the splits
+// added in order; four consecutive columns per thread.
+template <int EPI>
+__global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g) {
+  const int c = blockIdx.y;
+  const int64_t e = ((int64_t)blockIdx.x * GT + threadIdx.x) * 4;
+  if (e >= (int64_t)g.M * g.N) return;
+  const int m = (int)(e / g.N), n = (int)(e % g.N);
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int sp = 0; sp < g.splits; ++sp) {
+    const float4 v =
+        *reinterpret_cast<const float4*>(g.part + ((int64_t)sp * g.C + c) * g.M * g.N + e);
+    a.x += v.x;
+    a.y += v.y;
+    a.z += v.z;
+    a.w += v.w;
+  }
+  float r[4] = {a.x, a.y, a.z, a.w};
+  uint16_t o[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float v;
+    if (EPI == EPI_BIAS_RELU) {
+      v = fmaxf(r[t] + g.bias[(int64_t)c * g.sBias + n + t], 0.0f);
+    } else {
+      const uint16_t mk = g.mask[(int64_t)c * g.sM + (int64_t)m * g.ldm + n + t];
+      v = (mk != 0 && !(mk & 0x8000u)) ? r[t] : 0.0f;
+    }
+    o[t] = f2bf(v);
+  }
+  uint16_t* out = g.out + (int64_t)c * g.sO + (int64_t)m * g.ldo + n;
+  *reinterpret_cast<uint2*>(out) =
+      make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
 }
 
 // ------------------------------------------------------------------ head ------
@@ -482,6 +580,53 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
   }
 }
 
+// ------------------------------------------------------- small-leaf sums -----
+// Row-strip partial sums of the small leaves' gradients (flat order W3 [H2][10], b1,
+// b2, b3): for strip t, part[t][.] = sum over its kStrip rows of h2^T dL, dA1, dA2, dL.
+// Grid x: ceil(H2 / GT) blocks for W3 and b2, ceil(H1 / GT) for b1, one for b3.
+__global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
+  const int c = blockIdx.z, t = blockIdx.y;
+  const int nb2 = (s.H2 + GT - 1) / GT, nb1 = (s.H1 + GT - 1) / GT;
+  const int64_t nW3 = (int64_t)s.H2 * NCLS;
+  const int64_t total = nW3 + s.H1 + s.H2 + NCLS;
+  float* out = s.col_part + ((int64_t)c * (s.n / kStrip) + t) * total;
+  const int64_t r0 = (int64_t)c * s.n + (int64_t)t * kStrip;
+  const int bx = blockIdx.x;
+  if (bx < nb2) {
+    const int a = bx * GT + threadIdx.x;
+    if (a >= s.H2) return;
+    float w3[NCLS];
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) w3[k] = 0.f;
+    float b2 = 0.f;
+#pragma unroll 4
+    for (int r = 0; r < kStrip; ++r) {
+      const float h = bf2f(s.h2[(r0 + r) * s.H2 + a]);
+      b2 += bf2f(s.da2[(r0 + r) * s.H2 + a]);
+      const float* dl = s.dL + (r0 + r) * NCLS;
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w3[k] += h * dl[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) out[(int64_t)a * NCLS + k] = w3[k];
+    out[nW3 + s.H1 + a] = b2;
+  } else if (bx < nb2 + nb1) {
+    const int i = (bx - nb2) * GT + threadIdx.x;
+    if (i >= s.H1) return;
+    float b1 = 0.f;
+#pragma unroll 8
+    for (int r = 0; r < kStrip; ++r) b1 += bf2f(s.da1[(r0 + r) * s.H1 + i]);
+    out[nW3 + i] = b1;
+  } else {
+    const int k = threadIdx.x;
+    if (k >= NCLS) return;
+    float b3 = 0.f;
+#pragma unroll 8
+    for (int r = 0; r < kStrip; ++r) b3 += s.dL[(r0 + r) * NCLS + k];
+    out[nW3 + s.H1 + s.H2 + k] = b3;
+  }
+}
+
 // ----------------------------------------------------------- small params -----
 // W3, b1, b2, b3: gradient by a loop over the n rows, update, prior partial; the last
 // block of a chain forms U and advances the chain's keys (and the iteration counter
@@ -500,26 +645,28 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
   if (p < total) {
     int leaf;
     int64_t i;
+    const int strips = s.n / kStrip;
+    const float* cp = s.col_part + (int64_t)c * strips * total + p;
     float gsum = 0.f;
-    const int64_t rowb = (int64_t)c * s.n;
+    for (int t = 0; t < strips; t += 8) {
+      float x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = t + u < strips ? cp[(int64_t)(t + u) * total] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) gsum += x[u];
+    }
     if (p < nW3) {
       leaf = 2;
       i = p;
-      const int a = (int)(p / NCLS), k = (int)(p % NCLS);
-      for (int r = 0; r < s.n; ++r)
-        gsum += bf2f(s.h2[(rowb + r) * s.H2 + a]) * s.dL[(rowb + r) * NCLS + k];
     } else if (p < nW3 + s.H1) {
       leaf = 3;
       i = p - nW3;
-      for (int r = 0; r < s.n; ++r) gsum += bf2f(s.da1[(rowb + r) * s.H1 + i]);
     } else if (p < nW3 + s.H1 + s.H2) {
       leaf = 4;
       i = p - nW3 - s.H1;
-      for (int r = 0; r < s.n; ++r) gsum += bf2f(s.da2[(rowb + r) * s.H2 + i]);
     } else {
       leaf = 5;
       i = p - nW3 - s.H1 - s.H2;
-      for (int r = 0; r < s.n; ++r) gsum += s.dL[(rowb + r) * NCLS + i];
     }
     const int64_t base = (int64_t)c * s.P + s.off[leaf];
     if (s.update) {
@@ -625,21 +772,30 @@ __global__ void __launch_bounds__(GT) full_acc_kernel(Step s, double* acc) {
   if (threadIdx.x == 0) acc[c] += t;
 }
 
-// out[c] = (-acc[c] - log p(theta_c)) / T with the float64 Gaussian log prior.
+// out[c] = (-acc[c] - log p(theta_c)) / T with the float64 Gaussian log prior; grid
+// (kFinalBlocks, C), the chain's last-arriving block combines the block sums.
 __global__ void __launch_bounds__(GT) full_final_kernel(Step s, const double* acc, double* out) {
   __shared__ double red[GT / kWave];
-  const int c = blockIdx.x;
+  __shared__ int flag;
+  const int c = blockIdx.y;
   double lp = 0.0;
   if (s.has_prior) {
     const double iv = (double)s.prior_inv_var, ln = (double)s.prior_log_norm;
     const float* th = s.theta + (int64_t)c * s.P;
-    for (int64_t i = threadIdx.x; i < s.P; i += GT) {
+    for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.P;
+         i += (int64_t)gridDim.x * GT) {
       const double w = (double)th[i];
       lp += (ln + w * w * iv) / -2.0;
     }
   }
   lp = block_sum_d<GT>(lp, red);
-  if (threadIdx.x == 0) out[c] = (-acc[c] - lp) / (double)s.temperature;
+  if (threadIdx.x == 0) publish(s.fin_part + (int64_t)c * gridDim.x + blockIdx.x, lp);
+  if (!arrive_last(s.counters + c, gridDim.x, &flag)) return;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += GT)
+    t += consume(s.fin_part + (int64_t)c * gridDim.x + b);
+  t = block_sum_d<GT>(t, red);
+  if (threadIdx.x == 0) out[c] = (-acc[c] - t) / (double)s.temperature;
 }
 
 // ----------------------------------------------------------------- SGGMC MH ----
@@ -727,7 +883,7 @@ using namespace sgmc::mlp;
 namespace {
 
 struct Layout {
-  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, ke, facc, total;
+  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, ke, facc, col, fin, split, total;
   int nparts, g4_blocks, g5_blocks, small_blocks;
 };
 
@@ -752,6 +908,9 @@ Layout layout_of(int C, int n, int D_in, int H1, int H2) {
   L.prior = o; o = align256(o + (int64_t)C * L.nparts * 4);
   L.ke = o; o = align256(o + (int64_t)C * L.nparts * 4);
   L.facc = o; o = align256(o + (int64_t)C * 8);
+  L.col = o; o = align256(o + (int64_t)C * (n / kStrip) * small_total * 4);
+  L.fin = o; o = align256(o + (int64_t)C * kFinalBlocks * 8);
+  L.split = o; o = align256(o + (int64_t)4 * C * n * std::max(H1, H2) * 4);
   L.total = o;
   return L;
 }
@@ -780,6 +939,42 @@ int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* s
 }  // extern "C"
 
 namespace {
+
+// Deep k-tiles (128) when K allows, so the register prefetch of the next tile has a
+// whole tile of MFMA work to land behind; 32 otherwise (K is a multiple of 32).
+constexpr int kMaxSplits = 4;
+
+template <int BK, bool A_K, bool B_K, int EPI>
+void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part) {
+  g.C = (int)grid.z;
+  g.splits = 1;
+  if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
+    // Split K until the grid holds about four blocks per CU: the forward GEMMs have
+    // few output tiles but long K.
+    const int64_t blocks = (int64_t)grid.x * grid.y * grid.z;
+    const int ktiles = g.K / BK;
+    while (g.splits < kMaxSplits && blocks * g.splits * 2 <= 1024 && ktiles % (g.splits * 2) == 0)
+      g.splits *= 2;
+  }
+  if (g.splits > 1) {
+    g.part = part;
+    hipLaunchKernelGGL((gemm_kernel<BK, A_K, B_K, EPI_PARTIAL>),
+                       dim3(grid.x, grid.y, grid.z * g.splits), dim3(GT), 0, st, g, s);
+    const int64_t quads = (int64_t)g.M * g.N / 4;
+    hipLaunchKernelGGL(splitk_finish_kernel<EPI>, dim3((unsigned)((quads + GT - 1) / GT), g.C),
+                       dim3(GT), 0, st, g);
+  } else {
+    hipLaunchKernelGGL((gemm_kernel<BK, A_K, B_K, EPI>), grid, dim3(GT), 0, st, g, s);
+  }
+}
+
+template <bool A_K, bool B_K, int EPI>
+void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st) {
+  if (g.K % 128 == 0)
+    launch_gemm_bk<128, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  else
+    launch_gemm_bk<32, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+}
 
 // Validates the state and fills the kernels' view of it.
 int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
@@ -831,6 +1026,9 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   s.lik = reinterpret_cast<float*>(w + L.lik);
   s.prior_part = reinterpret_cast<float*>(w + L.prior);
   s.ke_part = reinterpret_cast<float*>(w + L.ke);
+  s.col_part = reinterpret_cast<float*>(w + L.col);
+  s.fin_part = reinterpret_cast<double*>(w + L.fin);
+  s.split_part = reinterpret_cast<float*>(w + L.split);
   s.nparts = L.nparts;
   s.U = q.U; s.grad = q.grad;
   s.samples = q.samples; s.samples_U = q.samples_U; s.n_slots = q.n_slots;
@@ -860,12 +1058,10 @@ void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   uint16_t* sh = q.shadow;
   Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, sh + s.off[0], q.H1, q.P, n, q.H1, q.D_in,
           s.h1, q.H1, (int64_t)n * q.H1, q.theta + s.off[3], q.P, nullptr, 0, 0, 0, 0};
-  hipLaunchKernelGGL((gemm_kernel<true, false, EPI_BIAS_RELU>), dim3(n / BM, q.H1 / BN, C),
-                     dim3(GT), 0, st_, g1, s);
+  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H1 / BN, C), g1, s, st_);
   Gemm g2{s.h1, q.H1, (int64_t)n * q.H1, sh + s.off[1], q.H2, q.P, n, q.H2, q.H1,
           s.h2, q.H2, (int64_t)n * q.H2, q.theta + s.off[4], q.P, nullptr, 0, 0, 0, 0};
-  hipLaunchKernelGGL((gemm_kernel<true, false, EPI_BIAS_RELU>), dim3(n / BM, q.H2 / BN, C),
-                     dim3(GT), 0, st_, g2, s);
+  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H2 / BN, C), g2, s, st_);
   hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
 }
 
@@ -886,8 +1082,7 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   // dA1 = (dA2 W2^T) * (H1 > 0): B[k=b][n=a] = W2[a][b], k-contiguous rows of W2.
   Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, sh + s.off[1], q.H2, q.P, n, q.H1, q.H2,
           s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};
-  hipLaunchKernelGGL((gemm_kernel<true, true, EPI_MASK>), dim3(n / BM, q.H1 / BN, C), dim3(GT),
-                     0, st_, g3, s);
+  launch_gemm<true, true, EPI_MASK>(dim3(n / BM, q.H1 / BN, C), g3, s, st_);
   // gW2 = H1^T dA2: A[m=a][k=i] = H1[i][a] (m-contiguous), B[k=i][n=b] = dA2[i][b].
   Gemm g4{s.h1, q.H1, (int64_t)n * q.H1, s.da2, q.H2, (int64_t)n * q.H2, q.H1, q.H2, n,
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 1, 0};
@@ -895,16 +1090,15 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   Gemm g5{s.Xb, q.D_in, (int64_t)n * q.D_in, s.da1, q.H1, (int64_t)n * q.H1, q.D_in, q.H1, n,
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, L.g4_blocks};
   if (update) {
-    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_UPDATE>), dim3(q.H1 / BM, q.H2 / BN, C),
-                       dim3(GT), 0, st_, g4, s);
-    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_UPDATE>), dim3(q.D_in / BM, q.H1 / BN, C),
-                       dim3(GT), 0, st_, g5, s);
+    launch_gemm<false, false, EPI_UPDATE>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_);
+    launch_gemm<false, false, EPI_UPDATE>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_);
   } else {
-    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_GRAD>), dim3(q.H1 / BM, q.H2 / BN, C),
-                       dim3(GT), 0, st_, g4, s);
-    hipLaunchKernelGGL((gemm_kernel<false, false, EPI_GRAD>), dim3(q.D_in / BM, q.H1 / BN, C),
-                       dim3(GT), 0, st_, g5, s);
+    launch_gemm<false, false, EPI_GRAD>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_);
+    launch_gemm<false, false, EPI_GRAD>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_);
   }
+  hipLaunchKernelGGL(colsum_kernel,
+                     dim3((q.H2 + GT - 1) / GT + (q.H1 + GT - 1) / GT + 1, n / kStrip, C),
+                     dim3(GT), 0, st_, s);
   hipLaunchKernelGGL(small_kernel, dim3(L.small_blocks, C), dim3(GT), 0, st_, s,
                      L.g4_blocks + L.g5_blocks);
   if (update && s.mode == MLP_SGLD && q.samples && q.save_slot) {
@@ -929,7 +1123,7 @@ int sgmc_mlp_full_potential(const sgmc_mlp_state* st, double* out, void* stream)
     launch_forward(s, q, st_);
     hipLaunchKernelGGL(full_acc_kernel, dim3(q.C), dim3(GT), 0, st_, s, acc);
   }
-  hipLaunchKernelGGL(full_final_kernel, dim3(q.C), dim3(GT), 0, st_, s, acc, out);
+  hipLaunchKernelGGL(full_final_kernel, dim3(kFinalBlocks, q.C), dim3(GT), 0, st_, s, acc, out);
   return check_launch("sgmc_mlp_full_potential");
 }
 
