@@ -122,10 +122,12 @@ __device__ __forceinline__ float seg_sum(float v) {
   return v;
 }
 
-// Rows in flight per segment per pipeline stage.
-template <int CPL>
+// Rows in flight per segment per pipeline stage: 4 (2 for D > 512); a stage of RPW-row
+// loads never exceeds one 64-row chunk.
+template <int LPR, int CPL>
 struct Unroll {
-  static constexpr int U = CPL >= 4 ? 2 : 4;
+  static constexpr int U0 = CPL >= 4 ? 2 : 4;
+  static constexpr int U = U0 < LPR ? U0 : LPR;
 };
 
 // Row ids of one 64-row chunk of a wavefront's range: lane l holds the id of row cb + l
@@ -143,7 +145,7 @@ __device__ __forceinline__ int chunk_rows(const RowSource& src, int cb, int r1) 
 
 template <int LPR, int CPL>
 struct RowGroup {
-  static constexpr int U = Unroll<CPL>::U;
+  static constexpr int U = Unroll<LPR, CPL>::U;
   f4 x[U][CPL];
   float y[U];
   bool ok[U];
@@ -179,6 +181,23 @@ __device__ __forceinline__ void load_group(gcfloat* __restrict__ X, int64_t ldx,
   }
 }
 
+// Dot product and accumulation of the row loop with fused multiply-adds.
+__device__ __forceinline__ float dot4_fma(f4 a, float4 b, float t) {
+  t = fmaf(a.x, b.x, t);
+  t = fmaf(a.y, b.y, t);
+  t = fmaf(a.z, b.z, t);
+  return fmaf(a.w, b.w, t);
+}
+
+__device__ __forceinline__ void axpy4_fma(float4& acc, float r, f4 x) {
+  acc.x = fmaf(r, x.x, acc.x);
+  acc.y = fmaf(r, x.y, acc.y);
+  acc.z = fmaf(r, x.z, acc.z);
+  acc.w = fmaf(r, x.w, acc.w);
+}
+
+// One row group: z = x . w per row (segment sums), the likelihood and dlik/dz of row u
+// evaluated once, by lane u of the row's segment, and broadcast; then acc += r x.
 template <int LPR, int CPL>
 __device__ __forceinline__ void process_group(RowGroup<LPR, CPL>& g, int row0, int D4,
                                               const float4 (&th)[CPL], const Lik& L,
@@ -186,17 +205,24 @@ __device__ __forceinline__ void process_group(RowGroup<LPR, CPL>& g, int row0, i
                                               float* lds_lik, int lds_row0) {
   constexpr int RPW = kWave / LPR;
   constexpr int U = RowGroup<LPR, CPL>::U;
+  static_assert(U <= LPR, "one lane per row of the group");
   const int lane = threadIdx.x & (kWave - 1);
   const int seg = lane / LPR;
   const int q = lane % LPR;
-  // chunks past D (clamped loads) and rows outside the data contribute zero
+  // chunks past D (clamped loads) and rows outside the data contribute zero; the
+  // masking is skipped when the whole wavefront's group is valid
+  bool bad = D4 < LPR * CPL;
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
+  for (int u = 0; u < U; ++u) bad = bad || !g.ok[u];
+  if (__any(bad)) {
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const bool valid = g.ok[u] && (q + LPR * k) < D4;
-      const f4 zero = {0.f, 0.f, 0.f, 0.f};
-      g.x[u][k] = valid ? g.x[u][k] : zero;
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const bool valid = g.ok[u] && (q + LPR * k) < D4;
+        const f4 zero = {0.f, 0.f, 0.f, 0.f};
+        g.x[u][k] = valid ? g.x[u][k] : zero;
+      }
     }
   }
   float z[U];
@@ -204,25 +230,39 @@ __device__ __forceinline__ void process_group(RowGroup<LPR, CPL>& g, int row0, i
   for (int u = 0; u < U; ++u) {
     float t = 0.0f;
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) t += dot4(g.x[u][k], th[k]);
-    z[u] = t;
+    for (int k = 0; k < CPL; ++k) t = dot4_fma(g.x[u][k], th[k], t);
+    z[u] = seg_sum<LPR>(t);
+  }
+  // lane q < U of each segment takes row q (bitwise blends: a select chain on q would be
+  // turned into an indexed table in scratch memory)
+  uint32_t zb = __float_as_uint(z[0]), yb = __float_as_uint(g.y[0]);
+  uint32_t okb = g.ok[0] ? 1u : 0u;
+#pragma unroll
+  for (int u = 1; u < U; ++u) {
+    const uint32_t m = 0u - (uint32_t)(q == u);
+    zb = (zb & ~m) | (__float_as_uint(z[u]) & m);
+    yb = (yb & ~m) | (__float_as_uint(g.y[u]) & m);
+    okb = (okb & ~m) | ((g.ok[u] ? 1u : 0u) & m);
+  }
+  const float zq = __uint_as_float(zb), yq = __uint_as_float(yb);
+  const bool okq = okb != 0u;
+  float likq, rq;
+  row_lik(L, zq, yq, likq, rq);
+  if (!okq) {
+    likq = 0.0f;
+    rq = 0.0f;
+  }
+  if (q < U && okq) {
+    lik_acc += likq;
+    lds_lik[row0 + q * RPW + seg - lds_row0] = likq;
   }
 #pragma unroll
-  for (int u = 0; u < U; ++u) z[u] = seg_sum<LPR>(z[u]);
-#pragma unroll
   for (int u = 0; u < U; ++u) {
-    float lik, r;
-    row_lik(L, z[u], g.y[u], lik, r);
-    if (!g.ok[u]) {
-      lik = 0.0f;
-      r = 0.0f;
-    }
+    float r;
+    if (RPW == 1) r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rq), u));
+    else r = __shfl(rq, seg * LPR + u, kWave);
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) axpy4(acc[k], r, g.x[u][k]);
-    if (q == 0 && g.ok[u]) {
-      lik_acc += lik;
-      lds_lik[row0 + u * RPW + seg - lds_row0] = lik;
-    }
+    for (int k = 0; k < CPL; ++k) axpy4_fma(acc[k], r, g.x[u][k]);
   }
 }
 

@@ -263,7 +263,7 @@ class FusedGLM:
     chain state afterwards."""
     if candidates is None:
       base = self.target_bytes
-      candidates = sorted({max(16 * 1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4)})
+      candidates = sorted({max(16 * 1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4, 8, 16)})
     snap = self._snapshot()
     best = None
     for tb in candidates:
