@@ -250,7 +250,7 @@ def run_ladder(args):
                     "target_bytes_per_block": tuned["target_bytes"], "grid_blocks": eng.nblocks},
          "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                      "kernel": "glm_fused_kernel<-1> + swap_kernel per iteration",
+                      "kernel": "glm_fused_kernel<-1> (mixed D classes) + swap_kernel",
                       "bytes_per_iteration": bytes_it},
          "exchanges": int(ex.swaps.sum().item()), "tests": int(ex.step.sum().item())}
   print(json.dumps(out), flush=True)

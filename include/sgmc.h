@@ -195,13 +195,17 @@ typedef struct sgmc_glm_chain {
 int sgmc_glm_dclass(int D);
 /* chains, block_chain (int32[nblocks], chain of each grid block), eps_table,
  * save_slot, iter and done (a zeroed unsigned) are device memory. uniform_dclass is
- * the common dclass of all chains, or -1 for a mixed table. */
+ * the common dclass of all chains, or -1 for a mixed table. With advance_iter != 0 the
+ * last chain to finish the launch's final leapfrog step (step == steps - 1, or every
+ * SGLD launch) advances iter; a table split into several launches per step (one per
+ * dclass) passes advance_iter = 1 on the last of them only. */
 int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,
                         int nblocks, int uniform_dclass, int family, float sigma, int has_prior,
                         float prior_scale, float pot_temperature, int mode, int step, int steps,
                         float friction, float alpha, float lmbd, const float* inv_mass,
                         const float* sqrt_mass, const float* eps_table, int table_len,
-                        const int32_t* save_slot, int32_t* iter, unsigned* done, void* stream);
+                        const int32_t* save_slot, int32_t* iter, unsigned* done,
+                        int advance_iter, void* stream);
 
 /* ---------------------------------------------------- replica exchange ----
  * parallel_tempering's swap step (solver.py:264-293, alias.py:122-207) for pairs of

@@ -110,7 +110,7 @@ _SIGNATURES = {
     "sgmc_glm_fused_step": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_float,
                                     c_int, c_float, c_float, c_int, c_int, c_int, c_float,
                                     c_float, c_float, c_void_p, c_void_p, c_void_p, c_int,
-                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+                                    c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "sgmc_resgld_swap": (c_int, [c_void_p, c_int, c_void_p, c_int, c_float, c_int, c_void_p,
                                  c_int, c_void_p, c_void_p]),
     "sgmc_mlp_full_potential": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -563,6 +563,7 @@ struct FusedArgs {
   const float* sqrt_mass;
   int table_len;
   int nchains;
+  int advance;             // the launch's last chain advances the iteration counter
   int mode;                // SGMC_MODE_*
   int step;                // leapfrog step within the iteration (SGHMC)
   int steps;
@@ -797,7 +798,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     const unsigned prev = atomicAdd(a.done, 1u);
     if (prev == (unsigned)a.nchains - 1) {
       atomicExch(a.done, 0u);
-      if (last_step) atomicAdd(a.iter, 1);
+      if (last_step && a.advance) atomicAdd(a.iter, 1);
     }
   }
 }
@@ -977,7 +978,8 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                         float prior_scale, float pot_temperature, int mode, int step, int steps,
                         float friction, float alpha, float lmbd, const float* inv_mass,
                         const float* sqrt_mass, const float* eps_table, int table_len,
-                        const int32_t* save_slot, int32_t* iter, unsigned* done, void* stream) {
+                        const int32_t* save_slot, int32_t* iter, unsigned* done,
+                        int advance_iter, void* stream) {
   SGMC_REQUIRE(chains && block_chain && eps_table && iter && done, SGMC_E_ARG,
                "sgmc_glm_fused_step: null pointer");
   SGMC_REQUIRE(nchains >= 1 && nblocks >= nchains && table_len >= 1, SGMC_E_ARG,
@@ -1009,6 +1011,7 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   a.L = make_lik(family, sigma);
   a.pr = make_prior(has_prior, prior_scale);
   a.pot_temperature = pot_temperature;
+  a.advance = advance_iter ? 1 : 0;
   hipStream_t s = as_stream(stream);
   dim3 grid(nblocks), block(kNT);
   switch (uniform_dclass) {

@@ -14,7 +14,6 @@ Chains may differ in dataset, N, D (<= 1024), minibatch size and temperature
 results equal the same chain run alone (chains are independent; every chain owns
 its keys and its reduction order is fixed by its own block count).
 """
-import ctypes
 import time
 
 import numpy as np
@@ -182,15 +181,17 @@ class FusedGLM:
     bc = np.repeat(np.arange(self.C, dtype=np.int32), self.nblocks_chain)
     self.block_chain = torch.as_tensor(bc, device=self.device)
     self._arr = arr  # keep the host copy alive for inspection
-
   # ------------------------------------------------------------ launching ----
   def _launch(self, step):
+    # A mixed-D table runs as one launch of the kernel with a block-uniform D-class
+    # switch: per-class launches leave the device underfilled on small problems.
     _lib.call("sgmc_glm_fused_step", _lib.ptr(self.table), self.C, _lib.ptr(self.block_chain),
               self.nblocks, self.uniform_dclass, self.family, self.sigma, self.has_prior,
               self.prior_scale, self.pot_temperature, self.mode, step, self.steps,
               self.friction, self.alpha, self.lmbd, _lib.ptr(self.inv_mass),
               _lib.ptr(self.sqrt_mass), _lib.ptr(self.eps_table), self.table_len,
-              _lib.ptr(self.save_slot), _lib.ptr(self.iter), _lib.ptr(self.done), _lib.stream())
+              _lib.ptr(self.save_slot), _lib.ptr(self.iter), _lib.ptr(self.done), 1,
+              _lib.stream())
 
   def _launch_steps(self):
     for s in range(self.steps):
