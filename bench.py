@@ -284,8 +284,9 @@ def run_mlp(args):
   K, W = args.steps, args.warmup
   G = 10
   eps = scheduler.polynomial_table(1e-4, 1e-5, K + W + 2 * G)
+  prec = args.precision or "bf16"
   eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0, eps, keys=keys, dkeys=dkeys, rms=True,
-                   prior_scale=1.0)
+                   prior_scale=1.0, precision=prec)
   g = eng.capture(G)
   for _ in range(max(1, W // G)):
     g.replay()
@@ -307,7 +308,7 @@ def run_mlp(args):
   out = {"metric": "chain-steps/sec, pSGLD Bayesian MLP (fused bf16 MFMA, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
          "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
-         "vs_baseline": None, "dtype": "bf16",
+         "vs_baseline": None, "dtype": prec,
          "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
          "config": {"workload": "pSGLD (rms_prop 0.9, 1e-5) MLP 3072-512-512-10, N=50000, "
                                 "batch 256, 16 chains", "chains": C, "P": eng.P,
@@ -359,8 +360,9 @@ def run_sggmc_mlp(args):
   dkeys = R.key_words(R.split(R.PRNGKey(2), C))
   K, W = max(1, args.steps), args.warmup
   eps = scheduler.polynomial_table(1e-4, 1e-4, K + W + 2)
+  prec = args.precision or "fp8"
   eng = M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys, prior_scale=1.0,
-                   integrator="sggmc", friction=1.0, steps=steps)
+                   integrator="sggmc", friction=1.0, steps=steps, precision=prec)
   g = eng.capture(1)
   for _ in range(max(1, W)):
     g.replay()
@@ -380,16 +382,17 @@ def run_sggmc_mlp(args):
   full_flops = 2 * MLP_N * macs
   flops = 2 * steps * grad_flops + full_flops
   achieved = C * flops / (ms / 1e3 / K) / 1e12
+  peak = 5000.0 if prec == "fp8" else 2500.0
   out = {"metric": "chain-steps/sec, SGGMC Bayesian MLP (fused MFMA, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
          "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
-         "vs_baseline": None, "dtype": "bf16",
+         "vs_baseline": None, "dtype": prec,
          "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
          "config": {"workload": "SGGMC: OBABO 100 steps per MH test, MLP 3072-1024-1024-10, "
                                 "batch 512, N=50000, float64 full-data potential",
                     "chains": C, "P": eng.P, "batch": n, "N": MLP_N, "obabo_steps": steps},
-         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0,
-                      "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                       "kernel": "all kernels of one SGGMC iteration (GEMM flops)",
                       "flops_per_chain_step": flops},
          "accepted": eng.accepted.tolist()}
@@ -458,6 +461,8 @@ def main():
                   choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp", "sggmc-mlp",
                            "sgld-linreg"])
   ap.add_argument("--chains", type=int, default=0, help="chains of the sggmc-mlp workload")
+  ap.add_argument("--precision", choices=["bf16", "fp8"], default=None,
+                  help="GEMM operands of the MLP workloads (sggmc-mlp default fp8, psgld-mlp bf16)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "

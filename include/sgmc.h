@@ -300,11 +300,23 @@ typedef struct sgmc_mlp_state {
   float* mh_stats;         /* [C][n_slots][3] acceptance ratio, step size, KE_end - KE_start
                               at the saved iterations, or NULL */
   int32_t* accepted;       /* [C] accepted proposals, or NULL */
+  /* precision 1: the GEMMs run on e4m3 (OCP fp8) operands with per-tensor delayed
+   * scaling -- inputs, first hidden layer and the two backward activations, and the
+   * W1 / W2 shadows -- each evaluation's amax setting the next scales (q[c][0..5]
+   * current scales of X, h1, dA2, dA1, W1, W2; q[c][6..7] the scales of the next W1 /
+   * W2 shadows; q[c][8..13] amax bits). precision 0: bf16. */
+  int32_t precision;
+  uint8_t* shadow8;        /* [C][P] e4m3 copy of W1, W2 (precision 1) */
+  float* q;                /* [C][16] scale words (precision 1) */
 } sgmc_mlp_state;
 
 int64_t sgmc_mlp_param_count(int D_in, int H1, int H2);
 int64_t sgmc_mlp_workspace_bytes(int C, int n, int D_in, int H1, int H2);
 int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* stream);
+/* precision 1: sets the weight scales from max |W1|, max |W2| of every chain, writes the
+ * e4m3 shadows and sets the activation scales to 1 (an evaluation in gradient mode then
+ * calibrates them). */
+int sgmc_mlp_init_fp8(const sgmc_mlp_state* st, void* stream);
 int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream);
 /* full_potential (potential.py:219-293) of every chain's current theta: the
  * likelihood of all N rows, n rows per pass, summed in float64 with the float64 log

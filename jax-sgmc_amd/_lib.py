@@ -66,7 +66,7 @@ class MlpState(ctypes.Structure):
       ("decay_table", c_void_p), ("noise_table", c_void_p), ("U_full", c_void_p),
       ("U_prop", c_void_p), ("theta_old", c_void_p), ("mom_old", c_void_p),
       ("U_old", c_void_p), ("mh_key", c_void_p), ("mh_stats", c_void_p),
-      ("accepted", c_void_p),
+      ("accepted", c_void_p), ("precision", c_int32), ("shadow8", c_void_p), ("q", c_void_p),
   ]
 
 
@@ -114,6 +114,7 @@ _SIGNATURES = {
     "sgmc_resgld_swap": (c_int, [c_void_p, c_int, c_void_p, c_int, c_float, c_int, c_void_p,
                                  c_int, c_void_p, c_void_p]),
     "sgmc_mlp_full_potential": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sgmc_mlp_init_fp8": (c_int, [c_void_p, c_void_p]),
     "sgmc_mlp_mh": (c_int, [c_void_p, c_void_p]),
     "sgmc_mlp_param_count": (c_int64, [c_int, c_int, c_int]),
     "sgmc_mlp_workspace_bytes": (c_int64, [c_int] * 5),

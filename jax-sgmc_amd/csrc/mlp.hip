@@ -29,6 +29,8 @@
 // is m- or n-contiguous is transposed while being written to LDS.
 #include <math.h>
 
+#include <type_traits>
+
 #include "sgmc_common.h"
 #include "sgmc_device.h"
 
@@ -53,6 +55,19 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// OCP fp8 e4m3 with a per-tensor scale: q = e4m3(clamp(v * scale, +-448)), v = q / scale.
+__device__ __forceinline__ uint8_t f2q(float v) {
+  v = fminf(fmaxf(v, -448.0f), 448.0f);
+  return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v, v, 0, false) & 0xFFu);
+}
+__device__ __forceinline__ float q2f(uint8_t b) { return __builtin_amdgcn_cvt_f32_fp8((int)b, 0); }
+
+// fp8 tensors and their per-chain scale words: scales [0, 6), the scale the next weight
+// shadows are written with [6, 8), amax bit patterns [8, 14).
+enum Q8 { Q_X = 0, Q_H1 = 1, Q_DA2 = 2, Q_DA1 = 3, Q_W1 = 4, Q_W2 = 5 };
+constexpr int kQWords = 16;
+constexpr float kQTarget = 224.0f;  // scale = kQTarget / amax: half the e4m3 range as headroom
 
 enum Epi { EPI_BIAS_RELU = 0, EPI_MASK = 1, EPI_UPDATE = 2, EPI_GRAD = 3, EPI_PARTIAL = 4 };
 
@@ -110,7 +125,31 @@ struct Step {
   // full-data forward pass (likelihoods of rows row0 .. row0 + n - 1 only)
   int full;
   int64_t row0;
+  // fp8 GEMM operands (Xb, h1, da2, da1 and the W1 / W2 shadows in e4m3)
+  int fp8;
+  float* q;                // [C][kQWords]
+  uint8_t* shadow8;        // [C][P] (W1 and W2 leaves)
 };
+
+// Records max |v| of a wavefront for tensor t of chain c (every lane of the wavefront
+// calls this).
+__device__ __forceinline__ void q_note_amax(const Step& s, int c, int t, float a) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) a = fmaxf(a, __shfl_xor(a, m, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0 && a > 0.0f)
+    atomicMax(reinterpret_cast<unsigned*>(s.q + (int64_t)c * kQWords + 8 + t), __float_as_uint(a));
+}
+
+// Shadow copy of an updated weight: bf16, or e4m3 at the next weight scale (W1, W2).
+__device__ __forceinline__ void write_shadow(const Step& s, int c, int leaf, int64_t base,
+                                             int64_t i, float wn, float& wmax) {
+  if (s.fp8 && leaf < 2) {
+    s.shadow8[base + i] = f2q(wn * s.q[(int64_t)c * kQWords + 6 + leaf]);
+    wmax = fmaxf(wmax, fabsf(wn));
+  } else {
+    s.shadow[base + i] = f2bf(wn);
+  }
+}
 
 enum MlpMode { MLP_SGLD = 0, MLP_OBABO_BA = 1, MLP_OBABO_BO = 2 };
 
@@ -119,9 +158,11 @@ __device__ __forceinline__ int cur_iter(const Step& s) {
 }
 
 // SGLD / rms_prop update of one parameter (integrator.py:882-912, adaption.py:254-291).
-__device__ __forceinline__ void sgld_param(const Step& s, float* th, float* v, uint16_t* sh,
+__device__ __forceinline__ void sgld_param(const Step& s, int c, int leaf, int64_t base,
                                            int64_t i, float gsum, float xi, float eps,
-                                           float noise_scale, float& lp) {
+                                           float noise_scale, float& lp, float& wmax) {
+  float* th = s.theta + base;
+  float* v = s.v ? s.v + base : nullptr;
   const float w = th[i];
   float g = gsum;
   if (s.has_prior) {
@@ -141,7 +182,7 @@ __device__ __forceinline__ void sgld_param(const Step& s, float* th, float* v, u
     wn = w + (sg + sn);
   }
   th[i] = wn;
-  sh[i] = f2bf(wn);
+  write_shadow(s, c, leaf, base, i, wn, wmax);
 }
 
 // Gradient-only variant: grad = (gsum + prior) / T, no update.
@@ -176,8 +217,9 @@ __device__ __forceinline__ Coef coef_of(const Step& s) {
 //       p2 = (-eps/2) g(theta) + p1; theta' = theta + eps p2
 //   BO: p3 = (-eps/2) g(theta') + p2; KE_end += p3 p3 / 2; p' = sqrt(pi) p3 + ... xi(s2)
 // with g the potential's gradient at the evaluation point (prior and temperature in).
-__device__ __forceinline__ void obabo_param(const Step& s, int64_t base, int64_t i, float gsum,
-                                            float xi, const Coef& k, float& lp, float& ke) {
+__device__ __forceinline__ void obabo_param(const Step& s, int c, int leaf, int64_t base,
+                                            int64_t i, float gsum, float xi, const Coef& k,
+                                            float& lp, float& ke, float& wmax) {
   float* th = s.theta + base;
   float* mom = s.mom + base;
   const float w = th[i];
@@ -195,7 +237,7 @@ __device__ __forceinline__ void obabo_param(const Step& s, int64_t base, int64_t
     mom[i] = p2;
     const float wn = w + k.eps * p2;
     th[i] = wn;
-    s.shadow[base + i] = f2bf(wn);
+    write_shadow(s, c, leaf, base, i, wn, wmax);
   } else {
     const float p3 = (-half) * g + mom[i];
     ke += p3 * p3;
@@ -203,13 +245,13 @@ __device__ __forceinline__ void obabo_param(const Step& s, int64_t base, int64_t
   }
 }
 
-__device__ __forceinline__ void update_param(const Step& s, int64_t base, int64_t i, float gsum,
-                                             float xi, const Coef& k, float& lp, float& ke) {
+__device__ __forceinline__ void update_param(const Step& s, int c, int leaf, int64_t base,
+                                             int64_t i, float gsum, float xi, const Coef& k,
+                                             float& lp, float& ke, float& wmax) {
   if (s.mode == MLP_SGLD)
-    sgld_param(s, s.theta + base, s.v ? s.v + base : nullptr, s.shadow + base, i, gsum, xi,
-               k.eps, k.noise_scale, lp);
+    sgld_param(s, c, leaf, base, i, gsum, xi, k.eps, k.noise_scale, lp, wmax);
   else
-    obabo_param(s, base, i, gsum, xi, k, lp, ke);
+    obabo_param(s, c, leaf, base, i, gsum, xi, k, lp, ke, wmax);
 }
 
 // Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
@@ -248,54 +290,71 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
     if (r < 0 || r >= s.N) r = 0;
   }
   const float* src = s.X + r * s.ldx;
-  uint16_t* dst = s.Xb + ((int64_t)c * s.n + j) * s.D_in;
-  for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
-    float4 x = *reinterpret_cast<const float4*>(src + f);
-    if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t lo = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
-    const uint32_t hi = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
-    *reinterpret_cast<uint2*>(dst + f) = make_uint2(lo, hi);
+  const int64_t row = (int64_t)c * s.n + j;
+  if (s.fp8) {
+    const float qs = s.q[(int64_t)c * kQWords + Q_X];
+    uint8_t* dst = reinterpret_cast<uint8_t*>(s.Xb) + row * s.D_in;
+    float amax = 0.f;
+    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
+      float4 x = *reinterpret_cast<const float4*>(src + f);
+      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
+      amax = fmaxf(amax, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
+      const uint32_t v = (uint32_t)f2q(x.x * qs) | ((uint32_t)f2q(x.y * qs) << 8) |
+                         ((uint32_t)f2q(x.z * qs) << 16) | ((uint32_t)f2q(x.w * qs) << 24);
+      *reinterpret_cast<uint32_t*>(dst + f) = v;
+    }
+    q_note_amax(s, c, Q_X, amax);
+  } else {
+    uint16_t* dst = s.Xb + row * s.D_in;
+    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
+      float4 x = *reinterpret_cast<const float4*>(src + f);
+      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint32_t lo = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
+      const uint32_t hi = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
+      *reinterpret_cast<uint2*>(dst + f) = make_uint2(lo, hi);
+    }
   }
-  if (lane == 0) s.lab[(int64_t)c * s.n + j] = valid ? s.labels[r] : -1;
+  if (lane == 0) s.lab[row] = valid ? s.labels[r] : -1;
 }
 
 // ------------------------------------------------------------------ GEMM ------
+// Operands are bf16, or e4m3 (F8) with per-tensor scales: element pointers are byte
+// addresses of 2- or 1-byte elements, strides are in elements.
 struct Gemm {
-  const uint16_t* A;
+  const void* A;
   int64_t lda, sA;   // sA: chain stride in elements
-  const uint16_t* B;
+  const void* B;
   int64_t ldb, sB;
   int M, N, K;
-  uint16_t* out;     // EPI_BIAS_RELU / EPI_MASK
+  void* out;         // EPI_BIAS_RELU / EPI_MASK
   int64_t ldo, sO;
   const float* bias; // chain-strided by sBias
   int64_t sBias;
-  const uint16_t* mask;
+  const void* mask;  // EPI_MASK: ReLU activations (same format as the operands)
   int64_t ldm, sM;
   int leaf;          // EPI_UPDATE / EPI_GRAD: parameter leaf of the [M][N] weight
   int part0;         // first prior-partial slot of this GEMM
   int splits;        // split-K factor (grid z = chains x splits)
   float* part;       // EPI_PARTIAL: [splits][C][M][N] float32 partial products
   int C;
+  int ta, tb;        // F8: scale slots of A and B
+  int tout;          // F8: scale slot of an e4m3 output, or -1 for a bf16 output
 };
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 // ---- operand tiles in LDS ----
-// A tile: BM rows (m) x BK (k); B tile: BN rows (n) x BK. Rows are BK bf16 long, made of
-// BK/8 16-byte chunks; chunk c of row r sits at position c ^ swz(r), so that both the
-// MFMA fragment reads (16 consecutive rows, one chunk) and the stores of transposed
-// 8x8 blocks (16 row groups 8 apart, one chunk) touch 16 distinct bank groups.
-template <int BK>
+// A tile: BM rows (m) x BK (k); B tile: BN rows (n) x BK. A row holds BK elements in
+// 16-byte chunks (8 bf16 or 16 e4m3); chunk c of row r sits at position c ^ swz(r), so
+// that the MFMA fragment reads (consecutive rows, one chunk) and the stores of
+// transposed 8x8 blocks (row groups 8 apart) spread over the LDS banks.
+template <int CH>
 __device__ __forceinline__ int swz(int r) {
-  return (r ^ (r >> 3)) & (BK / 8 - 1);
+  return (r ^ (r >> 3)) & (CH - 1);
 }
 
-template <int BK>
-__device__ __forceinline__ int lds_off(int r, int chunk) {
-  return r * BK + ((chunk ^ swz<BK>(r)) << 3);
-}
-
-// 8x8 transpose of bf16: in[i] holds 8 consecutive columns of row i; out[j] holds row
-// i = 0..7 of column j.
+// 8x8 transpose of 16-bit elements: in[i] holds 8 consecutive columns of row i; out[j]
+// holds rows 0..7 of column j.
 __device__ __forceinline__ void transpose8x8(const u32x4 (&in)[8], u32x4 (&out)[8]) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -314,62 +373,140 @@ __device__ __forceinline__ void transpose8x8(const u32x4 (&in)[8], u32x4 (&out)[
   }
 }
 
-// Loads of one operand tile (ROWS x BK at k0) into registers and their stores to LDS.
-// KC: the source is k-contiguous (row r at src + grow(r) * ld); otherwise it is
-// row-contiguous (k-row k at src + k * ld) and each thread moves one 8x8 block,
-// transposed in registers. grow maps tile rows to source rows (A: two bands M/2 apart).
-template <int ROWS, int BK, bool KC>
+// 8x8 transpose of bytes: in[i] holds 8 consecutive columns of row i.
+__device__ __forceinline__ void transpose8x8(const u32x2 (&in)[8], u32x2 (&out)[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t w = (j < 4) ? in[i].x : in[i].y;
+      const uint32_t byte = (w >> (8 * (j & 3))) & 0xFFu;
+      if (i < 4) lo |= byte << (8 * i);
+      else hi |= byte << (8 * (i - 4));
+    }
+    out[j].x = lo;
+    out[j].y = hi;
+  }
+}
+
+// Loads of one operand tile (ROWS x BK elements of EB bytes, at k0) into registers and
+// their stores to LDS. KC: the source is k-contiguous (row r at grow(r) * ld);
+// otherwise it is row-contiguous (k-row k at k * ld) and each thread moves one 8x8
+// block, transposed in registers. grow maps tile rows to source rows.
+template <int ROWS, int BK, bool KC, int EB>
 struct Tile {
-  static constexpr int CH = BK / 8;
-  static constexpr int NR = KC ? ROWS * CH / GT : 8;   // registers per thread
-  static constexpr int BLOCKS = ROWS * BK / 64;        // 8x8 blocks (transposed case)
-  u32x4 r[NR];
+  static constexpr int EPC = 16 / EB;                 // elements per chunk
+  static constexpr int CH = BK / EPC;                 // chunks per row
+  static constexpr int ROWB = BK * EB;                // bytes per LDS row
+  static constexpr int NR = KC ? ROWS * CH / GT : 8;  // registers per thread
+  static constexpr int BLOCKS = ROWS * BK / 64;       // 8x8 blocks (transposed case)
+  using R = typename std::conditional<KC || EB == 2, u32x4, u32x2>::type;
+  static_assert(!KC || (ROWS * CH) % GT == 0, "k-contiguous tile: whole chunks per thread");
+  R r[NR];
+
+  __device__ __forceinline__ static int off(int row, int chunk) {
+    return row * ROWB + ((chunk ^ swz<CH>(row)) << 4);
+  }
 
   template <typename Grow>
-  __device__ __forceinline__ void load(const uint16_t* src, int64_t ld, int k0, Grow grow) {
+  __device__ __forceinline__ void load(const uint8_t* src, int64_t ld, int k0, Grow grow) {
     const int t = threadIdx.x;
     if constexpr (KC) {
 #pragma unroll
       for (int h = 0; h < NR; ++h) {
         const int u = t + h * GT;
         const int row = u / CH, ch = u % CH;
-        r[h] = *reinterpret_cast<const u32x4*>(src + grow(row) * ld + k0 + ch * 8);
+        r[h] = *reinterpret_cast<const u32x4*>(src + (grow(row) * ld + k0 + ch * EPC) * EB);
       }
     } else {
       if (!(BLOCKS >= GT || t < BLOCKS)) return;
       const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
 #pragma unroll
       for (int i = 0; i < 8; ++i)
-        r[i] = *reinterpret_cast<const u32x4*>(src + (int64_t)(k0 + k8 + i) * ld + grow(r8));
+        r[i] = *reinterpret_cast<const R*>(src + ((int64_t)(k0 + k8 + i) * ld + grow(r8)) * EB);
     }
   }
 
-  __device__ __forceinline__ void store(uint16_t* lds) const {
+  __device__ __forceinline__ void store(uint8_t* lds) const {
     const int t = threadIdx.x;
     if constexpr (KC) {
 #pragma unroll
       for (int h = 0; h < NR; ++h) {
         const int u = t + h * GT;
         const int row = u / CH, ch = u % CH;
-        *reinterpret_cast<u32x4*>(lds + lds_off<BK>(row, ch)) = r[h];
+        *reinterpret_cast<u32x4*>(lds + off(row, ch)) = r[h];
       }
     } else {
       if (!(BLOCKS >= GT || t < BLOCKS)) return;
       const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
-      u32x4 o[8];
+      R o[8];
       transpose8x8(r, o);
+      const int kb = k8 * EB;  // byte offset of the block's k range in a row
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        *reinterpret_cast<u32x4*>(lds + lds_off<BK>(r8 + j, k8 / 8)) = o[j];
+        *reinterpret_cast<R*>(lds + off(r8 + j, kb >> 4) + (kb & 15)) = o[j];
     }
   }
 };
 
-template <int BK, bool A_K, bool B_K, int EPI>
+// One 32x32x16 MFMA step: a and b are read at chunk/offset of rows ra and rb.
+template <int BK, bool F8>
+__device__ __forceinline__ void mfma_step(const uint8_t* As, const uint8_t* Bs, int kk, int lane,
+                                          int w, f32x16& acc0, f32x16& acc1) {
+  constexpr int EB = F8 ? 1 : 2;
+  using TA = Tile<BM, BK, true, EB>;
+  const int k = kk + 8 * (lane >> 5);   // this lane's 8 consecutive k
+  const int kb = k * EB;
+  const int ra0 = lane & 31, ra1 = 32 + (lane & 31), rb0 = w * 32 + (lane & 31);
+  const int o0 = TA::off(ra0, kb >> 4) + (kb & 15);
+  const int o1 = TA::off(ra1, kb >> 4) + (kb & 15);
+  const int ob = TA::off(rb0, kb >> 4) + (kb & 15);
+  if constexpr (F8) {
+    const long a0 = *reinterpret_cast<const long*>(As + o0);
+    const long a1 = *reinterpret_cast<const long*>(As + o1);
+    const long b = *reinterpret_cast<const long*>(Bs + ob);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a0, b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a1, b, acc1, 0, 0, 0);
+  } else {
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As + o0);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As + o1);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + ob);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc1, 0, 0, 0);
+  }
+}
+
+// ReLU mask of an activation element (bf16 or e4m3): positive and nonzero.
+template <bool F8>
+__device__ __forceinline__ bool mask_on(const void* m, int64_t i) {
+  if constexpr (F8) {
+    const uint8_t b = reinterpret_cast<const uint8_t*>(m)[i];
+    return b != 0 && !(b & 0x80u);
+  } else {
+    const uint16_t h = reinterpret_cast<const uint16_t*>(m)[i];
+    return h != 0 && !(h & 0x8000u);
+  }
+}
+
+// Writes an activation: bf16, or e4m3 at scale qs (F8 with an e4m3 output).
+__device__ __forceinline__ void put_act(void* out, int64_t i, float v, bool q8, float qs) {
+  if (q8) reinterpret_cast<uint8_t*>(out)[i] = f2q(v * qs);
+  else reinterpret_cast<uint16_t*>(out)[i] = f2bf(v);
+}
+
+// Reads an activation of tensor t (e4m3 tensors in fp8 mode, bf16 otherwise).
+__device__ __forceinline__ float get_act(const Step& s, int c, int t, const void* p, int64_t i) {
+  if (s.fp8) return q2f(reinterpret_cast<const uint8_t*>(p)[i]) / s.q[(int64_t)c * kQWords + t];
+  return bf2f(reinterpret_cast<const uint16_t*>(p)[i]);
+}
+
+template <int BK, bool F8, bool A_K, bool B_K, int EPI>
 __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
   static_assert(BN == 2 * BM, "tile shapes");
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * BK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * BK];
+  constexpr int EB = F8 ? 1 : 2;
+  __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
   __shared__ float red[GT / kWave];
   const int bm = blockIdx.x, bn = blockIdx.y;
   const int c = blockIdx.z / g.splits, sp = blockIdx.z % g.splits;
@@ -383,13 +520,13 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     acc0[j] = 0.f;
     acc1[j] = 0.f;
   }
-  const uint16_t* A = g.A + (int64_t)c * g.sA;
-  const uint16_t* B = g.B + (int64_t)c * g.sB;
+  const uint8_t* A = reinterpret_cast<const uint8_t*>(g.A) + (int64_t)c * g.sA * EB;
+  const uint8_t* B = reinterpret_cast<const uint8_t*>(g.B) + (int64_t)c * g.sB * EB;
   const int hm = g.M / 2;
   auto growA = [&](int r) -> int64_t { return r < 32 ? bm * 32 + r : hm + bm * 32 + (r - 32); };
   auto growB = [&](int r) -> int64_t { return (int64_t)bn * BN + r; };
-  Tile<BM, BK, A_K> ta;
-  Tile<BN, BK, B_K> tb;
+  Tile<BM, BK, A_K, EB> ta;
+  Tile<BN, BK, B_K, EB> tb;
   ta.load(A, g.lda, kb, growA);
   tb.load(B, g.ldb, kb, growB);
   for (int k0 = kb; k0 < ke; k0 += BK) {
@@ -402,17 +539,12 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       tb.load(B, g.ldb, k0 + BK, growB);
     }
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 16) {
-      const int ch = kk / 8 + (lane >> 5);
-      const int ra0 = lane & 31, ra1 = 32 + (lane & 31), rb0 = w * 32 + (lane & 31);
-      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As + lds_off<BK>(ra0, ch));
-      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As + lds_off<BK>(ra1, ch));
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + lds_off<BK>(rb0, ch));
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc1, 0, 0, 0);
-    }
+    for (int kk = 0; kk < BK; kk += 16) mfma_step<BK, F8>(As, Bs, kk, lane, w, acc0, acc1);
   }
 
+  // F8: products carry the operand scales; ab undoes them
+  const float* qc = F8 ? s.q + (int64_t)c * kQWords : nullptr;
+  const float ab = F8 ? 1.0f / (qc[g.ta] * qc[g.tb]) : 1.0f;
   const int half = g.M / 2;
   const int n = bn * BN + w * 32 + (lane & 31);
   if (EPI == EPI_PARTIAL) {
@@ -427,25 +559,29 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       }
     }
   } else if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
-    uint16_t* out = g.out + (int64_t)c * g.sO;
+    const bool q8 = F8 && g.tout >= 0;
+    const float qs = q8 ? qc[g.tout] : 1.0f;
+    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * (q8 ? 1 : 2);
     const float bias = (EPI == EPI_BIAS_RELU) ? g.bias[(int64_t)c * g.sBias + n] : 0.f;
+    float amax = 0.f;
 #pragma unroll
     for (int band = 0; band < 2; ++band) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
         const int m = (band == 0 ? bm * 32 : half + bm * 32) + r;
-        const float a = band == 0 ? acc0[j] : acc1[j];
+        const float a = ab * (band == 0 ? acc0[j] : acc1[j]);
         float vv;
         if (EPI == EPI_BIAS_RELU) {
           vv = fmaxf(a + bias, 0.0f);
         } else {
-          const uint16_t mk = g.mask[(int64_t)c * g.sM + (int64_t)m * g.ldm + n];
-          vv = (mk != 0 && !(mk & 0x8000u)) ? a : 0.0f;
+          vv = mask_on<F8>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n) ? a : 0.0f;
         }
-        out[(int64_t)m * g.ldo + n] = f2bf(vv);
+        amax = fmaxf(amax, fabsf(vv));
+        put_act(out, (int64_t)m * g.ldo + n, vv, q8, qs);
       }
     }
+    if (q8) q_note_amax(s, c, g.tout, amax);
   } else {
     // Parameter epilogue: element (m, n) of the [M][N] leaf at flat i = m N + n, its
     // noise pair i + M N / 2 sits in acc1 at the same lane and register.
@@ -454,7 +590,7 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     const float* th = s.theta + base;
     const uint32_t S = (uint32_t)((int64_t)g.M * g.N);
     const Key lk = leaf_key(s, c, g.leaf);
-    float lp = 0.0f, ke = 0.0f;
+    float lp = 0.0f, ke = 0.0f, wmax = 0.0f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
@@ -463,14 +599,15 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       if (EPI == EPI_UPDATE) {
         uint32_t w0, w1;
         bits_pair(lk, S, (uint32_t)i0, w0, w1);
-        update_param(s, base, i0, acc0[j], bits_to_normal(w0), k, lp, ke);
-        update_param(s, base, i1, acc1[j], bits_to_normal(w1), k, lp, ke);
+        update_param(s, c, g.leaf, base, i0, ab * acc0[j], bits_to_normal(w0), k, lp, ke, wmax);
+        update_param(s, c, g.leaf, base, i1, ab * acc1[j], bits_to_normal(w1), k, lp, ke, wmax);
       } else {
         float* go = s.grad + base;
-        grad_param(s, th, go, i0, acc0[j], lp);
-        grad_param(s, th, go, i1, acc1[j], lp);
+        grad_param(s, th, go, i0, ab * acc0[j], lp);
+        grad_param(s, th, go, i1, ab * acc1[j], lp);
       }
     }
+    if (EPI == EPI_UPDATE && s.fp8 && g.leaf < 2) q_note_amax(s, c, Q_W1 + g.leaf, wmax);
     const int64_t slot = (int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn;
     const float tot = block_sum<GT>(lp, red);
     if (threadIdx.x == 0) s.prior_part[slot] = tot;
@@ -481,39 +618,43 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
   }
 }
 
-// Split-K finish: out = epilogue(sum over splits of the partial products), the splits
-// added in order; four consecutive columns per thread.
-template <int EPI>
-__global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g) {
+// Split-K finish: out = epilogue(ab * sum over splits of the partial products), the
+// splits added in order; four consecutive columns per thread.
+template <int EPI, bool F8>
+__global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
   const int c = blockIdx.y;
   const int64_t e = ((int64_t)blockIdx.x * GT + threadIdx.x) * 4;
-  if (e >= (int64_t)g.M * g.N) return;
-  const int m = (int)(e / g.N), n = (int)(e % g.N);
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int sp = 0; sp < g.splits; ++sp) {
-    const float4 v =
-        *reinterpret_cast<const float4*>(g.part + ((int64_t)sp * g.C + c) * g.M * g.N + e);
-    a.x += v.x;
-    a.y += v.y;
-    a.z += v.z;
-    a.w += v.w;
-  }
-  float r[4] = {a.x, a.y, a.z, a.w};
-  uint16_t o[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    float v;
-    if (EPI == EPI_BIAS_RELU) {
-      v = fmaxf(r[t] + g.bias[(int64_t)c * g.sBias + n + t], 0.0f);
-    } else {
-      const uint16_t mk = g.mask[(int64_t)c * g.sM + (int64_t)m * g.ldm + n + t];
-      v = (mk != 0 && !(mk & 0x8000u)) ? r[t] : 0.0f;
+  const float* qc = F8 ? s.q + (int64_t)c * kQWords : nullptr;
+  const bool q8 = F8 && g.tout >= 0;
+  float amax = 0.f;
+  if (e < (int64_t)g.M * g.N) {
+    const int m = (int)(e / g.N), n = (int)(e % g.N);
+    const float ab = F8 ? 1.0f / (qc[g.ta] * qc[g.tb]) : 1.0f;
+    const float qs = q8 ? qc[g.tout] : 1.0f;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int sp = 0; sp < g.splits; ++sp) {
+      const float4 v =
+          *reinterpret_cast<const float4*>(g.part + ((int64_t)sp * g.C + c) * g.M * g.N + e);
+      a.x += v.x;
+      a.y += v.y;
+      a.z += v.z;
+      a.w += v.w;
     }
-    o[t] = f2bf(v);
+    const float r[4] = {ab * a.x, ab * a.y, ab * a.z, ab * a.w};
+    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * (q8 ? 1 : 2);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float v;
+      if (EPI == EPI_BIAS_RELU) {
+        v = fmaxf(r[t] + g.bias[(int64_t)c * g.sBias + n + t], 0.0f);
+      } else {
+        v = mask_on<F8>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n + t) ? r[t] : 0.0f;
+      }
+      amax = fmaxf(amax, fabsf(v));
+      put_act(out, (int64_t)m * g.ldo + n + t, v, q8, qs);
+    }
   }
-  uint16_t* out = g.out + (int64_t)c * g.sO + (int64_t)m * g.ldo + n;
-  *reinterpret_cast<uint2*>(out) =
-      make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+  if (q8) q_note_amax(s, c, g.tout, amax);
 }
 
 // ------------------------------------------------------------------ head ------
@@ -570,14 +711,19 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) s.dL[((int64_t)c * s.n + j) * NCLS + k] = dl[k];
   }
-  uint16_t* da2 = s.da2 + ((int64_t)c * s.n + j) * s.H2;
+  const int64_t row = (int64_t)c * s.n + j;
+  const float qs = s.fp8 ? s.q[(int64_t)c * kQWords + Q_DA2] : 1.0f;
+  float amax = 0.f;
   for (int a = lane; a < s.H2; a += kWave) {
     float d = 0.f;
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) d += dl[k] * W3[(int64_t)a * NCLS + k];
     const uint16_t hb = h2[a];
-    da2[a] = f2bf((hb != 0 && !(hb & 0x8000u)) ? d : 0.0f);
+    const float v = (hb != 0 && !(hb & 0x8000u)) ? d : 0.0f;
+    amax = fmaxf(amax, fabsf(v));
+    put_act(s.da2, row * s.H2 + a, v, s.fp8 != 0, qs);
   }
+  if (s.fp8) q_note_amax(s, c, Q_DA2, amax);
 }
 
 // ------------------------------------------------------- small-leaf sums -----
@@ -602,7 +748,7 @@ __global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
 #pragma unroll 4
     for (int r = 0; r < kStrip; ++r) {
       const float h = bf2f(s.h2[(r0 + r) * s.H2 + a]);
-      b2 += bf2f(s.da2[(r0 + r) * s.H2 + a]);
+      b2 += get_act(s, c, Q_DA2, s.da2, (r0 + r) * s.H2 + a);
       const float* dl = s.dL + (r0 + r) * NCLS;
 #pragma unroll
       for (int k = 0; k < NCLS; ++k) w3[k] += h * dl[k];
@@ -615,7 +761,7 @@ __global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
     if (i >= s.H1) return;
     float b1 = 0.f;
 #pragma unroll 8
-    for (int r = 0; r < kStrip; ++r) b1 += bf2f(s.da1[(r0 + r) * s.H1 + i]);
+    for (int r = 0; r < kStrip; ++r) b1 += get_act(s, c, Q_DA1, s.da1, (r0 + r) * s.H1 + i);
     out[nW3 + i] = b1;
   } else {
     const int k = threadIdx.x;
@@ -672,7 +818,8 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
     if (s.update) {
       const Key lk = leaf_key(s, c, leaf);
       const float xi = bits_to_normal(bits_word(lk, (uint32_t)s.size[leaf], (uint32_t)i));
-      update_param(s, base, i, gsum, xi, cf, lp, ke);
+      float wmax = 0.f;
+      update_param(s, c, leaf, base, i, gsum, xi, cf, lp, ke, wmax);
     } else {
       grad_param(s, s.theta + base, s.grad + base, i, gsum, lp);
     }
@@ -721,6 +868,25 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
     } else {
       s.U[c] = Uc;
     }
+    if (s.fp8) {
+      // delayed scaling: this evaluation's amax sets the next scales; a weight shadow
+      // written by this update carries the scale it was written with
+      float* qc = s.q + (int64_t)c * kQWords;
+      unsigned* am = reinterpret_cast<unsigned*>(qc + 8);
+      for (int t = Q_X; t <= Q_DA1; ++t) {
+        const float a = __uint_as_float(am[t]);
+        if (a > 0.0f) qc[t] = kQTarget / a;
+        am[t] = 0u;
+      }
+      if (s.update) {
+        for (int l = 0; l < 2; ++l) {
+          qc[Q_W1 + l] = qc[6 + l];
+          const float a = __uint_as_float(am[Q_W1 + l]);
+          if (a > 0.0f) qc[6 + l] = kQTarget / a;
+          am[Q_W1 + l] = 0u;
+        }
+      }
+    }
     if (s.update && !obabo) {
       const int ss = s.save_slot ? s.save_slot[it] : -1;
       if (ss >= 0 && ss < s.n_slots && s.samples_U) s.samples_U[(int64_t)c * s.n_slots + ss] = Uc;
@@ -754,6 +920,52 @@ __global__ void __launch_bounds__(GT) save_kernel(Step s, int it_offset) {
 __global__ void __launch_bounds__(GT) shadow_kernel(const float* th, uint16_t* sh, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < n; i += (int64_t)gridDim.x * GT)
     sh[i] = f2bf(th[i]);
+}
+
+// ------------------------------------------------------------ fp8 set-up ------
+__global__ void __launch_bounds__(GT) q_zero_kernel(float* q, int n) {
+  const int i = blockIdx.x * GT + threadIdx.x;
+  if (i < n) q[i] = 0.0f;
+}
+
+// max |W1|, max |W2| of every chain into the amax words.
+__global__ void __launch_bounds__(GT) w_amax_kernel(Step s) {
+  const int c = blockIdx.y;
+  const float* th = s.theta + (int64_t)c * s.P;
+  float m0 = 0.f, m1 = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.off[2];
+       i += (int64_t)gridDim.x * GT) {
+    const float a = fabsf(th[i]);
+    if (i < s.off[1]) m0 = fmaxf(m0, a);
+    else m1 = fmaxf(m1, a);
+  }
+  q_note_amax(s, c, Q_W1, m0);
+  q_note_amax(s, c, Q_W2, m1);
+}
+
+// Weight scales from the amax words (current and next shadow), activation scales 1.
+__global__ void __launch_bounds__(GT) q_init_kernel(Step s) {
+  const int c = blockIdx.x * GT + threadIdx.x;
+  if (c >= s.C) return;
+  float* qc = s.q + (int64_t)c * kQWords;
+  unsigned* am = reinterpret_cast<unsigned*>(qc + 8);
+  for (int t = Q_X; t <= Q_DA1; ++t) qc[t] = 1.0f;
+  for (int l = 0; l < 2; ++l) {
+    const float a = __uint_as_float(am[Q_W1 + l]);
+    const float sc = a > 0.0f ? kQTarget / a : 1.0f;
+    qc[Q_W1 + l] = sc;
+    qc[6 + l] = sc;
+    am[Q_W1 + l] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(GT) w_quant_kernel(Step s) {
+  const int c = blockIdx.y;
+  const float* th = s.theta + (int64_t)c * s.P;
+  const float* qc = s.q + (int64_t)c * kQWords;
+  for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.off[2];
+       i += (int64_t)gridDim.x * GT)
+    s.shadow8[(int64_t)c * s.P + i] = f2q(th[i] * qc[i < s.off[1] ? Q_W1 : Q_W2]);
 }
 
 // ------------------------------------------------------ full-data potential ----
@@ -844,7 +1056,12 @@ __global__ void __launch_bounds__(GT) mh_kernel(Step s, MH m) {
       w = tho[i];
       th[i] = w;
       mo[i] = moo[i];
-      sh[i] = f2bf(w);
+      if (s.fp8 && i < s.off[2]) {
+        const int l = i < s.off[1] ? 0 : 1;
+        s.shadow8[(int64_t)c * s.P + i] = f2q(w * s.q[(int64_t)c * kQWords + Q_W1 + l]);
+      } else {
+        sh[i] = f2bf(w);
+      }
     }
     if (save) smp[i] = w;
   }
@@ -944,7 +1161,7 @@ namespace {
 // whole tile of MFMA work to land behind; 32 otherwise (K is a multiple of 32).
 constexpr int kMaxSplits = 4;
 
-template <int BK, bool A_K, bool B_K, int EPI>
+template <int BK, bool F8, bool A_K, bool B_K, int EPI>
 void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part) {
   g.C = (int)grid.z;
   g.splits = 1;
@@ -958,22 +1175,35 @@ void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* par
   }
   if (g.splits > 1) {
     g.part = part;
-    hipLaunchKernelGGL((gemm_kernel<BK, A_K, B_K, EPI_PARTIAL>),
+    hipLaunchKernelGGL((gemm_kernel<BK, F8, A_K, B_K, EPI_PARTIAL>),
                        dim3(grid.x, grid.y, grid.z * g.splits), dim3(GT), 0, st, g, s);
     const int64_t quads = (int64_t)g.M * g.N / 4;
-    hipLaunchKernelGGL(splitk_finish_kernel<EPI>, dim3((unsigned)((quads + GT - 1) / GT), g.C),
-                       dim3(GT), 0, st, g);
+    hipLaunchKernelGGL((splitk_finish_kernel<EPI, F8>),
+                       dim3((unsigned)((quads + GT - 1) / GT), g.C), dim3(GT), 0, st, g, s);
   } else {
-    hipLaunchKernelGGL((gemm_kernel<BK, A_K, B_K, EPI>), grid, dim3(GT), 0, st, g, s);
+    hipLaunchKernelGGL((gemm_kernel<BK, F8, A_K, B_K, EPI>), grid, dim3(GT), 0, st, g, s);
   }
 }
 
+// bf16: k-tiles of 128 (or 32); e4m3: 128 (or 64). K is a multiple of 64.
 template <bool A_K, bool B_K, int EPI>
 void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st) {
-  if (g.K % 128 == 0)
-    launch_gemm_bk<128, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
-  else
-    launch_gemm_bk<32, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  const bool deep = g.K % 128 == 0;
+  if (s.fp8) {
+    if (deep) launch_gemm_bk<128, true, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    else launch_gemm_bk<64, true, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  } else {
+    if (deep) launch_gemm_bk<128, false, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    else launch_gemm_bk<32, false, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  }
+}
+
+// Operand / output formats and scale slots of one GEMM.
+Gemm with_q(Gemm g, int ta, int tb, int tout) {
+  g.ta = ta;
+  g.tb = tb;
+  g.tout = tout;
+  return g;
 }
 
 // Validates the state and fills the kernels' view of it.
@@ -1047,6 +1277,11 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   s.decay_table = q.decay_table; s.noise_table = q.noise_table;
   s.full = 0;
   s.row0 = 0;
+  s.fp8 = q.precision == 1 ? 1 : 0;
+  s.q = q.q;
+  s.shadow8 = q.shadow8;
+  if (s.fp8)
+    SGMC_REQUIRE(q.q && q.shadow8, SGMC_E_ARG, "sgmc_mlp: fp8 precision needs q and shadow8");
   return SGMC_OK;
 }
 
@@ -1055,13 +1290,16 @@ void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   const int C = q.C, n = q.n;
   const dim3 rows_grid((n + 3) / 4, C);
   hipLaunchKernelGGL(gather_kernel, rows_grid, dim3(GT), 0, st_, s);
-  uint16_t* sh = q.shadow;
-  Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, sh + s.off[0], q.H1, q.P, n, q.H1, q.D_in,
+  const void* W1 = s.fp8 ? (const void*)(q.shadow8 + s.off[0]) : (const void*)(q.shadow + s.off[0]);
+  const void* W2 = s.fp8 ? (const void*)(q.shadow8 + s.off[1]) : (const void*)(q.shadow + s.off[1]);
+  Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, W1, q.H1, q.P, n, q.H1, q.D_in,
           s.h1, q.H1, (int64_t)n * q.H1, q.theta + s.off[3], q.P, nullptr, 0, 0, 0, 0};
-  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H1 / BN, C), g1, s, st_);
-  Gemm g2{s.h1, q.H1, (int64_t)n * q.H1, sh + s.off[1], q.H2, q.P, n, q.H2, q.H1,
+  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H1 / BN, C), with_q(g1, Q_X, Q_W1, Q_H1),
+                                          s, st_);
+  Gemm g2{s.h1, q.H1, (int64_t)n * q.H1, W2, q.H2, q.P, n, q.H2, q.H1,
           s.h2, q.H2, (int64_t)n * q.H2, q.theta + s.off[4], q.P, nullptr, 0, 0, 0, 0};
-  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H2 / BN, C), g2, s, st_);
+  launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H2 / BN, C), with_q(g2, Q_H1, Q_W2, -1),
+                                          s, st_);
   hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
 }
 
@@ -1078,17 +1316,20 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   hipStream_t st_ = as_stream(stream);
   const int C = q.C, n = q.n;
   launch_forward(s, q, st_);
-  uint16_t* sh = q.shadow;
+  const void* W2 = s.fp8 ? (const void*)(q.shadow8 + s.off[1]) : (const void*)(q.shadow + s.off[1]);
   // dA1 = (dA2 W2^T) * (H1 > 0): B[k=b][n=a] = W2[a][b], k-contiguous rows of W2.
-  Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, sh + s.off[1], q.H2, q.P, n, q.H1, q.H2,
+  Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, W2, q.H2, q.P, n, q.H1, q.H2,
           s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};
-  launch_gemm<true, true, EPI_MASK>(dim3(n / BM, q.H1 / BN, C), g3, s, st_);
+  launch_gemm<true, true, EPI_MASK>(dim3(n / BM, q.H1 / BN, C), with_q(g3, Q_DA2, Q_W2, Q_DA1), s,
+                                    st_);
   // gW2 = H1^T dA2: A[m=a][k=i] = H1[i][a] (m-contiguous), B[k=i][n=b] = dA2[i][b].
   Gemm g4{s.h1, q.H1, (int64_t)n * q.H1, s.da2, q.H2, (int64_t)n * q.H2, q.H1, q.H2, n,
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 1, 0};
   // gW1 = Xb^T dA1.
   Gemm g5{s.Xb, q.D_in, (int64_t)n * q.D_in, s.da1, q.H1, (int64_t)n * q.H1, q.D_in, q.H1, n,
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, L.g4_blocks};
+  g4 = with_q(g4, Q_H1, Q_DA2, -1);
+  g5 = with_q(g5, Q_X, Q_DA1, -1);
   if (update) {
     launch_gemm<false, false, EPI_UPDATE>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_);
     launch_gemm<false, false, EPI_UPDATE>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_);
@@ -1105,6 +1346,21 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
     hipLaunchKernelGGL(save_kernel, dim3(256, C), dim3(GT), 0, st_, s, 1);
   }
   return check_launch("sgmc_mlp_step");
+}
+
+int sgmc_mlp_init_fp8(const sgmc_mlp_state* st, void* stream) {
+  Layout L;
+  Step s;
+  const int rc = make_step(st, 1, L, s);
+  if (rc != SGMC_OK) return rc;
+  SGMC_REQUIRE(s.fp8, SGMC_E_ARG, "sgmc_mlp_init_fp8: state precision is not fp8");
+  hipStream_t st_ = as_stream(stream);
+  const int nq = st->C * kQWords;
+  hipLaunchKernelGGL(q_zero_kernel, dim3((nq + GT - 1) / GT), dim3(GT), 0, st_, s.q, nq);
+  hipLaunchKernelGGL(w_amax_kernel, dim3(512, st->C), dim3(GT), 0, st_, s);
+  hipLaunchKernelGGL(q_init_kernel, dim3((st->C + GT - 1) / GT), dim3(GT), 0, st_, s);
+  hipLaunchKernelGGL(w_quant_kernel, dim3(512, st->C), dim3(GT), 0, st_, s);
+  return check_launch("sgmc_mlp_init_fp8");
 }
 
 int sgmc_mlp_full_potential(const sgmc_mlp_state* st, double* out, void* stream) {

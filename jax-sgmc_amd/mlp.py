@@ -68,12 +68,13 @@ class FusedMLP:
   SGGMC (``integrator="sggmc"``: OBABO proposals with the full-data FP64 MH test).
 
   For SGGMC the keys are the solver's (split into the integrator key and the MH key
-  as solver.py:478-487 does)."""
+  as solver.py:478-487 does). precision "fp8" runs the GEMMs on e4m3 operands with
+  per-tensor delayed scaling (state and updates stay float32)."""
 
   def __init__(self, X, labels, dims, n, theta0, eps_table, keys=None, dkeys=None, rms=False,
                alpha=0.9, lmbd=1e-5, prior_scale=1.0, has_prior=True, temperature=1.0,
                noise_temperature=1.0, save_slots=None, device="cuda", integrator="sgld",
-               friction=1.0, steps=1):
+               friction=1.0, steps=1, precision="bf16"):
     _lib.require_gpu()
     _lib.load()
     dev = torch.device(device)
@@ -93,8 +94,13 @@ class FusedMLP:
     self.theta = torch.as_tensor(th, device=dev).contiguous()
     self.v = torch.ones_like(self.theta) if rms else None
     self.shadow = torch.empty(self.theta.shape, dtype=torch.int16, device=dev)
-    _lib.call("sgmc_mlp_shadow", _lib.ptr(self.theta), _lib.ptr(self.shadow), self.theta.numel(),
-              _lib.stream())
+    if precision not in ("bf16", "fp8"):
+      raise ValueError("precision must be 'bf16' or 'fp8'")
+    self.precision = precision
+    self.shadow8 = self.q = None
+    if precision == "fp8":
+      self.shadow8 = torch.zeros(self.theta.shape, dtype=torch.uint8, device=dev)
+      self.q = torch.zeros((self.C, 16), device=dev)
     kw = np.zeros((self.C, 2), np.uint32) if keys is None else np.asarray(keys, np.uint32)
     dk = np.zeros((self.C, 2), np.uint32) if dkeys is None else np.asarray(dkeys, np.uint32)
     self.keys = torch.as_tensor(kw.view(np.int32).copy(), device=dev)
@@ -142,6 +148,7 @@ class FusedMLP:
       self.mom = torch.zeros_like(self.theta)
       self.ke = torch.zeros((self.C, 2), dtype=torch.float64, device=dev)
       self.U_half = torch.zeros(self.C, device=dev)
+    self.refresh_shadow()
     if integrator == "sggmc":
       # solver.py:478-487: (integrator key, MH key) = split(key)
       from . import random as R
@@ -158,6 +165,25 @@ class FusedMLP:
       if self.n_slots:
         self.mh_stats = torch.zeros((self.C, self.n_slots, 3), device=dev)
       self.full_potential(out=self.U_full)
+
+  def refresh_shadow(self):
+    """Rewrites the GEMM copy of the parameters after theta changed outside the
+    engine (bf16 shadow, or the e4m3 shadows and their scales)."""
+    _lib.call("sgmc_mlp_shadow", _lib.ptr(self.theta), _lib.ptr(self.shadow),
+              self.theta.numel(), _lib.stream())
+    if self.precision == "fp8":
+      st = self._state()
+      _lib.call("sgmc_mlp_init_fp8", ctypes.byref(st), _lib.stream())
+      # two gradient-mode evaluations calibrate the activation scales
+      grad = self.grad
+      if self.grad is None:
+        self.grad = torch.empty_like(self.theta)
+      U = self.U.clone()
+      for _ in range(2):
+        st = self._state()
+        _lib.call("sgmc_mlp_step", ctypes.byref(st), 0, _lib.stream())
+      self.U.copy_(U)
+      self.grad = grad
 
   def _state(self):
     D_in, H1, H2 = self.dims
@@ -190,6 +216,8 @@ class FusedMLP:
     st.theta_old, st.mom_old, st.U_old = opt(self.theta_old), opt(self.mom_old), opt(self.U_old)
     st.mh_key, st.mh_stats, st.accepted = opt(self.mh_key), opt(self.mh_stats), opt(self.accepted)
     st.mode = 0
+    st.precision = 1 if self.precision == "fp8" else 0
+    st.shadow8, st.q = opt(self.shadow8), opt(self.q)
     return st
 
   def iteration(self):
@@ -290,8 +318,7 @@ class MLPPotential(torch.autograd.Function):
   def forward(ctx, theta, engine):
     with torch.no_grad():
       engine.theta.copy_(theta)
-      _lib.call("sgmc_mlp_shadow", _lib.ptr(engine.theta), _lib.ptr(engine.shadow),
-                engine.theta.numel(), _lib.stream())
+      engine.refresh_shadow()
       U, g = engine.potential_and_grad()
     ctx.save_for_backward(g)
     return U

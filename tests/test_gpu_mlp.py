@@ -234,3 +234,44 @@ def test_mlp_sggmc_graph_replay_matches_eager():
   assert torch.equal(a.theta, b.theta) and torch.equal(a.mom, b.mom)
   assert torch.equal(a.U_full, b.U_full) and torch.equal(a.samples, b.samples)
   assert a.iter.item() == 2 == b.iter.item()
+
+
+def test_mlp_fp8_gradient_against_oracle():
+  """e4m3 GEMM operands with per-tensor delayed scaling (after the engine's
+  calibration): potential within 3e-2 and gradient within 0.15 (relative norm) of the
+  float64 oracle on the same minibatch; the noise and update arithmetic stay float32."""
+  print("mlp fp8 gradient", flush=True)
+  X, lab = data()
+  keys, dkeys = [(0, 1), (0, 2)], [(3, 4), (5, 6)]
+  th = np.stack([M.init_params(10 + c, *DIMS) for c in range(2)])
+  eng = M.FusedMLP(torch.as_tensor(X, device=DEV), torch.as_tensor(lab, device=DEV), DIMS, 64,
+                   th, np.full(8, 1e-5, np.float32), keys=keys, dkeys=dkeys, prior_scale=1.0,
+                   precision="fp8")
+  U, g = eng.potential_and_grad()
+  torch.cuda.synchronize()
+  for c in range(2):
+    Ur, gr, _ = oracle_potential(X, lab, th[c], dkeys[c], 64)
+    gk = g[c].cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(gk - gr) / np.linalg.norm(gr)
+    print("chain", c, "U", U[c].item(), Ur, "grad rel", rel, flush=True)
+    assert abs(U[c].item() - Ur) <= 3e-2 * abs(Ur)
+    assert rel < 0.15, rel
+  q = eng.q.cpu().numpy()
+  assert np.all(q[:, :6] > 0) and np.all(np.isfinite(q[:, :8]))
+
+
+def test_mlp_fp8_sggmc_graph_replay_matches_eager():
+  print("mlp fp8 sggmc graph", flush=True)
+  X, lab = data()
+  th = np.stack([M.init_params(60 + c, *DIMS) for c in range(2)])
+  mk = lambda: M.FusedMLP(torch.as_tensor(X, device=DEV), torch.as_tensor(lab, device=DEV), DIMS,
+                          64, th, np.full(4, 1e-3, np.float32), keys=[(0, 1), (0, 2)],
+                          dkeys=[(1, 3), (1, 4)], prior_scale=1.0, integrator="sggmc", steps=2,
+                          save_slots=np.arange(4), precision="fp8")
+  a, b = mk(), mk()
+  a.run(2, graph_iterations=1)
+  b.run(2)
+  torch.cuda.synchronize()
+  assert torch.isfinite(a.theta).all()
+  assert torch.equal(a.theta, b.theta) and torch.equal(a.U_full, b.U_full)
+  assert torch.equal(a.q, b.q)
