@@ -1159,7 +1159,7 @@ namespace {
 
 // Deep k-tiles (128) when K allows, so the register prefetch of the next tile has a
 // whole tile of MFMA work to land behind; 32 otherwise (K is a multiple of 32).
-constexpr int kMaxSplits = 4;
+constexpr int kMaxSplits = 2;
 
 template <int BK, bool F8, bool A_K, bool B_K, int EPI>
 void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part) {
