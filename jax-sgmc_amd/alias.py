@@ -167,6 +167,84 @@ def _run_fused_resgld(potential_fn, data_loader, batch_size, init_samples, itera
   return out
 
 
+_MLP_LEAVES = ("W1", "W2", "W3", "b1", "b2", "b3")
+
+
+def _mlp_eligible(potential_fn, data_loader, init_samples, batch_size, mass=None):
+  """The MLP engine applies to an MLPSoftmax likelihood with a Gaussian (or no) prior
+  over a DeviceNumpyDataLoader, identity mass, samples {"W1".."b3"} with D_in and n
+  multiples of 64 and H1, H2 multiples of 128."""
+  from . import mlp as ML
+  lik = getattr(potential_fn, "likelihood", None)
+  if not (isinstance(potential_fn, PT.StochasticPotential) and isinstance(lik, ML.MLPSoftmax)
+          and isinstance(data_loader, DT.DeviceNumpyDataLoader) and mass is None
+          and PT._prior_spec(potential_fn.prior) is not None):
+    return False
+  X = data_loader.reference_data.get(lik.x)
+  if X is None or X.dim() != 2 or X.dtype != torch.float32:
+    return False
+  for smp in init_samples:
+    if not isinstance(smp, dict) or sorted(smp) != list(_MLP_LEAVES):
+      return False
+    D_in, H1 = tuple(smp["W1"].shape)
+    H2 = smp["W2"].shape[1]
+    if (D_in != X.shape[1] or D_in % 64 or H1 % 128 or H2 % 128 or batch_size % 64 or
+        tuple(smp["W3"].shape) != (H2, ML.NCLS)):
+      return False
+  return True
+
+
+def _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                   integrator, keys, data_keys, to_numpy=True, rms=False, alpha=0.9, lmbd=1e-5,
+                   steps=1, friction=1.0, result_key="likelihood", precision="bf16"):
+  """alias.sgld / obabo / sggmc on the MLP engine (bf16 or fp8 GEMM operands, float32
+  state): every chain advances in the same launches."""
+  from . import mlp as ML
+  init_fn = sched[0]
+  st, _ = init_fn(iterations)
+  eps, temp, keep = init_fn.tables(st)
+  slots = np.full(iterations, -1, np.int32)
+  kept = np.nonzero(keep)[0]
+  slots[kept] = np.arange(kept.size, dtype=np.int32)
+  lik = potential_fn.likelihood
+  X = data_loader.reference_data[lik.x]
+  labels = data_loader.reference_data[lik.y]
+  has_prior, pscale = PT._prior_spec(potential_fn.prior)
+  first = init_samples[0]
+  dims = (int(first["W1"].shape[0]), int(first["W1"].shape[1]), int(first["W2"].shape[1]))
+  th0 = np.stack([np.concatenate([np.asarray(smp[k].detach().cpu() if isinstance(smp[k], torch.Tensor)
+                                             else smp[k], np.float32).reshape(-1)
+                                  for k in _MLP_LEAVES]) for smp in init_samples])
+  C = len(init_samples)
+  kw = [_key_words(keys[c] if keys else None) for c in range(C)]
+  dk = [_key_words(data_keys[c] if data_keys else None) for c in range(C)]
+  eng = ML.FusedMLP(X, labels, dims, batch_size, th0, eps, keys=kw, dkeys=dk, rms=rms,
+                    alpha=alpha, lmbd=lmbd, prior_scale=pscale, has_prior=bool(has_prior),
+                    temperature=potential_fn.temperature,
+                    noise_temperature=float(temp[0]) if temp.size else 1.0,
+                    save_slots=slots if kept.size else None, integrator=integrator,
+                    friction=friction, steps=steps, precision=precision)
+  eng.run(iterations)
+  torch.cuda.synchronize()
+  shapes = ML.leaf_shapes(*dims)
+  out = []
+  for c in range(C):
+    tr = eng.samples[c, :kept.size] if eng.samples is not None else eng.theta[c:c + 1, :0]
+    var, off = {}, 0
+    for k in _MLP_LEAVES:
+      size = int(np.prod(shapes[k]))
+      var[k] = tr[:, off:off + size].reshape((kept.size,) + shapes[k]).clone()
+      off += size
+    Us = eng.samples_U[c, :kept.size] if eng.samples_U is not None else eng.U[:0]
+    val = -Us if result_key == "likelihood" else Us
+    samples = {"variables": var, result_key: val.clone(), "model_state": None}
+    if to_numpy:
+      samples = T.tree_map(lambda x: x.cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                           samples)
+    out.append({"sample_count": int(kept.size), "samples": samples})
+  return out
+
+
 def sgld(potential_fn, data_loader, cache_size=512, batch_size=32, first_step_size=0.05,
          last_step_size=0.001, burn_in=0, accepted_samples=1000, rms_prop=False, alpha=0.9,
          lmbd=1e-5, save_to_numpy=True, progress_bar=True):
@@ -180,7 +258,12 @@ def sgld(potential_fn, data_loader, cache_size=512, batch_size=32, first_step_si
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
-             data_keys=None, fused=None):
+             data_keys=None, fused=None, precision="bf16"):
+    if (fused and init_model_state is None and
+        _mlp_eligible(potential_fn, data_loader, init_samples, batch_size)):
+      return _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations,
+                            sched, "sgld", keys, data_keys, to_numpy=save_to_numpy,
+                            rms=rms_prop, alpha=alpha, lmbd=lmbd, precision=precision)
     if init_model_state is None and _fused_eligible(potential_fn, data_loader, init_samples,
                                                     fused):
       return _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations,
@@ -250,7 +333,16 @@ def obabo(potential_fn, data_loader, cache_size=512, batch_size=32, integration_
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
-             data_keys=None):
+             data_keys=None, fused=None, precision="bf16"):
+    if (fused and init_model_state is None and np.ndim(friction) == 0 and
+        _mlp_eligible(potential_fn, data_loader, init_samples, batch_size, mass)):
+      return _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations,
+                            sched, "obabo", keys, data_keys, to_numpy=save_to_numpy,
+                            steps=integration_steps, friction=float(friction),
+                            result_key="energy", precision=precision)
+    if fused:
+      raise ValueError("fused=True needs an MLPSoftmax potential over a DeviceNumpyDataLoader "
+                       "with identity mass")
     states = []
     for c, s in enumerate(init_samples):
       kw = dict(init_model_state=init_model_state)
@@ -314,8 +406,25 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
   saving = IO.save(to_numpy=True) if save_to_numpy else None
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
-  def run_fn(*init_samples, init_model_state=None, iterations=1000):
-    states = [solver[0](s, init_model_state=init_model_state) for s in init_samples]
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, fused=None,
+             precision="bf16"):
+    from . import mlp as ML
+    if (fused and init_model_state is None and mass is None and
+        isinstance(getattr(full_potential_fn, "likelihood", None), ML.MLPSoftmax) and
+        _mlp_eligible(stochastic_potential_fn, data_loader, init_samples, batch_size)):
+      return _run_fused_mlp(stochastic_potential_fn, data_loader, batch_size, init_samples,
+                            iterations, sched, "sggmc", keys, None, to_numpy=save_to_numpy,
+                            steps=integration_steps, friction=float(friction_coefficient),
+                            result_key="energy", precision=precision)
+    if fused:
+      raise ValueError("fused=True needs MLPSoftmax potentials over a DeviceNumpyDataLoader "
+                       "with identity mass")
+    states = []
+    for c, s in enumerate(init_samples):
+      kw = dict(init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      states.append(solver[0](s, **kw))
     return mcmc(*states, iterations=iterations)
 
   return run_fn

@@ -270,3 +270,39 @@ def test_sgld_posterior_moments_linear_regression():
   rel = np.abs(np.diag(emp) / np.diag(S) - 1)
   # variance estimate of 2560 draws: relative standard error sqrt(2/2560) = 2.8%
   assert np.all(rel < 0.15), rel
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_sgld_mlp_engine_matches_generic_path(precision):
+  """alias.sgld(rms_prop=True) on an MLPSoftmax potential: the MLP engine (fused=True)
+  against the generic float32 torch path with the same keys. The noise and minibatch
+  indices are bit-identical, so the samples differ only through the bf16 / e4m3 GEMM
+  gradients: per leaf, within 5e-2 (bf16) or 0.2 (fp8) of the generic path's whole
+  displacement from the start."""
+  from jax_sgmc_amd import mlp as ML
+  print("alias mlp engine", precision, flush=True)
+  dims = (128, 128, 128)
+  rng = np.random.default_rng(0)
+  N = 1000
+  X = rng.random((N, dims[0]), dtype=np.float32)
+  y = rng.integers(0, 10, N).astype(np.int32)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), ML.MLPSoftmax(), strategy="vmap")
+  run = alias.sgld(pot, loader, cache_size=1, batch_size=64, first_step_size=2e-4,
+                   last_step_size=1e-4, accepted_samples=3, rms_prop=True)
+  flat = ML.init_params(7, *dims)
+  shapes = ML.leaf_shapes(*dims)
+  p0, off = {}, 0
+  for k in ("W1", "W2", "W3", "b1", "b2", "b3"):
+    size = int(np.prod(shapes[k]))
+    p0[k] = torch.as_tensor(flat[off:off + size].reshape(shapes[k]), device=DEV)
+    off += size
+  a = run(p0, iterations=3, fused=True, precision=precision)[0]["samples"]["variables"]
+  b = run(p0, iterations=3, fused=False)[0]["samples"]["variables"]
+  tol = 5e-2 if precision == "bf16" else 0.2
+  for k in p0:
+    start = p0[k].cpu().numpy()[None]
+    d_ref = np.linalg.norm(b[k] - start)
+    d = np.linalg.norm(a[k] - b[k])
+    print(k, d, d_ref, flush=True)
+    assert d <= tol * d_ref + 1e-6, (k, d, d_ref)
