@@ -141,8 +141,13 @@ class FusedGLM:
   def _build_table(self):
     nbs = [blocks_for(int(n), int(d), self.target_bytes) for n, d in zip(self.n, self.D)]
     self.nblocks_chain = np.array(nbs)
-    self.block_begin = np.concatenate([[0], np.cumsum(nbs)]).astype(np.int64)
-    self.nblocks = int(self.block_begin[-1])
+    # Grid order: chains with the most bytes per block first, so that the longest
+    # blocks start in the first wave of the dispatch (longest-processing-time first).
+    per_block = self.n * 4 * ((self.D + 3) // 4 * 4) / np.maximum(1, self.nblocks_chain)
+    order = np.argsort(-per_block, kind="stable")
+    self.block_begin = np.zeros(self.C, np.int64)
+    self.block_begin[order] = np.concatenate([[0], np.cumsum(self.nblocks_chain[order])[:-1]])
+    self.nblocks = int(self.nblocks_chain.sum())
     D4 = (self.D + 3) // 4
     stride = 4 * D4 + 2
     pof = np.concatenate([[0], np.cumsum(self.nblocks_chain * stride)]).astype(np.int64)
@@ -178,7 +183,7 @@ class FusedGLM:
       e.temperature = float(c.get("temperature", 1.0))
     raw = bytes(arr)
     self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
-    bc = np.repeat(np.arange(self.C, dtype=np.int32), self.nblocks_chain)
+    bc = np.repeat(order.astype(np.int32), self.nblocks_chain[order])
     self.block_chain = torch.as_tensor(bc, device=self.device)
     self._arr = arr  # keep the host copy alive for inspection
   # ------------------------------------------------------------ launching ----
