@@ -254,6 +254,71 @@ __device__ __forceinline__ void update_param(const Step& s, int c, int leaf, int
     obabo_param(s, c, leaf, base, i, gsum, xi, k, lp, ke, wmax);
 }
 
+// The same updates on values, so that a thread can issue the loads of all its
+// parameters before the arithmetic: w and aux (the rms_prop second moment, or the
+// OBABO momentum) are loaded, advanced, then stored.
+struct PState {
+  float w, aux;
+};
+
+__device__ __forceinline__ PState load_pstate(const Step& s, int64_t gi) {
+  PState p;
+  p.w = s.theta[gi];
+  p.aux = s.mode == MLP_SGLD ? (s.rms ? s.v[gi] : 0.0f) : s.mom[gi];
+  return p;
+}
+
+__device__ __forceinline__ void step_pstate(const Step& s, PState& p, float gsum, float xi,
+                                            const Coef& k, float& lp, float& ke) {
+  const float w = p.w;
+  float g = gsum;
+  if (s.has_prior) {
+    g = g + w * s.prior_inv_var;
+    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
+  }
+  g = g / s.temperature;
+  if (s.mode == MLP_SGLD) {
+    const float sg = (-k.eps) * g;
+    const float sn = k.noise_scale * xi;
+    if (s.rms) {
+      const float vn = s.alpha * p.aux + (1.0f - s.alpha) * (g * g);
+      p.aux = vn;
+      const float G = 1.0f / (s.lmbd + sqrtf(vn));
+      p.w = w + ((0.0f + G * sg) + sqrtf(G) * sn);
+    } else {
+      p.w = w + (sg + sn);
+    }
+  } else if (s.mode == MLP_OBABO_BA) {
+    const float half = 0.5f * k.eps;
+    const float p1 = k.decay * p.aux + k.noise * xi;
+    ke += p1 * p1;
+    const float p2 = (-half) * g + p1;
+    p.aux = p2;
+    p.w = w + k.eps * p2;
+  } else {
+    const float half = 0.5f * k.eps;
+    const float p3 = (-half) * g + p.aux;
+    ke += p3 * p3;
+    p.aux = k.decay * p3 + k.noise * xi;
+  }
+}
+
+__device__ __forceinline__ void store_pstate(const Step& s, int c, int leaf, int64_t base,
+                                             int64_t i, const PState& p, float& wmax) {
+  const int64_t gi = base + i;
+  if (s.mode == MLP_SGLD) {
+    s.theta[gi] = p.w;
+    if (s.rms) s.v[gi] = p.aux;
+    write_shadow(s, c, leaf, base, i, p.w, wmax);
+  } else if (s.mode == MLP_OBABO_BA) {
+    s.mom[gi] = p.aux;
+    s.theta[gi] = p.w;
+    write_shadow(s, c, leaf, base, i, p.w, wmax);
+  } else {
+    s.mom[gi] = p.aux;
+  }
+}
+
 // Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
 // (key, sub) = split(key) for SGLD; s1 / s2 of (key, s1, s2) = split(key, 3) for the
 // two momentum refreshes of an OBABO step.
@@ -589,22 +654,57 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     const int64_t base = (int64_t)c * s.P + s.off[g.leaf];
     const float* th = s.theta + base;
     const uint32_t S = (uint32_t)((int64_t)g.M * g.N);
-    const Key lk = leaf_key(s, c, g.leaf);
     float lp = 0.0f, ke = 0.0f, wmax = 0.0f;
+    if (EPI == EPI_UPDATE) {
+      const Key lk = leaf_key(s, c, g.leaf);
+      // every load of the thread's 32 parameters first, then the arithmetic and stores
+      PState p0[16], p1[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-      const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-      const int64_t i1 = i0 + (int64_t)half * g.N;
-      if (EPI == EPI_UPDATE) {
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+        p0[j] = load_pstate(s, base + i0);
+        p1[j] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+        const int64_t i1 = i0 + (int64_t)half * g.N;
         uint32_t w0, w1;
         bits_pair(lk, S, (uint32_t)i0, w0, w1);
-        update_param(s, c, g.leaf, base, i0, ab * acc0[j], bits_to_normal(w0), k, lp, ke, wmax);
-        update_param(s, c, g.leaf, base, i1, ab * acc1[j], bits_to_normal(w1), k, lp, ke, wmax);
-      } else {
-        float* go = s.grad + base;
-        grad_param(s, th, go, i0, ab * acc0[j], lp);
-        grad_param(s, th, go, i1, ab * acc1[j], lp);
+        step_pstate(s, p0[j], ab * acc0[j], bits_to_normal(w0), k, lp, ke);
+        step_pstate(s, p1[j], ab * acc1[j], bits_to_normal(w1), k, lp, ke);
+        store_pstate(s, c, g.leaf, base, i0, p0[j], wmax);
+        store_pstate(s, c, g.leaf, base, i1, p1[j], wmax);
+      }
+    } else {
+      float w0v[16], w1v[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+        w0v[j] = th[i0];
+        w1v[j] = th[i0 + (int64_t)half * g.N];
+      }
+      float* go = s.grad + base;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+        const int64_t i1 = i0 + (int64_t)half * g.N;
+        const float gs[2] = {ab * acc0[j], ab * acc1[j]};
+        const float wv[2] = {w0v[j], w1v[j]};
+        const int64_t ii[2] = {i0, i1};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          float gg = gs[h];
+          if (s.has_prior) {
+            gg = gg + wv[h] * s.prior_inv_var;
+            lp += (s.prior_log_norm + (wv[h] * wv[h]) * s.prior_inv_var) / -2.0f;
+          }
+          go[ii[h]] = gg / s.temperature;
+        }
       }
     }
     if (EPI == EPI_UPDATE && s.fp8 && g.leaf < 2) q_note_amax(s, c, Q_W1 + g.leaf, wmax);
