@@ -152,6 +152,23 @@ int sgmc_obabo_ba(float* theta, float* p, const float* grad, const float* inv_ma
                   float eps, void* stream);
 /* obabo second B (integrator.py:248-251): p = (-eps/2) g + p. */
 int sgmc_obabo_b(float* p, const float* grad, int64_t P, float eps, void* stream);
+/* reversible_leapfrog position step (_position_update, integrator.py:383-388):
+ * theta += scale (inv_mass p); scale is eps/2 for the leading half step. */
+int sgmc_amagold_position(float* theta, const float* p, const float* inv_mass, int64_t P,
+                          float scale, void* stream);
+/* reversible_leapfrog._body_fun after the step's gradient (integrator.py:412-453):
+ * (key_out, sub) = split(key_in); noise = sqrt(4 f eps) (sqrt_mass xi(sub));
+ * p' = (((1 - eps f) p + (-eps) g) + noise) / (1 + eps f);
+ * *energy_out = 0.5 eps ((p + p') . (inv_mass g)) + *energy_in.
+ * With theta != NULL the position step that follows is applied too:
+ * theta += next_scale (inv_mass p') (eps before the next step, eps/2 after the last,
+ * integrator.py:401-408, 540-541). workspace: sgmc_obabo_workspace_floats(nblocks)
+ * floats whose counter word starts at zero (the kernel leaves it at zero). */
+int sgmc_amagold_momentum(float* p, const float* grad, const float* inv_mass,
+                          const float* sqrt_mass, const int64_t* leaf_table, int nblocks,
+                          const uint32_t* key_in, uint32_t* key_out, float eps, float friction,
+                          const float* energy_in, float* energy_out, float* theta,
+                          float next_scale, float* workspace, void* stream);
 
 /* --------------------------------------------------- fused multi-chain ----
  * One launch advances every chain of a chain table by one gradient evaluation:

@@ -106,6 +106,11 @@ _SIGNATURES = {
     "sgmc_obabo_ba": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float,
                               c_void_p]),
     "sgmc_obabo_b": (c_int, [c_void_p, c_void_p, c_int64, c_float, c_void_p]),
+    "sgmc_amagold_position": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_float,
+                                      c_void_p]),
+    "sgmc_amagold_momentum": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                      c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p,
+                                      c_void_p, c_float, c_void_p, c_void_p]),
     "sgmc_glm_dclass": (c_int, [c_int]),
     "sgmc_glm_fused_step": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_float,
                                     c_int, c_float, c_float, c_int, c_int, c_int, c_float,

@@ -43,6 +43,23 @@ def _schedule(first_step_size, last_step_size, burn_in, accepted_samples, temper
                            temperature=temperature)
 
 
+def _mh_schedule(adaptive, first_step_size, last_step_size, burn_in, accepted_samples,
+                 stabilization_constant, decay_constant, speed_constant,
+                 target_acceptance_rate):
+  """Schedules of the MH solvers (alias.py:287-311, 410-434): dual-averaging step size
+  during burn-in when ``adaptive`` (no thinning then), the polynomial first/last
+  schedule otherwise."""
+  if not adaptive:
+    return _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  assert accepted_samples is None, ("Thinning currently not supported for"
+                                    " adaptive step size.")
+  step = SC.adaptive_step_size(burn_in=burn_in, initial_step_size=first_step_size,
+                               stabilization_constant=stabilization_constant,
+                               decay_constant=decay_constant, speed_constant=speed_constant,
+                               target_acceptance_rate=target_acceptance_rate)
+  return SC.init_scheduler(step_size=step, burn_in=SC.initial_burn_in(burn_in))
+
+
 def _fused_eligible(potential_fn, data_loader, init_samples, fused):
   if fused is False:
     return False
@@ -395,21 +412,21 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
           decay_constant=0.75, speed_constant=0.05, target_acceptance_rate=0.25, burn_in=0,
           accepted_samples=None, mass=None, save_to_numpy=True, progress_bar=True):
   """alias.py:330-449: OBABO proposals with a full-data Metropolis-Hastings test."""
-  if adaptive_step_size:
-    raise NotImplementedError("adaptive_step_size is not provided by this build")
   random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
   full_map = DT.full_reference_data(data_loader, cache_size, batch_size)
   integ = IG.obabo(stochastic_potential_fn, random_data, integration_steps,
                    friction_coefficient, mass)
   solver = SV.sggmc(integ, full_potential_fn, full_map)
-  sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
+  sched = _mh_schedule(adaptive_step_size, first_step_size, last_step_size, burn_in,
+                       accepted_samples, stabilization_constant, decay_constant,
+                       speed_constant, target_acceptance_rate)
   saving = IO.save(to_numpy=True) if save_to_numpy else None
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, fused=None,
              precision="bf16"):
     from . import mlp as ML
-    if (fused and init_model_state is None and mass is None and
+    if (fused and not adaptive_step_size and init_model_state is None and mass is None and
         isinstance(getattr(full_potential_fn, "likelihood", None), ML.MLPSoftmax) and
         _mlp_eligible(stochastic_potential_fn, data_loader, init_samples, batch_size)):
       return _run_fused_mlp(stochastic_potential_fn, data_loader, batch_size, init_samples,
@@ -418,7 +435,38 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
                             result_key="energy", precision=precision)
     if fused:
       raise ValueError("fused=True needs MLPSoftmax potentials over a DeviceNumpyDataLoader "
-                       "with identity mass")
+                       "with identity mass and a fixed step-size schedule")
+    states = []
+    for c, s in enumerate(init_samples):
+      kw = dict(init_model_state=init_model_state)
+      if keys:
+        kw["key"] = keys[c]
+      states.append(solver[0](s, **kw))
+    return mcmc(*states, iterations=iterations)
+
+  return run_fn
+
+
+def amagold(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=512,
+            batch_size=32, integration_steps=10, friction=0.25, first_step_size=0.001,
+            last_step_size=0.001, adaptive_step_size=False, stabilization_constant=10,
+            decay_constant=0.75, speed_constant=0.05, target_acceptance_rate=0.25, burn_in=0,
+            accepted_samples=None, mass=None, save_to_numpy=True, progress_bar=True):
+  """alias.py:211-327: reversible-leapfrog proposals with the amortized full-data
+  Metropolis-Hastings test (AMAGOLD). The result's samples hold "variables",
+  "energy", "acceptance_ratio" and "step_size"."""
+  random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  full_map = DT.full_reference_data(data_loader, cache_size, batch_size)
+  integ = IG.reversible_leapfrog(stochastic_potential_fn, random_data, integration_steps,
+                                 friction, mass)
+  solver = SV.amagold(integ, full_potential_fn, full_map)
+  sched = _mh_schedule(adaptive_step_size, first_step_size, last_step_size, burn_in,
+                       accepted_samples, stabilization_constant, decay_constant,
+                       speed_constant, target_acceptance_rate)
+  saving = IO.save(to_numpy=True) if save_to_numpy else None
+  mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
+
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None):
     states = []
     for c, s in enumerate(init_samples):
       kw = dict(init_model_state=init_model_state)

@@ -200,6 +200,61 @@ __global__ void __launch_bounds__(kNT) obabo_b_kernel(float* __restrict__ p,
   p[i] = neg_half_eps * grad[i] + p[i];
 }
 
+// reversible_leapfrog position step (integrator.py:383-388): theta += scale (M^-1 p).
+__global__ void __launch_bounds__(kNT) amagold_position_kernel(float* __restrict__ theta,
+                                                               const float* __restrict__ p,
+                                                               const float* __restrict__ inv_mass,
+                                                               int64_t P, float scale) {
+  const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
+  if (i >= P) return;
+  const float pi = p[i];
+  const float s = inv_mass ? inv_mass[i] * pi : pi;
+  theta[i] = theta[i] + scale * s;
+}
+
+// reversible_leapfrog momentum step (integrator.py:412-453) after the gradient g of
+// this step:
+//   (key_out, sub) = split(key_in); noise = c_n (M^{1/2} xi(sub))
+//   p' = r (((1 - eps f) p + (-eps) g) + noise),  r = 1 / (1 + eps f)
+//   energy_out = 0.5 eps ((p + p') . (M^-1 g)) + energy_in
+// and, when theta is given, the position step that follows it:
+//   theta += next_scale (M^-1 p')  (eps before the next step, eps/2 after the last).
+__global__ void __launch_bounds__(kNT) amagold_momentum_kernel(
+    float* __restrict__ p, const float* __restrict__ grad, const float* __restrict__ inv_mass,
+    const float* __restrict__ sqrt_mass, const int64_t* __restrict__ table,
+    const uint32_t* __restrict__ key_in, uint32_t* __restrict__ key_out, float decay,
+    float neg_eps, float noise_scale, float rescale, float half_eps,
+    const float* __restrict__ energy_in, float* __restrict__ energy_out,
+    float* __restrict__ theta, float next_scale, float* __restrict__ partial,
+    unsigned* __restrict__ counter) {
+  __shared__ float lds[kNT / kWave];
+  __shared__ int flag;
+  Key next, sub;
+  split2(load_key(key_in), next, sub);
+  if (blockIdx.x == 0 && threadIdx.x == 0) store_key(key_out, next);
+  const LeafBlock lb = leaf_block(table, sub);
+  float dot = 0.0f;
+  for_each_noise(lb, [&](int64_t i, float xi) {
+    const float pi = p[i];
+    const float gi = grad[i];
+    const float sm = sqrt_mass ? sqrt_mass[i] : 1.0f;
+    const float im = inv_mass ? inv_mass[i] : 1.0f;
+    const float noise = noise_scale * (sm * xi);
+    const float pn = rescale * ((decay * pi + neg_eps * gi) + noise);
+    p[i] = pn;
+    dot += (pi + pn) * (im * gi);
+    if (theta) theta[i] = theta[i] + next_scale * (im * pn);
+  });
+  const float tot = block_sum<kNT>(dot, lds);
+  if (threadIdx.x == 0) publish(partial + blockIdx.x, tot);
+  if (arrive_last(counter, gridDim.x, &flag)) {
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT) s += consume(partial + b);
+    s = block_sum<kNT>(s, lds);
+    if (threadIdx.x == 0) energy_out[0] = half_eps * s + energy_in[0];
+  }
+}
+
 static inline dim3 flat_grid(int64_t P) { return dim3((unsigned)((P + kNT - 1) / kNT)); }
 
 }  // namespace sgmc
@@ -302,6 +357,42 @@ int sgmc_obabo_b(float* p, const float* grad, int64_t P, float eps, void* stream
   hipLaunchKernelGGL(obabo_b_kernel, flat_grid(P), dim3(kNT), 0, as_stream(stream), p, grad, P,
                      -1.0f * half);
   return check_launch("sgmc_obabo_b");
+}
+
+int sgmc_amagold_position(float* theta, const float* p, const float* inv_mass, int64_t P,
+                          float scale, void* stream) {
+  SGMC_REQUIRE(theta && p, SGMC_E_ARG, "sgmc_amagold_position: null pointer");
+  if (P <= 0) return SGMC_OK;
+  hipLaunchKernelGGL(amagold_position_kernel, flat_grid(P), dim3(kNT), 0, as_stream(stream),
+                     theta, p, inv_mass, P, scale);
+  return check_launch("sgmc_amagold_position");
+}
+
+int sgmc_amagold_momentum(float* p, const float* grad, const float* inv_mass,
+                          const float* sqrt_mass, const int64_t* leaf_table, int nblocks,
+                          const uint32_t* key_in, uint32_t* key_out, float eps, float friction,
+                          const float* energy_in, float* energy_out, float* theta,
+                          float next_scale, float* workspace, void* stream) {
+  SGMC_REQUIRE(p && grad && leaf_table && key_in && key_out && energy_in && energy_out &&
+               workspace, SGMC_E_ARG, "sgmc_amagold_momentum: null pointer");
+  SGMC_REQUIRE(key_in != key_out, SGMC_E_ARG, "sgmc_amagold_momentum: keys must not alias");
+  SGMC_REQUIRE(energy_in != energy_out, SGMC_E_ARG,
+               "sgmc_amagold_momentum: energies must not alias");
+  if (nblocks <= 0) return SGMC_OK;
+  // Coefficients in float32, each product rounded as the reference rounds it.
+  const float ef = eps * friction;
+  const float decay = 1.0f - ef;
+  const float rescale = 1.0f / (1.0f + ef);
+  const float four_f = (float)(4.0 * (double)friction);
+  const float noise_scale = sqrtf(four_f * eps);
+  const float half_eps = 0.5f * eps;
+  float* partial = workspace;
+  unsigned* counter = reinterpret_cast<unsigned*>(workspace + nblocks);
+  hipLaunchKernelGGL(amagold_momentum_kernel, dim3(nblocks), dim3(kNT), 0, as_stream(stream), p,
+                     grad, inv_mass, sqrt_mass, leaf_table, key_in, key_out, decay, -1.0f * eps,
+                     noise_scale, rescale, half_eps, energy_in, energy_out, theta, next_scale,
+                     partial, counter);
+  return check_launch("sgmc_amagold_momentum");
 }
 
 }  // extern "C"

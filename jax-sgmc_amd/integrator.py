@@ -6,13 +6,16 @@ Each constructor returns the reference's triplet:
 - friction_leapfrog(potential_fn, batch_fn, steps, friction, const_mass,
   noise_model) -> (init_fn, integrate, get_fn)  (integrator.py:563-765);
 - obabo(potential_fn, batch_fn, steps, friction, const_mass) -> (init_fn, integrate,
-  get_fn)  (integrator.py:138-346).
+  get_fn)  (integrator.py:138-346);
+- reversible_leapfrog(potential_fn, batch_fn, steps, friction, const_mass) ->
+  (init_fn, integrate, get_fn)  (integrator.py:349-560), AMAGOLD's proposal.
 States are NamedTuples with the reference's fields; parameters are kept as one flat
 float32 device vector (ravel order) plus the pytree structure, and every update
 returns a new state (the input state is not modified).
 """
 from typing import Any, NamedTuple
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -277,6 +280,85 @@ def obabo(potential_fn, batch_fn, steps=10, friction=1.0, const_mass=None):
     return ObaboState(positions=lay.unravel(theta), momentum=lay.unravel(p), potential=U,
                       model_state=ms, data_state=data_state, key=key,
                       kinetic_energy_start=ks.reshape(()), kinetic_energy_end=ke.reshape(()))
+
+  def get_fn(state):
+    return {"variables": state.positions, "energy": state.potential,
+            "model_state": state.model_state}
+
+  return init_fn, integrate, get_fn
+
+
+def reversible_leapfrog(potential_fn, batch_fn, steps=10, friction=0.25, const_mass=None):
+  """AMAGOLD's reversible leapfrog (integrator.py:349-560): half position step,
+  ``steps`` x {position step (skipped at step 0), gradient, momentum
+  p' = ((1 - eps f) p - eps g + sqrt(4 f eps) M^{1/2} xi) / (1 + eps f), energy
+  += 0.5 eps (p + p') . M^-1 g}, final half position step. The momentum is drawn once
+  in init_fn and carried across iterations; the state's ``potential`` is the
+  accumulated energy. Each momentum launch also applies the position step that
+  follows it."""
+  init_data, get_data, _ = batch_fn
+  cmass = init_mass(T.tree_map(lambda x: T.as_tensor(x, "cuda"), const_mass)) \
+      if const_mass else None
+  leaves, _ = T.flatten(friction)
+  if len(leaves) != 1 or (isinstance(leaves[0], torch.Tensor) and leaves[0].numel() != 1):
+    raise NotImplementedError("reversible_leapfrog takes a scalar friction")
+  fr = float(leaves[0])
+  ws_cache = {}
+
+  def init_fn(init_sample, key=None, batch_kwargs=None, init_model_state=None, mass=None):
+    dev = _device_of(init_sample)
+    sample = _to_device_tree(init_sample, dev)
+    lay = T.Layout(sample)
+    _, sqrt_m = _flat_mass(mass if mass else cmass, lay, dev)
+    key = R.PRNGKey(0, dev) if key is None else key
+    table, nb = lay.leaf_table(dev, _lib.PAIRS_PER_BLOCK)
+    p = torch.zeros(lay.size, device=dev)
+    key_out = _new_key(dev)
+    _lib.call("sgmc_sghmc_resample", _lib.ptr(p), _lib.ptr(sqrt_m), _lib.ptr(table), nb,
+              _lib.ptr(key), _lib.ptr(key_out), _lib.stream())
+    return LeapfrogState(positions=sample, momentum=lay.unravel(p),
+                         potential=torch.zeros((), device=dev), model_state=init_model_state,
+                         data_state=init_data(**(batch_kwargs or {})), key=key_out,
+                         extra_fields=None)
+
+  def integrate(state, parameters, mass=None):
+    dev = state.key.device
+    lay = T.Layout(state.positions)
+    inv_m, sqrt_m = _flat_mass(mass if mass else cmass, lay, dev)
+    table, nb = lay.leaf_table(dev, _lib.PAIRS_PER_BLOCK)
+    wkey = (str(dev), nb)
+    if wkey not in ws_cache:
+      ws_cache[wkey] = torch.zeros(int(_lib.query("sgmc_obabo_workspace_floats", nb)),
+                                   device=dev)
+    ws = ws_cache[wkey]
+    theta = lay.ravel(state.positions, dev)
+    p = lay.ravel(state.momentum, dev)
+    eps = float(parameters.step_size)
+    half = float(np.float32(0.5) * np.float32(eps))
+    _lib.call("sgmc_amagold_position", _lib.ptr(theta), _lib.ptr(p), _lib.ptr(inv_m), lay.size,
+              half, _lib.stream())
+    energy = torch.zeros(1, device=dev)
+    data_state, ms = state.data_state, state.model_state
+    keys = (_new_key(dev), _new_key(dev))
+    key = state.key
+    for step in range(int(steps)):
+      data_state, mb = get_data(data_state, information=True)
+      (_, ms), g = potential_fn.value_and_grad(lay.unravel(theta), mb, state=ms)
+      e_out = torch.empty(1, device=dev)
+      k_out = keys[step % 2]
+      nxt = eps if step < int(steps) - 1 else half
+      _lib.call("sgmc_amagold_momentum", _lib.ptr(p), _lib.ptr(g.contiguous()), _lib.ptr(inv_m),
+                _lib.ptr(sqrt_m), _lib.ptr(table), nb, _lib.ptr(key), _lib.ptr(k_out), eps, fr,
+                _lib.ptr(energy), _lib.ptr(e_out), _lib.ptr(theta), nxt, _lib.ptr(ws),
+                _lib.stream())
+      key, energy = k_out, e_out
+    if int(steps) == 0:
+      _lib.call("sgmc_amagold_position", _lib.ptr(theta), _lib.ptr(p), _lib.ptr(inv_m),
+                lay.size, half, _lib.stream())
+      key = state.key.clone()
+    return LeapfrogState(positions=lay.unravel(theta), momentum=lay.unravel(p),
+                         potential=energy.reshape(()), model_state=ms, data_state=data_state,
+                         key=key, extra_fields=state.extra_fields)
 
   def get_fn(state):
     return {"variables": state.positions, "energy": state.potential,

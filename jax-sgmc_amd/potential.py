@@ -187,7 +187,7 @@ class FullPotential:
     data = loader.get_full_data()
     X = data[self.likelihood.x]
     y = data[self.likelihood.y].to(torch.float32).contiguous()
-    w = T.flatten(sample)[0][0].contiguous().to(torch.float32)
+    w = T.as_tensor(T.flatten(sample)[0][0], X.device).to(torch.float32).contiguous()
     N, D = X.shape
     rpb = 1024
     key = (N, rpb)

@@ -21,6 +21,11 @@ static_information = namedtuple("static_information", ["samples_collected"])
 F32 = np.float32
 
 
+def _static_update(state, iteration, **kw):
+  """Update of a schedule fixed at init (tables, constants)."""
+  return state
+
+
 def polynomial_table(first, last, iterations, gamma=0.33):
   """eps_i = a (b + i)^(-gamma) with eps_0 = first and eps_{I-1} = last, float32
   (scheduler.py:512-532)."""
@@ -50,7 +55,7 @@ def polynomial_step_size(a=1.0, b=1.0, gamma=0.33):
     n = np.arange(iterations).astype(np.float32)
     return (F32(a) * np.power(F32(b) + n, F32(-gamma))).astype(np.float32)
 
-  return specific_scheduler(init_fn, lambda s, i, **kw: s, lambda s, i, **kw: s[i])
+  return specific_scheduler(init_fn, _static_update, lambda s, i, **kw: s[i])
 
 
 def polynomial_step_size_first_last(first=1.0, last=1.0, gamma=0.33):
@@ -59,7 +64,44 @@ def polynomial_step_size_first_last(first=1.0, last=1.0, gamma=0.33):
   def init_fn(iterations, first=first, last=last, gamma=gamma):
     return polynomial_table(first, last, iterations, gamma)
 
-  return specific_scheduler(init_fn, lambda s, i, **kw: s, lambda s, i, **kw: s[i])
+  return specific_scheduler(init_fn, _static_update, lambda s, i, **kw: s[i])
+
+
+def adaptive_step_size(burn_in=0, initial_step_size=0.05, stabilization_constant=100,
+                       decay_constant=0.75, speed_constant=0.05, target_acceptance_rate=0.02):
+  """scheduler.py:376-444: dual averaging of log(eps) towards a target acceptance rate
+  during burn-in, in float32. The state is (burn_in, x_bar, h_bar, target, t0, kappa,
+  gamma, mu) with mu = log(10 eps0); update(state, i, acceptance_ratio) with m = i + 1:
+  h = (1 - 1/(m + t0)) h + 1/(m + t0) (target - a); x = mu - sqrt(m)/gamma h;
+  x_bar = (1 - m^-kappa) x_bar + m^-kappa x while i < burn_in; get = exp(x_bar). As in
+  the reference, the update uses the target given to this constructor."""
+
+  def init_fn(iterations, burn_in=burn_in, initial_step_size=initial_step_size,
+              stabilization_constant=stabilization_constant, decay_constant=decay_constant,
+              speed_constant=speed_constant, target_acceptance_rate=target_acceptance_rate):
+    del iterations
+    return (burn_in, np.log(F32(initial_step_size)), F32(0.0), target_acceptance_rate,
+            stabilization_constant, decay_constant, speed_constant,
+            np.log(F32(10 * initial_step_size)))
+
+  def update_fn(state, iteration, acceptance_ratio=0.0, **kw):
+    del kw
+    burn, x_bar, h_bar, alpha, t0, kappa, gamma, mu = state
+    m = iteration + 1
+    w = F32(1.0) / F32(m + t0)
+    h_bar = F32(h_bar * F32(F32(1.0) - w))
+    a = F32(float(acceptance_ratio))
+    h_bar = F32(h_bar + w * F32(F32(target_acceptance_rate) - a))
+    x = F32(mu - F32(np.sqrt(F32(m)) / F32(gamma)) * h_bar)
+    lr = np.power(F32(m), F32(-kappa))
+    new_x_bar = F32(F32(x_bar * F32(F32(1.0) - lr)) + F32(lr * x))
+    x_bar = new_x_bar if iteration < burn else x_bar
+    return burn, x_bar, h_bar, alpha, t0, kappa, gamma, mu
+
+  def get_fn(state, iteration, **kw):
+    return np.exp(F32(state[1]))
+
+  return specific_scheduler(init_fn, update_fn, get_fn)
 
 
 def constant_temperature(tau=1.0):
@@ -131,7 +173,12 @@ def init_scheduler(step_size=None, temperature=None, burn_in=None, thinning=None
 
   def update_fn(state, **kw):
     it, total = state.state
-    return state._replace(state=(it + 1, total))
+    return state._replace(
+        state=(it + 1, total),
+        step_size_state=step_size.update(state.step_size_state, it, **kw),
+        temperature_state=temperature.update(state.temperature_state, it, **kw),
+        burn_in_state=burn_in.update(state.burn_in_state, it, **kw),
+        thinning_state=thinning.update(state.thinning_state, it, **kw))
 
   def get_fn(state, **kw):
     it, _ = state.state
@@ -142,6 +189,9 @@ def init_scheduler(step_size=None, temperature=None, burn_in=None, thinning=None
 
   def tables(state):
     """Whole-run tables (step size, temperature, keep flags) for device-driven loops."""
+    if step_size.update.__qualname__.startswith("adaptive_step_size."):
+      raise NotImplementedError("a device-driven loop needs a step-size table fixed in "
+                                "advance; this step-size schedule adapts while running")
     _, total = state.state
     eps = np.array([step_size.get(state.step_size_state, i) for i in range(total)], np.float32)
     temp = np.array([temperature.get(state.temperature_state, i) for i in range(total)],

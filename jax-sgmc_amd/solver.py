@@ -12,7 +12,10 @@
   swapped otherwise);
 - sggmc(integrator, full_potential_fn, full_data_map) (solver.py:434-577): OBABO
   proposal and a Metropolis-Hastings test on the full-data potential, accumulated in
-  FP64.
+  FP64;
+- amagold(integrator, full_potential_fn, full_data_map) (solver.py:301-432):
+  reversible-leapfrog proposal, the amortized test log a = min(0, U - U' + energy)
+  on the FP64 full-data potential, momentum flipped on rejection.
 """
 import math
 from typing import Any, NamedTuple
@@ -21,6 +24,16 @@ import torch
 
 from . import io as sgio
 from . import random as R
+from .tree import tree_map
+
+
+class AMAGOLDState(NamedTuple):
+  full_data_state: Any
+  potential: Any
+  integrator_state: Any
+  key: Any
+  acceptance_ratio: Any
+  mass_state: Any
 
 
 class SGGMCState(NamedTuple):
@@ -165,6 +178,66 @@ def sggmc(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None):
     d["step_size"] = state.acceptance_ratio[1]
     d["kinetic_energy"] = state.acceptance_ratio[2]
     d["potential"] = state.potential
+    return d
+
+  return init, update, get
+
+
+def amagold(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None):
+  """solver.py:301-432. init splits the key into the integrator key and the MH key and
+  evaluates the full potential at the initial sample; every update splits the MH key,
+  integrates, evaluates the full potential at the proposal and accepts iff
+  log u < min(0, U - U' + energy). A rejected proposal keeps the previous positions,
+  potential and model state and flips the momentum; the integrator key and the data
+  state come from the proposal either way."""
+  if mass_adaption:
+    raise NotImplementedError("mass adaption is not provided by this build")
+  init_i, update_i, get_i = integrator_fn
+  init_full, full_map, _ = full_data_map
+
+  def init(init_sample, key=None, initial_mass=None, full_data_kwargs=None, **kwargs):
+    fds = init_full(**(full_data_kwargs or {}))
+    ims = kwargs.get("init_model_state")
+    U, (fds, new_ms) = full_potential_fn(init_sample, fds, full_map, state=ims)
+    kwargs["init_model_state"] = new_ms
+    key = R.PRNGKey(0) if key is None else key
+    k = R.split(key)
+    ist = init_i(init_sample, key=k[0].clone(), **kwargs)
+    z = torch.zeros((), device=k.device)
+    return AMAGOLDState(full_data_state=fds, potential=U.to(torch.float64),
+                        integrator_state=ist, key=k[1].clone(), acceptance_ratio=(z, z),
+                        mass_state=None)
+
+  def update(state, schedule):
+    k = R.split(state.key)
+    key, sub = k[0].clone(), k[1].clone()
+    prop = update_i(state.integrator_state, schedule, mass=None)
+    U_new, (fds, new_ms) = full_potential_fn(prop.positions, state.full_data_state, full_map,
+                                             state=prop.model_state)
+    U_new = U_new.to(torch.float64)
+    log_alpha = state.potential - U_new + prop.potential.double()
+    log_alpha = torch.where(log_alpha > 0.0, torch.zeros_like(log_alpha), log_alpha)
+    u = R.uniform(sub)
+    accept = bool((torch.log(u.double()) < log_alpha).item())
+    if accept:
+      src, U_keep, ms, mom = prop, U_new, new_ms, prop.momentum
+    else:
+      old = state.integrator_state
+      src, U_keep, ms = old, state.potential, old.model_state
+      mom = tree_map(lambda x: -1.0 * x, old.momentum)
+    new_int = type(prop)(positions=src.positions, momentum=mom, potential=src.potential,
+                         model_state=ms, data_state=prop.data_state, key=prop.key,
+                         extra_fields=prop.extra_fields)
+    ratio = torch.exp(log_alpha)
+    new = AMAGOLDState(full_data_state=fds, potential=U_keep, integrator_state=new_int,
+                       key=key, acceptance_ratio=(ratio, torch.tensor(float(schedule.step_size))),
+                       mass_state=None)
+    return new, {"acceptance_ratio": ratio}
+
+  def get(state):
+    d = get_i(state.integrator_state)
+    d["acceptance_ratio"] = state.acceptance_ratio[0]
+    d["step_size"] = state.acceptance_ratio[1]
     return d
 
   return init, update, get

@@ -276,3 +276,115 @@ def resgld(pot, theta_normal, theta_hot, iterations, eps, T_hot, N, n, leaf_size
       saved_U.append(st[0]["U"])
   return {"normal": st[0], "hot": st[1], "swaps": np.asarray(swaps), "ssq": ssq,
           "samples": np.asarray(saved), "samples_U": np.asarray(saved_U), "key": skey}
+
+
+# --------------------------------------------------------------- AMAGOLD ----
+
+def reversible_leapfrog_init(theta, key, leaf_sizes, sqrt_mass=None):
+  """reversible_leapfrog init_fn (integrator.py:465-501): key, split = split(key);
+  p = M^{1/2} * random_tree(split, theta); the state keeps ``key``."""
+  key, sub = jr.split(np.asarray(key, np.uint32))
+  xi = jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
+  p = xi if sqrt_mass is None else np.asarray(sqrt_mass, np.float64) * xi
+  return p, key
+
+
+def reversible_leapfrog_integrate(pot, theta, p, key, dkey, eps, steps, friction, N, n,
+                                  leaf_sizes, inv_mass=None, sqrt_mass=None, grad_fn=None):
+  """reversible_leapfrog.integrate (integrator.py:503-553) over _body_fun (:397-453):
+  half position step; ``steps`` x {position step (not at step 0), split key, noise
+  sqrt(4 f eps) M^{1/2} xi, minibatch gradient g, p' = ((1 - eps f) p + (-eps) g +
+  noise) / (1 + eps f), energy += 0.5 eps (p + p') . (M^-1 g)}; final half position
+  step. Coefficients are formed in float32 as the reference forms them.
+  ``grad_fn(theta) -> g`` replaces the minibatch gradient (no index draw) when given.
+  Returns theta, p, key, dkey, energy and the last minibatch potential."""
+  P = theta.size
+  inv_mass = np.ones(P) if inv_mass is None else np.asarray(inv_mass, np.float64)
+  sqrt_mass = np.ones(P) if sqrt_mass is None else np.asarray(sqrt_mass, np.float64)
+  e = F32(eps)
+  fr = F32(friction)
+  half = float(F32(F32(0.5) * e))
+  noise_scale = float(np.sqrt(F32(F32(4.0 * friction) * e)))
+  decay = float(F32(F32(1.0) - F32(e * fr)))
+  rescale = float(F32(F32(1.0) / F32(F32(1.0) + F32(e * fr))))
+  theta = theta + half * (inv_mass * p)
+  energy = 0.0
+  U = 0.0
+  for step in range(steps):
+    if step > 0:
+      theta = theta + float(e) * (inv_mass * p)
+    key, sub = jr.split(key)
+    xi = jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
+    noise = noise_scale * (sqrt_mass * xi)
+    if grad_fn is None:
+      dkey, idx = draw_indices(dkey, N, n)
+      U, g, _ = pot(theta, idx)
+    else:
+      g = grad_fn(theta)
+    p_new = rescale * ((decay * p + float(-e) * g) + noise)
+    energy = half * np.dot(p + p_new, inv_mass * g) + energy
+    p = p_new
+  theta = theta + half * (inv_mass * p)
+  return theta, p, key, dkey, energy, U
+
+
+def amagold(pot, full_pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
+            key=None, dkey=None, inv_mass=None, sqrt_mass=None):
+  """solver.amagold (solver.py:301-432) over reversible_leapfrog: init splits the
+  solver key (integrator key, MH key); every iteration splits the MH key, integrates,
+  evaluates the full-data potential at the proposal and accepts iff
+  log u < min(0, U_full - U_full' + energy). A rejected proposal keeps theta and U_full
+  and flips the momentum; keys and the data state advance either way."""
+  key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
+  dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
+  theta = np.asarray(theta0, np.float64).copy()
+  eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
+  U_full = full_pot(theta)
+  ikey, key = jr.split(key)
+  p, ikey = reversible_leapfrog_init(theta, ikey, leaf_sizes, sqrt_mass)
+  out = {"theta": [], "p": [], "accept": [], "log_alpha": [], "U_full": [], "energy": []}
+  for t in range(iterations):
+    key, sub = jr.split(key)
+    th_new, p_new, ikey, dkey, energy, _ = reversible_leapfrog_integrate(
+        pot, theta, p, ikey, dkey, eps[t], steps, friction, N, n, leaf_sizes, inv_mass,
+        sqrt_mass)
+    U_new = full_pot(th_new)
+    log_alpha = min(U_full - U_new + energy, 0.0)
+    u = jr.uniform(sub)
+    acc = float(np.log(np.float64(u))) < log_alpha
+    if acc:
+      theta, p, U_full = th_new, p_new, U_new
+    else:
+      p = -p
+    for k_, val in (("theta", theta), ("p", p), ("accept", acc), ("log_alpha", log_alpha),
+                    ("U_full", U_full), ("energy", energy)):
+      out[k_].append(np.copy(val))
+  res = {k_: np.asarray(v_) for k_, v_ in out.items()}
+  res["key"], res["ikey"], res["dkey"] = key, ikey, dkey
+  return res
+
+
+def adaptive_step_size(acceptance, burn_in, initial_step_size, stabilization_constant,
+                       decay_constant, speed_constant, target_acceptance_rate):
+  """scheduler.adaptive_step_size (scheduler.py:376-444), dual averaging in float32:
+  m = i + 1; h = (1 - 1/(m + t0)) h + 1/(m + t0) (target - a_i);
+  x = mu - sqrt(m)/gamma h; x_bar = (1 - m^-kappa) x_bar + m^-kappa x while i < burn_in;
+  mu = log(10 eps0). Returns the step size used at every iteration (exp(x_bar) before
+  the update of that iteration)."""
+  x_bar = np.log(F32(initial_step_size))
+  h_bar = F32(0.0)
+  mu = np.log(F32(10.0 * initial_step_size))
+  t0 = stabilization_constant
+  out = []
+  for i, a in enumerate(np.asarray(acceptance, np.float32)):
+    out.append(np.exp(F32(x_bar)))
+    m = i + 1
+    w = F32(1.0) / F32(m + t0)
+    h_bar = F32(h_bar * F32(F32(1.0) - w))
+    h_bar = F32(h_bar + w * F32(F32(target_acceptance_rate) - a))
+    x = F32(mu - F32(np.sqrt(F32(m)) / F32(speed_constant)) * h_bar)
+    lr = np.power(F32(m), F32(-decay_constant))
+    xb = F32(F32(x_bar * F32(F32(1.0) - lr)) + F32(lr * x))
+    if i < burn_in:
+      x_bar = xb
+  return np.asarray(out, np.float32)

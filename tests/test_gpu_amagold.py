@@ -1,0 +1,151 @@
+"""AMAGOLD on the GPU (integrator.py:349-560, solver.py:301-432, alias.py:211-327):
+the step kernels against a float32 replay of the reference's op order (keys, noise,
+momentum and positions bit for bit; the energy within float32 summation error), the
+alias trajectory and its accept/reject decisions against the oracle, the reference's
+Kolmogorov-Smirnov fixture (tests/test_alias.py:165-182), and the dual-averaging step
+size (scheduler.py:376-444) on both MH solvers."""
+import numpy as np
+import pytest
+import torch
+from scipy import stats
+
+from oracle import jaxrandom as jr
+from oracle import model as om
+from oracle import steps as ost
+
+import jax_sgmc_amd._lib as L
+from jax_sgmc_amd import alias
+from jax_sgmc_amd import data as DT
+from jax_sgmc_amd import likelihoods as LK
+from jax_sgmc_amd import potential as PT
+from jax_sgmc_amd import random as R
+from jax_sgmc_amd import scheduler as SC
+from jax_sgmc_amd import tree as T
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+f = np.float32
+
+
+def words(t):
+  return t.detach().cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("with_mass", [False, True])
+def test_amagold_step_kernels_exact(with_mass):
+  print("amagold kernels mass", with_mass, flush=True)
+  sizes = [7, 2500]
+  lay = T.Layout([torch.zeros(s) for s in sizes])
+  P = lay.size
+  rng = np.random.default_rng(3)
+  theta = rng.standard_normal(P).astype(f)
+  p0 = rng.standard_normal(P).astype(f)
+  grad = rng.standard_normal(P).astype(f)
+  mass = (0.5 + rng.random(P)).astype(f)
+  inv_m = (f(1) / mass).astype(f) if with_mass else np.ones(P, f)
+  sqrt_m = np.sqrt(mass).astype(f) if with_mass else np.ones(P, f)
+  im = torch.as_tensor(inv_m, device=DEV) if with_mass else None
+  sm = torch.as_tensor(sqrt_m, device=DEV) if with_mass else None
+  table, nb = lay.leaf_table(torch.device(DEV), L.PAIRS_PER_BLOCK)
+  th = torch.as_tensor(theta, device=DEV)
+  p = torch.as_tensor(p0, device=DEV)
+  eps, fr = f(0.004), f(0.3)
+  half = f(f(0.5) * eps)
+  L.call("sgmc_amagold_position", L.ptr(th), L.ptr(p), L.ptr(im), P, float(half), L.stream())
+  th_ref = theta + half * (inv_m * p0)
+  np.testing.assert_array_equal(th.cpu().numpy(), th_ref)
+  ws = torch.zeros(int(L.query("sgmc_obabo_workspace_floats", nb)), device=DEV)
+  k0 = R.key_from_words((21, 22))
+  k1 = torch.empty_like(k0)
+  e_in = torch.tensor([0.75], device=DEV)
+  e_out = torch.empty(1, device=DEV)
+  L.call("sgmc_amagold_momentum", L.ptr(p), L.ptr(torch.as_tensor(grad, device=DEV)), L.ptr(im),
+         L.ptr(sm), L.ptr(table), nb, L.ptr(k0), L.ptr(k1), float(eps), float(fr), L.ptr(e_in),
+         L.ptr(e_out), L.ptr(th), float(eps), L.ptr(ws), L.stream())
+  nxt, sub = jr.split(np.asarray((21, 22), np.uint32))
+  xi = jr.random_tree_flat(sub, sizes)
+  ef = f(eps * fr)
+  noise = np.sqrt(f(f(4.0 * float(fr)) * eps)) * (sqrt_m * xi)
+  p1 = f(f(1) / f(f(1) + ef)) * ((f(f(1) - ef) * p0 + f(-eps) * grad) + noise)
+  np.testing.assert_array_equal(words(k1), nxt)
+  np.testing.assert_array_equal(p.cpu().numpy(), p1)
+  np.testing.assert_array_equal(th.cpu().numpy(), th_ref + eps * (inv_m * p1))
+  e_ref = 0.75 + 0.5 * float(eps) * np.dot((p0 + p1).astype(np.float64),
+                                           (inv_m * grad).astype(np.float64))
+  np.testing.assert_allclose(e_out.item(), e_ref, rtol=2e-5, atol=1e-5)
+  assert ws[nb:nb + 1].view(torch.int32).item() == 0  # counter left at zero
+
+
+def logreg(N=3000, D=24, seed=1):
+  rng = np.random.default_rng(seed)
+  X = (rng.standard_normal((N, D)) / np.sqrt(D)).astype(np.float32)
+  w = rng.standard_normal(D)
+  y = (rng.random(N) < 1 / (1 + np.exp(-X @ w))).astype(np.float32)
+  return X, y
+
+
+def test_amagold_matches_oracle():
+  print("amagold alias parity", flush=True)
+  X, y = logreg()
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  full = PT.full_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  its, steps, fr = 25, 4, 0.5
+  run = alias.amagold(pot, full, loader, cache_size=1, batch_size=128,
+                      integration_steps=steps, friction=fr, first_step_size=0.02,
+                      last_step_size=0.005, accepted_samples=its)
+  res = run(torch.zeros(24), iterations=its)[0]["samples"]
+  eps = SC.polynomial_table(0.02, 0.005, its)
+  fam = om.Family("logistic", X, y)
+  r = ost.amagold(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
+                  lambda th: om.full_potential(fam, th, 1.0, 3000), np.zeros(24), its, eps,
+                  steps, fr, 3000, 128, [24])
+  ratio = np.asarray(res["acceptance_ratio"], np.float64)
+  acc_ref = r["accept"]
+  assert acc_ref.any() and (~acc_ref).any(), "the case should exercise both branches"
+  np.testing.assert_allclose(ratio, np.exp(r["log_alpha"]), rtol=1e-3, atol=1e-4)
+  got = np.asarray(res["variables"], np.float64)
+  ref = r["theta"]
+  tol = 1e-5 * np.abs(ref) + 1e-5 * np.max(np.abs(ref))
+  assert np.all(np.abs(got - ref) <= tol), np.max(np.abs(got - ref))
+  kept, e_ref = 0.0, []
+  for t in range(its):
+    kept = r["energy"][t] if acc_ref[t] else kept
+    e_ref.append(kept)
+  np.testing.assert_allclose(np.asarray(res["energy"], np.float64), e_ref, rtol=1e-3,
+                             atol=1e-3)
+
+
+def test_ks_amagold():
+  """tests/test_alias.py:165-182 with the 1-D Gaussian target of :32-65."""
+  sigma = 0.5
+  x = sigma * jr.normal(jr.PRNGKey(11), (100,))
+  loader = DT.DeviceNumpyDataLoader(x=np.zeros((2, 1), np.float32))
+
+  def lik(sample, _):
+    return -0.5 * (sample / sigma) ** 2
+
+  pot = PT.minibatch_potential(lambda s: torch.zeros((), device=DEV), lik, strategy="vmap")
+  full = PT.full_potential(lambda s: torch.zeros((), device=DEV), lik, strategy="vmap")
+  run = alias.amagold(pot, full, loader, cache_size=1, batch_size=1, first_step_size=0.5,
+                      last_step_size=0.1, burn_in=100)
+  sampled = run(torch.tensor(float(x[0]), device=DEV), iterations=200)[0]["samples"]
+  pval = stats.kstest(np.asarray(sampled["variables"]).reshape(-1), x).pvalue
+  assert pval > 0.05, f"KS p-value {pval}"
+
+
+@pytest.mark.parametrize("solver", ["amagold", "sggmc"])
+def test_adaptive_step_size_freezes_after_burn_in(solver):
+  print("adaptive step size", solver, flush=True)
+  X, y = logreg(N=800, D=8, seed=4)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  full = PT.full_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  make = alias.amagold if solver == "amagold" else alias.sggmc
+  run = make(pot, full, loader, cache_size=1, batch_size=64, integration_steps=3,
+             first_step_size=0.05, adaptive_step_size=True, burn_in=20,
+             target_acceptance_rate=0.25)
+  res = run(torch.zeros(8), iterations=30)[0]["samples"]
+  steps = np.asarray(res["step_size"], np.float32)
+  assert len(steps) == 10
+  assert np.all(steps == steps[0]) and steps[0] != np.float32(0.05)

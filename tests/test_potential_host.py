@@ -161,3 +161,27 @@ def test_random_reference_data_argument_checks():
     DT.random_reference_data(loader, 1, 6)
   with pytest.raises(ValueError):
     DT.random_reference_data(loader, 0, 2)
+
+
+def test_adaptive_step_size_scheduler_matches_oracle():
+  """scheduler.adaptive_step_size driven through init_scheduler with the solver's
+  acceptance ratios reproduces the oracle's dual averaging (scheduler.py:376-444)."""
+  from oracle import steps as ost
+  from jax_sgmc_amd import scheduler as SC
+  rng = np.random.default_rng(9)
+  acc = rng.random(60).astype(np.float32)
+  init, nxt, get = SC.init_scheduler(
+      step_size=SC.adaptive_step_size(burn_in=40, initial_step_size=0.03,
+                                      stabilization_constant=10, decay_constant=0.75,
+                                      speed_constant=0.05, target_acceptance_rate=0.25),
+      burn_in=SC.initial_burn_in(40))
+  st, _ = init(60)
+  got = []
+  for a in acc:
+    got.append(get(st).step_size)
+    st = nxt(st, acceptance_ratio=a)
+  ref = ost.adaptive_step_size(acc, 40, 0.03, 10, 0.75, 0.05, 0.25)
+  np.testing.assert_array_equal(np.asarray(got, np.float32), ref)
+  import pytest
+  with pytest.raises(NotImplementedError):
+    init.tables(st)
