@@ -41,6 +41,7 @@ extern "C" {
 #define SGMC_MODE_SGLD 0
 #define SGMC_MODE_SGLD_RMS 1
 #define SGMC_MODE_SGHMC 2
+#define SGMC_MODE_AMAGOLD 3
 
 const char* sgmc_last_error(void);
 int sgmc_abi_version(void);
@@ -107,6 +108,20 @@ int sgmc_glm_full_potential(int family, const float* X, int64_t ldx, const float
                             float prior_scale, double temperature, double* out_U,
                             double* workspace, int64_t workspace_doubles, int rows_per_block,
                             void* stream);
+
+/* full_potential (potential.py:219-293) of nchains chains sharing one dataset, chain c's
+ * position at thetas + c * ld_theta: out_U[c] = (-sum_i lik_i(theta_c) - log p) / T,
+ * each sum over rows accumulated in FP64. The row dot products of all chains run as one
+ * fp32 matrix product, reading X once per 64 chains. nblocks: grid size (one block per
+ * compute unit is the intended shape); workspace:
+ * sgmc_glm_full_chains_workspace_doubles(nblocks) doubles whose counter word starts at
+ * zero (left at zero). */
+int64_t sgmc_glm_full_chains_workspace_doubles(int nblocks);
+int sgmc_glm_full_potential_chains(int family, const float* X, int64_t ldx, const float* y,
+                                   int64_t N, int D, const float* thetas, int64_t ld_theta,
+                                   int nchains, float sigma, int has_prior, float prior_scale,
+                                   double temperature, double* out_U, double* workspace,
+                                   int64_t workspace_doubles, int nblocks, void* stream);
 
 /* ------------------------------------------------------ step updates ----
  * Elementwise integrator updates over a flattened pytree; noise drawn in registers
@@ -176,7 +191,13 @@ int sgmc_amagold_momentum(float* p, const float* grad, const float* inv_mass,
  * backward, cross-block reduction, and the integrator update with in-register noise:
  *   SGMC_MODE_SGLD / _SGLD_RMS: one launch per iteration (integrator.py:859-922);
  *   SGMC_MODE_SGHMC: `steps` launches per iteration, step = 0..steps-1; step 0 also
- *   resamples the momentum (integrator.py:715-757 with _body_fun 603-666).
+ *   resamples the momentum (integrator.py:715-757 with _body_fun 603-666);
+ *   SGMC_MODE_AMAGOLD: `steps` launches of reversible_leapfrog (integrator.py:503-553,
+ *   _body_fun 397-453): the evaluation point is theta + eps/2 M^-1 p at step 0 and
+ *   theta + eps M^-1 p after; the momentum is carried (drawn once at init), energy
+ *   accumulates 0.5 eps (p + p') . M^-1 g, step 0 keeps theta_old / mom_old and the
+ *   last step adds the final half position step. Samples are saved by
+ *   sgmc_glm_amagold_mh.
  * It replaces solver.mcmc's per-chain sequential loop (solver.py:165-171, strategy
  * 'map') for chains that share a likelihood family. The step size of iteration t is
  * eps_table[t] with t = *iter (a device counter the launch advances after the last
@@ -207,6 +228,9 @@ typedef struct sgmc_glm_chain {
   int32_t nblocks;     /* grid blocks of this chain */
   int32_t dclass;      /* sgmc_glm_dclass(D) */
   float temperature;   /* schedule temperature of this chain (SGLD noise) */
+  float* energy;       /* [1] AMAGOLD: energy accumulated over the iteration, or NULL */
+  float* theta_old;    /* [D] AMAGOLD: position at the start of the iteration, or NULL */
+  float* mom_old;      /* [D] AMAGOLD: momentum at the start of the iteration, or NULL */
 } sgmc_glm_chain;
 
 int sgmc_glm_dclass(int D);
@@ -223,6 +247,20 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                         const float* sqrt_mass, const float* eps_table, int table_len,
                         const int32_t* save_slot, int32_t* iter, unsigned* done,
                         int advance_iter, void* stream);
+
+/* AMAGOLD's amortized Metropolis-Hastings test (solver.py:366-419) for every chain of
+ * a fused chain table stepped with SGMC_MODE_AMAGOLD, after the iteration's last step
+ * (it = iter[0] - 1) and after U_prop[c], the full potential of chain c's proposal
+ * (sgmc_glm_full_potential_chains): (key', sub) = split(keys[c]); u = uniform(sub);
+ * log a = min(0, U_full[c] - U_prop[c] + energy) in FP64; accept iff log u < log a.
+ * Accepted: U_full[c] = U_prop[c], energy_kept[c] = energy. Rejected: theta and mom
+ * return to theta_old and -mom_old. At save_slot[it] >= 0 the chain's samples get the
+ * post-test theta, samples_U the kept energy and ratio_samples[c][slot] exp(log a)
+ * (ratio_samples may be NULL). */
+int sgmc_glm_amagold_mh(const sgmc_glm_chain* chains, int nchains, const double* U_prop,
+                        double* U_full, float* energy_kept, uint32_t* keys, float* ratio_samples,
+                        int n_slots, const int32_t* save_slot, int table_len,
+                        const int32_t* iter, void* stream);
 
 /* ---------------------------------------------------- replica exchange ----
  * parallel_tempering's swap step (solver.py:264-293, alias.py:122-207) for pairs of

@@ -27,6 +27,7 @@ LIK_LOGISTIC = 1
 MODE_SGLD = 0
 MODE_SGLD_RMS = 1
 MODE_SGHMC = 2
+MODE_AMAGOLD = 3
 
 
 class GlmChain(ctypes.Structure):
@@ -37,7 +38,8 @@ class GlmChain(ctypes.Structure):
       ("var", c_void_p), ("lik", c_void_p), ("partial", c_void_p), ("counter", c_void_p),
       ("samples", c_void_p), ("samples_U", c_void_p), ("N", c_int64), ("ldx", c_int64), ("D", c_int32),
       ("n", c_int32), ("block_begin", c_int32), ("nblocks", c_int32), ("dclass", c_int32),
-      ("temperature", c_float),
+      ("temperature", c_float), ("energy", c_void_p), ("theta_old", c_void_p),
+      ("mom_old", c_void_p),
   ]
 
 
@@ -92,6 +94,13 @@ _SIGNATURES = {
     "sgmc_glm_full_potential": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int,
                                         c_void_p, c_float, c_int, c_float, c_double, c_void_p,
                                         c_void_p, c_int64, c_int, c_void_p]),
+    "sgmc_glm_full_chains_workspace_doubles": (c_int64, [c_int]),
+    "sgmc_glm_full_potential_chains": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                               c_int, c_void_p, c_int64, c_int, c_float, c_int,
+                                               c_float, c_double, c_void_p, c_void_p, c_int64,
+                                               c_int, c_void_p]),
+    "sgmc_glm_amagold_mh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "sgmc_sgld_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int, c_void_p,
                                c_void_p, c_float, c_float, c_float, c_float, c_void_p]),
     "sgmc_sghmc_resample": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,

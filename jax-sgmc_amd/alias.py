@@ -184,6 +184,72 @@ def _run_fused_resgld(potential_fn, data_loader, batch_size, init_samples, itera
   return out
 
 
+def _amagold_fused_eligible(stochastic_potential_fn, full_potential_fn, data_loader,
+                            init_samples, mass, adaptive, fused):
+  """The fused AMAGOLD engine applies when the stochastic and the full potential are
+  the same GLM family, prior and temperature over a DeviceNumpyDataLoader, the mass is
+  the identity and the step-size schedule is fixed in advance."""
+  if fused is False:
+    return False
+  sp, fp = stochastic_potential_fn, full_potential_fn
+  ok = (not adaptive and mass is None and isinstance(fp, PT.FullPotential) and fp.fast
+        and _fused_eligible(sp, data_loader, init_samples, None)
+        and type(fp.likelihood) is type(sp.likelihood)
+        and getattr(fp.likelihood, "sigma", None) == getattr(sp.likelihood, "sigma", None)
+        and fp.likelihood.x == sp.likelihood.x and fp.likelihood.y == sp.likelihood.y
+        and PT._prior_spec(fp.prior) == PT._prior_spec(sp.prior)
+        and fp.temperature == sp.temperature)
+  if fused and not ok:
+    raise ValueError("fused=True needs matching GLM stochastic and full potentials over a "
+                     "DeviceNumpyDataLoader, identity mass and a fixed step-size schedule")
+  return ok
+
+
+def _run_fused_amagold(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                       steps, friction, keys, data_keys, to_numpy=True):
+  """alias.amagold on the fused engine: every chain's reversible-leapfrog iteration
+  runs in ``steps`` fused launches, then one pass over the data gives every proposal's
+  full potential and the MH kernel accepts or rejects each chain (solver.py:301-432)."""
+  init_fn = sched[0]
+  st, _ = init_fn(iterations)
+  eps, _, keep = init_fn.tables(st)
+  slots = np.full(iterations, -1, np.int32)
+  kept = np.nonzero(keep)[0]
+  slots[kept] = np.arange(kept.size, dtype=np.int32)
+  lik = potential_fn.likelihood
+  X = data_loader.reference_data[lik.x]
+  y = data_loader.reference_data[lik.y].to(torch.float32).contiguous()
+  has_prior, pscale = PT._prior_spec(potential_fn.prior)
+  chains = []
+  for c, s in enumerate(init_samples):
+    leaf = T.flatten(s)[0][0]
+    th0 = leaf.detach().cpu().numpy() if isinstance(leaf, torch.Tensor) else np.asarray(leaf)
+    chains.append(dict(X=X, y=y, D=X.shape[1], n=batch_size,
+                       theta0=np.asarray(th0, np.float32).reshape(-1),
+                       dkey=_key_words(data_keys[c] if data_keys else None)))
+  from . import fused as FU
+  eng = FU.FusedGLM(chains, lik.family, "amagold", eps, steps=steps, friction=friction,
+                    sigma=lik.sigma, prior_scale=pscale, has_prior=bool(has_prior),
+                    pot_temperature=potential_fn.temperature, save_slots=slots,
+                    n_slots=max(1, kept.size))
+  mh = FU.AmagoldMH(eng, keys=[_key_words(keys[c]) for c in range(len(chains))]
+                    if keys else None)
+  eng.run(iterations)
+  torch.cuda.synchronize()
+  out = []
+  for c, s in enumerate(init_samples):
+    _, td = T.flatten(s)
+    samples = {"variables": T.unflatten(td, [eng.sample_trace(c)[:kept.size].clone()]),
+               "energy": eng.sample_potentials(c)[:kept.size].clone(),
+               "acceptance_ratio": mh.acceptance_ratios(c)[:kept.size].clone(),
+               "step_size": torch.as_tensor(eps[kept]), "model_state": None}
+    if to_numpy:
+      samples = T.tree_map(lambda x: x.cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                           samples)
+    out.append({"sample_count": int(kept.size), "samples": samples})
+  return out
+
+
 _MLP_LEAVES = ("W1", "W2", "W3", "b1", "b2", "b3")
 
 
@@ -466,7 +532,16 @@ def amagold(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=
   saving = IO.save(to_numpy=True) if save_to_numpy else None
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
-  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None):
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, data_keys=None,
+             fused=None):
+    if init_model_state is None and _amagold_fused_eligible(
+        stochastic_potential_fn, full_potential_fn, data_loader, init_samples, mass,
+        adaptive_step_size, fused):
+      return _run_fused_amagold(stochastic_potential_fn, data_loader, batch_size, init_samples,
+                                iterations, sched, integration_steps, float(friction), keys,
+                                data_keys, to_numpy=save_to_numpy)
+    if data_keys:
+      raise ValueError("data_keys is supported by the fused path only")
     states = []
     for c, s in enumerate(init_samples):
       kw = dict(init_model_state=init_model_state)

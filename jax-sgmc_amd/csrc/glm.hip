@@ -20,6 +20,7 @@
 // block to arrive (deterministic order).
 #include <math.h>
 
+#include "glm_common.h"
 #include "sgmc_common.h"
 #include "sgmc_device.h"
 
@@ -29,34 +30,6 @@ constexpr int kNT = 256;
 constexpr int kWaves = kNT / kWave;
 constexpr int kMaxD = 1024;  // fused path: features per row
 constexpr int kLdsRows = 4096;  // per-row likelihoods staged in LDS per block
-
-struct Lik {
-  int family;
-  float inv_s2;    // 1 / sigma^2 (linear-Gaussian)
-  float log_norm;  // log(2 pi sigma^2)
-};
-
-struct Prior {
-  int has_prior;
-  float inv_var;   // 1 / s_p^2
-  float log_norm;  // log(2 pi s_p^2)
-};
-
-__device__ __forceinline__ void row_lik(const Lik& L, float z, float y, float& lik, float& r) {
-  if (L.family == SGMC_LIK_LINEAR_GAUSSIAN) {
-    const float d = y - z;
-    lik = (L.log_norm + (d * d) * L.inv_s2) / -2.0f;
-    r = d * L.inv_s2;
-  } else {
-    const float az = fabsf(z);
-    const float e = expf(-az);
-    const float sp = fmaxf(z, 0.0f) + log1pf(e);
-    lik = y * z - sp;
-    const float inv1pe = 1.0f / (1.0f + e);
-    const float sig = z >= 0.0f ? inv1pe : e * inv1pe;
-    r = y - sig;
-  }
-}
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -568,6 +541,7 @@ struct FusedArgs {
   int step;                // leapfrog step within the iteration (SGHMC)
   int steps;
   float friction;
+  float four_friction;     // float32(4 friction) (AMAGOLD noise scale)
   float alpha, lmbd;       // rms_prop
   Lik L;
   Prior pr;
@@ -598,7 +572,13 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   gcfloat* g_sqrt_mass = to_global(a.sqrt_mass);
   const Key key0 = load_key(ch.key);
   const bool hmc = a.mode == SGMC_MODE_SGHMC;
+  const bool amg = a.mode == SGMC_MODE_AMAGOLD;
+  const bool has_mom = hmc || amg;
   const bool resample = hmc && a.step == 0;
+  // Position step before this evaluation: eps, or eps/2 at the first step of an
+  // AMAGOLD iteration (integrator.py:521-523, 401-408).
+  const float half = 0.5f * eps;
+  const float pos_scale = (amg && a.step == 0) ? half : eps;
 
   // Momentum used by this step: resampled p = sqrt_mass * xi at the first step.
   Key key_noise = key0;  // key split for this step's noise
@@ -610,7 +590,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     key_noise = k1;
   }
 
-  // Evaluation point: theta (SGLD) or theta + eps M^-1 p (SGHMC).
+  // Evaluation point: theta (SGLD) or theta + scale M^-1 p (SGHMC, AMAGOLD).
   float4 th[CPL];
   {
     const int q = (threadIdx.x & (kWave - 1)) % LPR;
@@ -623,7 +603,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         float v = 0.0f;
         if (c < D) {
           v = g_theta[c];
-          if (hmc) {
+          if (has_mom) {
             float p;
             if (resample) {
               const float xi = leaf_normal(resample_leaf, (uint32_t)D, (uint32_t)c);
@@ -632,7 +612,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
               p = g_mom[c];
             }
             const float s = a.inv_mass ? g_inv_mass[c] * p : p;
-            v = v + eps * s;
+            v = v + pos_scale * s;
           }
         }
         e[t] = v;
@@ -712,10 +692,19 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   Key knext, ksub;
   split2(key_noise, knext, ksub);
   const Key leaf = split_key(ksub, 1, 0);
-  const int slot = a.save_slot ? a.save_slot[it] : -1;
-  const bool last_step = !hmc || a.step == a.steps - 1;
+  // AMAGOLD samples are saved after the Metropolis-Hastings test, not here.
+  const int slot = (a.save_slot && !amg) ? a.save_slot[it] : -1;
+  const bool last_step = !has_mom || a.step == a.steps - 1;
   const float Tn = ch.temperature;
   const bool rms = a.mode == SGMC_MODE_SGLD_RMS;
+  // reversible_leapfrog momentum coefficients (integrator.py:420-437), float32.
+  const float ef = eps * a.friction;
+  const float am_decay = 1.0f - ef;
+  const float am_rescale = 1.0f / (1.0f + ef);
+  const float am_noise = sqrtf(a.four_friction * eps);
+  gfloat* g_theta_old = to_global(ch.theta_old);
+  gfloat* g_mom_old = to_global(ch.mom_old);
+  float edot = 0.0f;
 
   float lp = 0.0f;
   const uint32_t S = (uint32_t)D;
@@ -734,9 +723,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     for (int h = 0; h < 2; ++h) {
       const uint32_t j = jj[h];
       wv[h] = g_theta[j];
-      pv[h] = (hmc && !resample) ? g_mom[j] : 0.0f;
-      imv[h] = (hmc && a.inv_mass) ? g_inv_mass[j] : 1.0f;
-      smv[h] = (resample && a.sqrt_mass) ? g_sqrt_mass[j] : 1.0f;
+      pv[h] = (has_mom && !resample) ? g_mom[j] : 0.0f;
+      imv[h] = (has_mom && a.inv_mass) ? g_inv_mass[j] : 1.0f;
+      smv[h] = ((resample || amg) && a.sqrt_mass) ? g_sqrt_mass[j] : 1.0f;
       vv[h] = rms ? g_v[j] : 0.0f;
     }
 #pragma unroll
@@ -746,7 +735,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       const float xi = bits_to_normal(h ? w1 : w0);
       float wj = wv[h];
       float p = 0.0f, sm = 0.0f;
-      if (hmc) {
+      if (has_mom) {
         if (resample) {
           const float xr = leaf_normal(resample_leaf, S, j);
           p = a.sqrt_mass ? smv[h] * xr : xr;
@@ -754,7 +743,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
           p = pv[h];
         }
         sm = a.inv_mass ? imv[h] * p : p;
-        wj = wj + eps * sm;  // evaluation point
+        wj = wj + pos_scale * sm;  // evaluation point
       }
       float g = c * s[h];
       if (a.pr.has_prior) {
@@ -763,7 +752,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       }
       g = g / a.pot_temperature;
       float wn;
-      if (!hmc) {
+      if (!has_mom) {
         const float sg = (-eps) * g;
         const float sn = sqrtf((2.0f * Tn) * eps) * xi;
         if (rms) {
@@ -774,20 +763,34 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         } else {
           wn = wj + (sg + sn);
         }
-      } else {
+      } else if (hmc) {
         const float C = a.friction;
         float pn = p + ((-eps) * C) * sm;
         pn = pn + (-eps) * g;
         pn = pn + C * (sqrtf(2.0f * eps) * xi);
         g_mom[j] = pn;
         wn = wj;
+      } else {
+        // p' = (((1 - eps f) p + (-eps) g) + sqrt(4 f eps) (M^{1/2} xi)) / (1 + eps f);
+        // energy term (p + p') . (M^-1 g); final half position step after the last.
+        const float noise = am_noise * (smv[h] * xi);
+        const float pn = am_rescale * ((am_decay * p + (-eps) * g) + noise);
+        edot += (p + pn) * (imv[h] * g);
+        if (a.step == 0) {
+          g_theta_old[j] = wv[h];
+          g_mom_old[j] = p;
+        }
+        g_mom[j] = pn;
+        wn = last_step ? wj + half * (imv[h] * pn) : wj;
       }
       g_theta[j] = wn;
       if (slot >= 0 && last_step && ch.samples) g_samples[(int64_t)slot * D + j] = wn;
     }
   }
   lp = block_sum<kNT>(lp, lds_small);
+  if (amg) edot = block_sum<kNT>(edot, lds_small);
   if (threadIdx.x == 0) {
+    if (amg) ch.energy[0] = half * edot + (a.step == 0 ? 0.0f : ch.energy[0]);
     const float Uc = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
     ch.U[0] = Uc;
     if (slot >= 0 && last_step && ch.samples_U) ch.samples_U[slot] = Uc;
@@ -830,24 +833,6 @@ static int dclass_of(int D) {
   if (D <= 256) return 2;
   if (D <= 512) return 3;
   return 4;
-}
-
-static Lik make_lik(int family, float sigma) {
-  Lik L;
-  L.family = family;
-  const float s2 = sigma * sigma;
-  L.inv_s2 = 1.0f / s2;
-  L.log_norm = logf(2.0f * (float)M_PI * s2);
-  return L;
-}
-
-static Prior make_prior(int has_prior, float scale) {
-  Prior p;
-  p.has_prior = has_prior;
-  const float s2 = scale * scale;
-  p.inv_var = has_prior ? 1.0f / s2 : 0.0f;
-  p.log_norm = has_prior ? logf(2.0f * (float)M_PI * s2) : 0.0f;
-  return p;
 }
 
 }  // namespace sgmc
@@ -986,7 +971,8 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                "sgmc_glm_fused_step: bad sizes");
   SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
                "sgmc_glm_fused_step: unknown family %d", family);
-  SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS || mode == SGMC_MODE_SGHMC,
+  SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS ||
+               mode == SGMC_MODE_SGHMC || mode == SGMC_MODE_AMAGOLD,
                SGMC_E_ARG, "sgmc_glm_fused_step: unknown mode %d", mode);
   SGMC_REQUIRE(steps >= 1 && step >= 0 && step < steps, SGMC_E_ARG,
                "sgmc_glm_fused_step: step %d of %d", step, steps);
@@ -1006,6 +992,7 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   a.step = step;
   a.steps = steps;
   a.friction = friction;
+  a.four_friction = (float)(4.0 * (double)friction);
   a.alpha = alpha;
   a.lmbd = lmbd;
   a.L = make_lik(family, sigma);

@@ -21,7 +21,9 @@ import torch
 
 from . import _lib
 
-MODES = {"sgld": _lib.MODE_SGLD, "sgld_rms": _lib.MODE_SGLD_RMS, "sghmc": _lib.MODE_SGHMC}
+MODES = {"sgld": _lib.MODE_SGLD, "sgld_rms": _lib.MODE_SGLD_RMS, "sghmc": _lib.MODE_SGHMC,
+         "amagold": _lib.MODE_AMAGOLD}
+_STEPPED = (_lib.MODE_SGHMC, _lib.MODE_AMAGOLD)
 FAMILIES = {"linear_gaussian": _lib.LIK_LINEAR_GAUSSIAN, "logistic": _lib.LIK_LOGISTIC}
 NT = 256
 
@@ -57,7 +59,7 @@ class FusedGLM:
     self.device = torch.device(device)
     self.family = FAMILIES[family] if isinstance(family, str) else int(family)
     self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
-    self.steps = int(steps) if self.mode == _lib.MODE_SGHMC else 1
+    self.steps = int(steps) if self.mode in _STEPPED else 1
     self.friction = float(friction)
     self.alpha, self.lmbd = float(alpha), float(lmbd)
     self.sigma, self.prior_scale = float(sigma), float(prior_scale)
@@ -103,6 +105,7 @@ class FusedGLM:
     self.target_bytes = target_bytes or max(64 * 1024, total_bytes // 2048)
     self.keep_lik = bool(keep_lik)
     self.exchange = None
+    self.mh = None
     self._allocate()
     self._build_table()
 
@@ -116,8 +119,12 @@ class FusedGLM:
     self.poff = np.concatenate([[0], np.cumsum(Da)]).astype(np.int64)
     P = int(self.poff[-1])
     self.theta_flat = torch.zeros(P, device=dev)
-    self.mom_flat = torch.zeros(P, device=dev) if self.mode == _lib.MODE_SGHMC else None
+    self.mom_flat = torch.zeros(P, device=dev) if self.mode in _STEPPED else None
     self.v_flat = torch.ones(P, device=dev) if self.mode == _lib.MODE_SGLD_RMS else None
+    amg = self.mode == _lib.MODE_AMAGOLD
+    self.energy = torch.zeros(self.C, device=dev) if amg else None
+    self.theta_old_flat = torch.zeros(P, device=dev) if amg else None
+    self.mom_old_flat = torch.zeros(P, device=dev) if amg else None
     for i, c in enumerate(ch):
       self.theta_flat[self.poff[i]:self.poff[i] + self.D[i]] = torch.as_tensor(
           np.asarray(c["theta0"], np.float32).reshape(-1), device=dev)
@@ -181,6 +188,10 @@ class FusedGLM:
       e.nblocks = int(self.nblocks_chain[i])
       e.dclass = _lib.query("sgmc_glm_dclass", int(self.D[i]))
       e.temperature = float(c.get("temperature", 1.0))
+      if self.energy is not None:
+        e.energy = self.energy.data_ptr() + fs * i
+        e.theta_old = self.theta_old_flat.data_ptr() + fs * int(self.poff[i])
+        e.mom_old = self.mom_old_flat.data_ptr() + fs * int(self.poff[i])
     raw = bytes(arr)
     self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
     bc = np.repeat(order.astype(np.int32), self.nblocks_chain[order])
@@ -208,6 +219,8 @@ class FusedGLM:
     self._launch_steps()
     if self.exchange is not None:
       self.exchange.launch()
+    if self.mh is not None:
+      self.mh.launch()
 
   def capture(self, iterations_per_graph=1):
     """Captures ``iterations_per_graph`` iterations into a HIP graph."""
@@ -293,7 +306,8 @@ class FusedGLM:
 
   def _snapshot(self):
     t = {k: getattr(self, k).clone() for k in ("theta_flat", "keys", "dkeys", "iter", "U", "var")}
-    for k in ("mom_flat", "v_flat", "samples", "samples_U"):
+    for k in ("mom_flat", "v_flat", "samples", "samples_U", "energy", "theta_old_flat",
+              "mom_old_flat"):
       v = getattr(self, k)
       t[k] = v.clone() if v is not None else None
     return t
@@ -392,3 +406,84 @@ class ReplicaExchange:
   def sample_potentials(self, p):
     k = self.save_index[p]
     return self.samples_U[k * self.n_slots:(k + 1) * self.n_slots]
+
+
+class AmagoldMH:
+  """AMAGOLD's amortized Metropolis-Hastings test (solver.py:301-432) for a FusedGLM in
+  mode "amagold" whose chains share one dataset: after every iteration the full-data
+  potential of all proposals runs as one pass over X (``sgmc_glm_full_potential_chains``,
+  FP64 sums) and ``sgmc_glm_amagold_mh`` accepts or rejects each chain.
+
+  keys: one (solver key, integrator key) pair per chain as the reference's init derives
+  them (solver.py:340-352): the integrator key draws the initial momentum
+  sqrt_mass * xi (integrator.py:488-492) and the MH key starts the test stream.
+  """
+
+  def __init__(self, engine, keys=None, nblocks=None):
+    if engine.mode != _lib.MODE_AMAGOLD:
+      raise ValueError("AmagoldMH needs a FusedGLM in mode 'amagold'")
+    spec = engine.spec
+    c0 = spec[0]
+    for c in spec:
+      if (c["X"].data_ptr() != c0["X"].data_ptr() or c["y"].data_ptr() != c0["y"].data_ptr()
+          or int(c["D"]) != int(c0["D"]) or int(c.get("N", c["X"].shape[0])) !=
+          int(c0.get("N", c0["X"].shape[0]))):
+        raise ValueError("AmagoldMH needs chains that share one dataset")
+    from . import random as R
+    dev = engine.device
+    self.engine = engine
+    self.X, self.y = c0["X"], c0["y"]
+    self.N = int(c0.get("N", c0["X"].shape[0]))
+    self.D = int(c0["D"])
+    self.ld_theta = int(engine.poff[1] - engine.poff[0])
+    C = engine.C
+    self.nblocks = int(nblocks or torch.cuda.get_device_properties(dev).multi_processor_count)
+    self.ws = torch.zeros(int(_lib.query("sgmc_glm_full_chains_workspace_doubles",
+                                         self.nblocks)), dtype=torch.float64, device=dev)
+    self.U_full = torch.zeros(C, dtype=torch.float64, device=dev)
+    self.U_prop = torch.zeros(C, dtype=torch.float64, device=dev)
+    self.energy_kept = torch.zeros(C, device=dev)
+    self.n_slots = engine.n_slots
+    self.ratio_samples = torch.zeros(max(1, C * self.n_slots), device=dev)
+    # init (solver.py:340-352, integrator.py:484-492): (k_int, k_mh) = split(key);
+    # (k_int', sub) = split(k_int); p = sqrt_mass * random_tree(sub, theta0).
+    mh_words = np.zeros((C, 2), np.uint32)
+    int_words = np.zeros((C, 2), np.uint32)
+    lay_table = None
+    for c in range(C):
+      kw = (0, 0) if keys is None else tuple(int(v) for v in keys[c])
+      k = R.split(R.key_from_words(kw, device=dev))
+      mh_words[c] = R.key_words(k[1])
+      k_out = torch.empty(2, dtype=torch.int32, device=dev)
+      if lay_table is None:
+        from . import tree as T
+        lay_table = T.Layout([torch.zeros(self.D)]).leaf_table(dev, _lib.PAIRS_PER_BLOCK)
+      table, nb = lay_table
+      mom = engine.momentum(c)
+      _lib.call("sgmc_sghmc_resample", _lib.ptr(mom), _lib.ptr(engine.sqrt_mass),
+                _lib.ptr(table), nb, _lib.ptr(k[0].clone()), _lib.ptr(k_out), _lib.stream())
+      int_words[c] = R.key_words(k_out)
+    engine.keys.copy_(torch.as_tensor(int_words.view(np.int32).copy(), device=dev))
+    self.keys = torch.as_tensor(mh_words.view(np.int32).copy(), device=dev)
+    self._full(self.U_full)
+    engine.mh = self
+
+  def _full(self, out):
+    eng = self.engine
+    _lib.call("sgmc_glm_full_potential_chains", eng.family, _lib.ptr(self.X),
+              int(self.X.stride(0)), _lib.ptr(self.y), self.N, self.D, _lib.ptr(eng.theta_flat),
+              self.ld_theta, eng.C, eng.sigma, eng.has_prior, eng.prior_scale,
+              eng.pot_temperature, _lib.ptr(out), _lib.ptr(self.ws), self.ws.numel(),
+              self.nblocks, _lib.stream())
+
+  def launch(self):
+    eng = self.engine
+    self._full(self.U_prop)
+    _lib.call("sgmc_glm_amagold_mh", _lib.ptr(eng.table), eng.C, _lib.ptr(self.U_prop),
+              _lib.ptr(self.U_full), _lib.ptr(self.energy_kept), _lib.ptr(self.keys),
+              _lib.ptr(self.ratio_samples), self.n_slots, _lib.ptr(eng.save_slot),
+              eng.table_len, _lib.ptr(eng.iter), _lib.stream())
+
+  def acceptance_ratios(self, i):
+    """[n_slots] exp(log a) of the test at each saved iteration of chain i."""
+    return self.ratio_samples[i * self.n_slots:(i + 1) * self.n_slots]

@@ -84,8 +84,9 @@ def logreg(N=3000, D=24, seed=1):
   return X, y
 
 
-def test_amagold_matches_oracle():
-  print("amagold alias parity", flush=True)
+@pytest.mark.parametrize("fused", [False, True])
+def test_amagold_matches_oracle(fused):
+  print("amagold alias parity fused", fused, flush=True)
   X, y = logreg()
   loader = DT.DeviceNumpyDataLoader(x=X, y=y)
   pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
@@ -94,7 +95,7 @@ def test_amagold_matches_oracle():
   run = alias.amagold(pot, full, loader, cache_size=1, batch_size=128,
                       integration_steps=steps, friction=fr, first_step_size=0.02,
                       last_step_size=0.005, accepted_samples=its)
-  res = run(torch.zeros(24), iterations=its)[0]["samples"]
+  res = run(torch.zeros(24), iterations=its, fused=fused)[0]["samples"]
   eps = SC.polynomial_table(0.02, 0.005, its)
   fam = om.Family("logistic", X, y)
   r = ost.amagold(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
@@ -149,3 +150,58 @@ def test_adaptive_step_size_freezes_after_burn_in(solver):
   steps = np.asarray(res["step_size"], np.float32)
   assert len(steps) == 10
   assert np.all(steps == steps[0]) and steps[0] != np.float32(0.05)
+
+
+def test_fused_amagold_chains_match_generic_path():
+  """Four chains with their own keys in one fused table equal the same chains run one
+  by one through the generic solver (decisions, positions, energies)."""
+  print("fused amagold vs generic, 4 chains", flush=True)
+  X, y = logreg(N=2000, D=40, seed=7)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  full = PT.full_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  run = alias.amagold(pot, full, loader, cache_size=1, batch_size=100, integration_steps=3,
+                      friction=0.3, first_step_size=0.03, last_step_size=0.01,
+                      accepted_samples=12)
+  keys = [jr.PRNGKey(s) for s in (1, 2, 3, 4)]
+  inits = [0.1 * np.random.default_rng(s).standard_normal(40).astype(np.float32)
+           for s in range(4)]
+  a = run(*[torch.as_tensor(v) for v in inits], iterations=12, keys=keys, fused=True)
+  b = run(*[torch.as_tensor(v) for v in inits], iterations=12,
+          keys=[R.key_from_words(k, device=DEV) for k in keys], fused=False)
+  for c in range(4):
+    sa, sb = a[c]["samples"], b[c]["samples"]
+    np.testing.assert_allclose(sa["acceptance_ratio"], np.asarray(sb["acceptance_ratio"]),
+                               rtol=1e-3, atol=1e-4)
+    ga, gb = np.asarray(sa["variables"]), np.asarray(sb["variables"])
+    assert np.max(np.abs(ga - gb)) <= 1e-5 * max(1.0, np.max(np.abs(gb))), c
+    np.testing.assert_allclose(sa["energy"], np.asarray(sb["energy"]), rtol=1e-3, atol=1e-3)
+
+
+def test_fused_amagold_graph_replay_matches_eager():
+  print("fused amagold graph replay", flush=True)
+  from jax_sgmc_amd import fused as FU
+  X, y = logreg(N=3000, D=64, seed=8)
+  Xd = torch.as_tensor(X, device=DEV)
+  yd = torch.as_tensor(y, device=DEV)
+  its = 8
+  eps = np.full(its, 0.02, np.float32)
+
+  def make():
+    chains = [dict(X=Xd, y=yd, D=64, n=256, theta0=np.zeros(64, np.float32), dkey=(0, c))
+              for c in range(3)]
+    eng = FU.FusedGLM(chains, "logistic", "amagold", eps, steps=4, friction=0.25,
+                      save_slots=np.arange(its, dtype=np.int32), n_slots=its)
+    FU.AmagoldMH(eng, keys=[(0, 10 + c) for c in range(3)])
+    return eng
+
+  e1, e2 = make(), make()
+  e1.run(its)
+  e2.run(its, graph_iterations=4)
+  torch.cuda.synchronize()
+  assert e1.iteration_count() == e2.iteration_count() == its
+  for c in range(3):
+    np.testing.assert_array_equal(e1.sample_trace(c).cpu().numpy(),
+                                  e2.sample_trace(c).cpu().numpy())
+    np.testing.assert_array_equal(e1.mh.acceptance_ratios(c).cpu().numpy(),
+                                  e2.mh.acceptance_ratios(c).cpu().numpy())

@@ -227,6 +227,37 @@ def test_glm_full_potential_f64(family, N, D):
   assert abs(out.item() - ref) <= 1e-6 * (abs(ref) + N), (out.item(), ref)
 
 
+@pytest.mark.parametrize("family", [L.LIK_LINEAR_GAUSSIAN, L.LIK_LOGISTIC])
+@pytest.mark.parametrize("N,D,C", [(1, 3, 1), (3001, 37, 33), (2500, 512, 64), (777, 1000, 70),
+                                   (4097, 8, 2)])
+def test_glm_full_potential_chains(family, N, D, C):
+  """Many chains in one pass over X: every chain's full potential against the FP64
+  oracle, and bit-equal to a second call (fixed reduction order)."""
+  print("glm_full_chains", family, N, D, C, flush=True)
+  X, y = _glm_data(N, D, family, seed=5)
+  thetas = (0.2 * np.random.default_rng(6).standard_normal((C, D))).astype(np.float32)
+  Xd, ldx = _padded(X)
+  yd = torch.as_tensor(y, device=DEV)
+  ld_theta = D + 3
+  th = torch.zeros((C, ld_theta), device=DEV)
+  th[:, :D] = torch.as_tensor(thetas, device=DEV)
+  nblocks = 96
+  ws = torch.zeros(int(L.query("sgmc_glm_full_chains_workspace_doubles", nblocks)),
+                   dtype=torch.float64, device=DEV)
+  outs = []
+  for _ in range(2):
+    out = torch.empty(C, dtype=torch.float64, device=DEV)
+    L.call("sgmc_glm_full_potential_chains", family, L.ptr(Xd), ldx, L.ptr(yd), N, D, L.ptr(th),
+           ld_theta, C, 0.5, 1, 1.0, 2.0, L.ptr(out), L.ptr(ws), ws.numel(), nblocks, L.stream())
+    outs.append(out.cpu().numpy())
+  np.testing.assert_array_equal(outs[0], outs[1])
+  kind = "linear_gaussian" if family == L.LIK_LINEAR_GAUSSIAN else "logistic"
+  fam = om.Family(kind, X, y, sigma=0.5)
+  for c in range(C):
+    ref = om.full_potential(fam, thetas[c], 1.0, mb=N, temperature=2.0)
+    assert abs(outs[0][c] - ref) <= 1e-6 * (abs(ref) + N), (c, outs[0][c], ref)
+
+
 def _layout(sizes):
   return T.Layout([torch.zeros(s) for s in sizes])
 
