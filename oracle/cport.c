@@ -7,6 +7,8 @@
  *     integrator.py:119-135, 630, 736 and data/numpy_loader.py:132-134;
  *   friction_leapfrog.integrate / _body_fun (integrator.py:715-757, 603-666);
  *   langevin_diffusion.update_fn (integrator.py:859-922);
+ *   reversible_leapfrog.integrate (integrator.py:503-553) with solver.amagold's
+ *     full-data Metropolis-Hastings test (solver.py:366-419);
  *   minibatch potential and gradient (potential.py:159-214) for the logistic and
  *   linear-Gaussian families with a Gaussian prior.
  * Chains run in parallel under OpenMP; within a chain the arithmetic is float32 with
@@ -289,6 +291,122 @@ int cport_sgld(int family, const float* X, int64_t ldx, const float* y, int64_t 
     free(idx);
   }
   return 0;
+}
+
+/* Full-data potential (potential.py:219-293): float32 rows, float64 sum and prior. */
+static double full_potential_c(int family, const float* X, int64_t ldx, const float* y,
+                               int64_t N, int D, const float* w, float sigma,
+                               float prior_scale) {
+  const float inv_s2 = 1.0f / (sigma * sigma);
+  const float ln = logf(2.0f * 3.14159265358979f * sigma * sigma);
+  double s = 0.0;
+  for (int64_t i = 0; i < N; ++i) {
+    const float* x = X + i * ldx;
+    float z = 0.0f;
+    for (int j = 0; j < D; ++j) z += x[j] * w[j];
+    float lik;
+    if (family == 0) {
+      float d = y[i] - z;
+      lik = (ln + d * d * inv_s2) / -2.0f;
+    } else {
+      lik = y[i] * z - (fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z))));
+    }
+    s += lik;
+  }
+  const double iv = 1.0 / ((double)prior_scale * prior_scale);
+  const double pln = log(2.0 * 3.14159265358979 * (double)prior_scale * prior_scale);
+  double lp = 0.0;
+  for (int j = 0; j < D; ++j) lp += (pln + (double)w[j] * w[j] * iv) / -2.0;
+  return -s - lp;
+}
+
+/* AMAGOLD (unit mass): per iteration split the MH key, run reversible_leapfrog from
+ * (theta, p), evaluate the full potential of the proposal and accept iff
+ * log u < min(0, U_full - U' + energy); a rejection keeps theta and flips the
+ * momentum. theta/mom [C][D], keys/mh_keys/dkeys [C][2] and U_full [C] advance in place;
+ * accepts [C][iters] (nullable) records the decisions. */
+int cport_amagold(int family, const float* X, int64_t ldx, const float* y, int64_t N, int D,
+                  int n, int C, float* theta, float* mom, uint32_t* keys, uint32_t* mh_keys,
+                  uint32_t* dkeys, double* U_full, const float* eps, int iters, int steps, float friction,
+                  float sigma, float prior_scale, int8_t* accepts, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int c = 0; c < C; ++c) {
+    float* w = theta + (int64_t)c * D;
+    float* p = mom + (int64_t)c * D;
+    float* th = (float*)malloc(sizeof(float) * D);
+    float* p_old = (float*)malloc(sizeof(float) * D);
+    float* g = (float*)malloc(sizeof(float) * D);
+    float* xi = (float*)malloc(sizeof(float) * D);
+    double* acc = (double*)malloc(sizeof(double) * D);
+    int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * n);
+    key_t2 key = {keys[2 * c], keys[2 * c + 1]}, mkey = {mh_keys[2 * c], mh_keys[2 * c + 1]};
+    key_t2 dkey = {dkeys[2 * c], dkeys[2 * c + 1]};
+    for (int t = 0; t < iters; ++t) {
+      key_t2 nk, sub, mnext, msub;
+      split2(mkey, &mnext, &msub);
+      mkey = mnext;
+      const float e = eps[t];
+      const float half = 0.5f * e;
+      const float ef = e * friction;
+      const float decay = 1.0f - ef;
+      const float rescale = 1.0f / (1.0f + ef);
+      const float ns = sqrtf((float)(4.0 * (double)friction) * e);
+      memcpy(p_old, p, sizeof(float) * D);
+      for (int j = 0; j < D; ++j) th[j] = w[j] + half * p[j];
+      float energy = 0.0f;
+      for (int s = 0; s < steps; ++s) {
+        if (s > 0)
+          for (int j = 0; j < D; ++j) th[j] = th[j] + e * p[j];
+        split2(key, &nk, &sub);
+        key = nk;
+        leaf_normals(sub, (uint32_t)D, xi);
+        draw_indices(&dkey, (uint32_t)n, (uint32_t)N, idx);
+        potential(family, X, ldx, y, N, D, n, idx, th, sigma, prior_scale, g, acc);
+        double dot = 0.0;
+        for (int j = 0; j < D; ++j) {
+          const float pn = rescale * ((decay * p[j] + (-e) * g[j]) + ns * xi[j]);
+          dot += (double)((p[j] + pn) * g[j]);
+          p[j] = pn;
+        }
+        energy = half * (float)dot + energy;
+      }
+      for (int j = 0; j < D; ++j) th[j] = th[j] + half * p[j];
+      const double U_new = full_potential_c(family, X, ldx, y, N, D, th, sigma, prior_scale);
+      double log_alpha = U_full[c] - U_new + (double)energy;
+      if (log_alpha > 0.0) log_alpha = 0.0;
+      uint32_t fb = (bits_word(msub, 1u, 0u) >> 9) | 0x3F800000u;
+      float u;
+      memcpy(&u, &fb, 4);
+      u = u - 1.0f;
+      if (u < 0.0f) u = 0.0f;
+      const int ok = log((double)u) < log_alpha;
+      if (ok) {
+        memcpy(w, th, sizeof(float) * D);
+        U_full[c] = U_new;
+      } else {
+        for (int j = 0; j < D; ++j) p[j] = -1.0f * p_old[j];
+      }
+      if (accepts) accepts[(int64_t)c * iters + t] = (int8_t)ok;
+    }
+    keys[2 * c] = key.k0;
+    keys[2 * c + 1] = key.k1;
+    mh_keys[2 * c] = mkey.k0;
+    mh_keys[2 * c + 1] = mkey.k1;
+    dkeys[2 * c] = dkey.k0;
+    dkeys[2 * c + 1] = dkey.k1;
+    free(th);
+    free(p_old);
+    free(g);
+    free(xi);
+    free(acc);
+    free(idx);
+  }
+  return 0;
+}
+
+double cport_full_potential(int family, const float* X, int64_t ldx, const float* y, int64_t N,
+                            int D, const float* w, float sigma, float prior_scale) {
+  return full_potential_c(family, X, ldx, y, N, D, w, sigma, prior_scale);
 }
 
 /* Raw jax.random words and normals for cross-checks. */

@@ -59,6 +59,47 @@ def sgld(family, X, y, n, theta, keys, dkeys, eps, temperature=1.0, sigma=1.0,
   return U
 
 
+def amagold_init(theta0, key):
+  """solver.amagold init (solver.py:340-352) for one chain with unit mass: returns the
+  initial momentum, the integrator key and the MH key (numpy oracle RNG)."""
+  from oracle import jaxrandom as jr
+  from oracle import steps as ost
+  k_int, k_mh = jr.split(np.asarray(key, np.uint32))
+  p, k_int = ost.reversible_leapfrog_init(np.asarray(theta0), k_int, [np.size(theta0)])
+  return p.astype(np.float32), k_int, k_mh
+
+
+def amagold(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, steps, friction,
+            sigma=1.0, prior_scale=1.0, threads=1):
+  """Runs len(eps) AMAGOLD iterations for theta.shape[0] chains in place; returns the
+  accept decisions [C, iterations]."""
+  lib = load()
+  X = np.ascontiguousarray(X, np.float32)
+  y = np.ascontiguousarray(y, np.float32)
+  eps = np.ascontiguousarray(eps, np.float32)
+  C, D = theta.shape
+  acc = np.zeros((C, eps.size), np.int8)
+  lib.cport_amagold(ctypes.c_int(family), _p(X), ctypes.c_int64(X.shape[1]), _p(y),
+                    ctypes.c_int64(X.shape[0]), ctypes.c_int(D), ctypes.c_int(n),
+                    ctypes.c_int(C), _p(theta), _p(mom), _p(keys), _p(mh_keys), _p(dkeys),
+                    _p(U_full), _p(eps), ctypes.c_int(eps.size), ctypes.c_int(steps),
+                    ctypes.c_float(friction), ctypes.c_float(sigma),
+                    ctypes.c_float(prior_scale), _p(acc), ctypes.c_int(threads))
+  return acc.astype(bool)
+
+
+def full_potential(family, X, y, theta, sigma=1.0, prior_scale=1.0):
+  """FP64 full-data potential of one position (the C port's MH reference value)."""
+  lib = load()
+  lib.cport_full_potential.restype = ctypes.c_double
+  X = np.ascontiguousarray(X, np.float32)
+  y = np.ascontiguousarray(y, np.float32)
+  w = np.ascontiguousarray(theta, np.float32)
+  return lib.cport_full_potential(ctypes.c_int(family), _p(X), ctypes.c_int64(X.shape[1]),
+                                  _p(y), ctypes.c_int64(X.shape[0]), ctypes.c_int(X.shape[1]),
+                                  _p(w), ctypes.c_float(sigma), ctypes.c_float(prior_scale))
+
+
 def normal(key, S):
   lib = load()
   out = np.zeros(S, np.float32)

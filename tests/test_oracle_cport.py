@@ -67,3 +67,27 @@ def test_cport_sgld_matches_numpy_oracle():
                              atol=1e-5 * np.max(np.abs(r["theta"][-1])))
   np.testing.assert_array_equal(keys[0], r["key"])
   np.testing.assert_array_equal(dkeys[0], r["dkey"])
+
+
+def test_cport_amagold_matches_numpy_oracle():
+  """AMAGOLD (reversible leapfrog + full-data MH) in the C port against the numpy
+  oracle: same accept/reject decisions, positions within float32 rounding."""
+  N, D, n, its, steps, fr = 600, 6, 32, 25, 3, 0.3
+  X, y = _data(N, D, "logistic", seed=3)
+  fam = om.Family("logistic", X, y)
+  eps = np.full(its, 0.05, np.float32)
+  r = ost.amagold(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
+                  lambda th: om.full_potential(fam, th, 1.0, N), np.zeros(D), its, eps, steps,
+                  fr, N, n, [D], key=jr.PRNGKey(5), dkey=jr.PRNGKey(6))
+  assert r["accept"].any() and (~r["accept"]).any()
+  p0, k_int, k_mh = cport.amagold_init(np.zeros(D), jr.PRNGKey(5))
+  theta = np.zeros((1, D), np.float32)
+  mom = p0.reshape(1, D).copy()
+  keys = np.array([k_int], np.uint32)
+  mh_keys = np.array([k_mh], np.uint32)
+  dkeys = np.array([jr.PRNGKey(6)], np.uint32)
+  U = np.array([om.full_potential(fam, np.zeros(D), 1.0, N)])
+  acc = cport.amagold(1, X, y, n, theta, mom, keys, mh_keys, dkeys, U, eps, steps, fr)
+  np.testing.assert_array_equal(acc[0], r["accept"])
+  np.testing.assert_allclose(theta[0], r["theta"][-1], rtol=1e-5, atol=1e-6)
+  np.testing.assert_allclose(mom[0], r["p"][-1], rtol=1e-4, atol=1e-5)
