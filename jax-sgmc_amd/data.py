@@ -158,9 +158,13 @@ class DeviceNumpyDataLoader(DeviceDataLoader):
 
 
 class NumpyDataLoader(HostDataLoader):
-  """Host-resident arrays with numpy PCG64 draws (data/numpy_loader.py:147-460);
-  random draws with replacement and ordered batches. Batches are copied to the
-  device."""
+  """Host-resident arrays with numpy PCG64 index draws (data/numpy_loader.py:147-460).
+
+  Random pipelines draw with replacement (default), shuffle (successive permutations
+  of all observations, consumed as one stream) or shuffle in epochs (each epoch a
+  fresh permutation; the last batch of an epoch is padded and its surplus entries
+  masked). Ordered pipelines walk the rows with wrap-around. Chain state can be saved
+  and restored (save_state / load_state). Batches are copied to the device."""
 
   def __init__(self, copy=True, device="cuda", **reference_data):
     self.device = torch.device(device)
@@ -183,47 +187,132 @@ class NumpyDataLoader(HostDataLoader):
   def static_information(self):
     return {"observation_count": self._observation_count}
 
+  def initializer_batch(self, mb_size=None):
+    """Zero batch of mb_size observations with the dtypes a batch has on the device
+    (numpy_loader.py:60-86)."""
+    out = {}
+    for k, v in self._reference_data.items():
+      t = _to_device_array(v[:1], "cpu")
+      shape = (mb_size,) + tuple(v.shape[1:]) if mb_size is not None else tuple(v.shape[1:])
+      out[k] = torch.zeros(shape, dtype=t.dtype, device=self.device)
+    return out
+
+  # ---------------------------------------------------------------- chains ----
   def register_random_pipeline(self, cache_size=1, mb_size=None, in_epochs=False,
                                shuffle=False, **kwargs):
     if mb_size > self._observation_count:
       raise ValueError(f"The batch size cannot be bigger than the observation count. "
                        f"Provided {mb_size} and {self._observation_count}")
-    if shuffle or in_epochs:
-      raise NotImplementedError("shuffled host pipelines are not provided")
+    if in_epochs and not shuffle:
+      raise ValueError("in_epochs = True can only be used for shuffle = True.")
     chain_id = len(self._chains)
     seed = kwargs.get("seed", chain_id)
     rng = np.random.default_rng(np.random.SeedSequence(seed).spawn(1)[0])
+    N = self._observation_count
     self._chains.append({"type": "random", "rng": rng, "mb_size": mb_size,
-                         "cache_size": cache_size})
+                         "cache_size": cache_size, "shuffle": bool(shuffle),
+                         "in_epochs": bool(in_epochs), "buffer": None, "draws": -(-N // mb_size),
+                         "remaining": 0})
     return chain_id
 
   def register_ordered_pipeline(self, cache_size=1, mb_size=None, **kwargs):
-    assert mb_size <= self._observation_count
+    assert mb_size <= self._observation_count, (
+        f"The batch size cannot be bigger than the observation count. Provided {mb_size} "
+        f"and {self._observation_count}")
     chain_id = len(self._chains)
     self._chains.append({"type": "ordered", "idx_offset": 0, "mb_size": mb_size,
                          "cache_size": cache_size})
     return chain_id
 
+  def save_state(self, chain_id):
+    """Everything needed to continue chain_id from here (numpy_loader.py:185-205)."""
+    c = self._chains[chain_id]
+    if c["type"] == "ordered":
+      return {"ordered": c["idx_offset"]}
+    fields = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.items()
+              if k != "rng"}
+    return {"random": (c["rng"].bit_generator.state, fields)}
+
+  def load_state(self, chain_id, data):
+    """Restores a state from save_state into chain_id (numpy_loader.py:207-224)."""
+    kind, value = dict(data).popitem()
+    c = self._chains[chain_id]
+    if kind == "ordered":
+      c["idx_offset"] = value
+    elif kind == "random":
+      rng_state, fields = value
+      c["rng"].bit_generator.state = rng_state
+      for k, v in fields.items():
+        c[k] = v.copy() if isinstance(v, np.ndarray) else v
+    else:
+      raise ValueError(f"Chain type {kind} is unknown.")
+
+  # --------------------------------------------------------------- indices ----
+  def _permutation(self, chain):
+    N = self._observation_count
+    return chain["rng"].choice(np.arange(0, N), size=N, replace=False)
+
+  def _shuffled(self, chain):
+    """Shuffle without epochs (numpy_loader.py:393-433): the batches read a ring buffer
+    of (floor(N / n) + 2) n slots that is refilled with a fresh permutation, written
+    right after the unread entries, whenever fewer than n entries remain; the stream
+    of batches is the concatenation of successive permutations."""
+    N, n = self._observation_count, chain["mb_size"]
+    slots = (N // n + 2) * n
+    if chain["remaining"] < n:
+      perm = self._permutation(chain)
+      if chain["buffer"] is None:
+        chain["draws"] = 0
+        chain["buffer"] = np.zeros(slots, dtype=np.int64)
+      pos = np.mod(np.arange(N) + chain["draws"] * n + chain["remaining"], slots)
+      chain["buffer"][pos] = perm
+      chain["remaining"] += N
+    take = np.mod(np.arange(n) + chain["draws"] * n, slots)
+    sel = chain["buffer"][take].copy()
+    chain["draws"] = (chain["draws"] + 1) % (N // n + 2)
+    chain["remaining"] -= n
+    return sel, np.ones(n, dtype=bool)
+
+  def _epochs(self, chain):
+    """Shuffle in epochs (numpy_loader.py:435-460): ceil(N / n) batches per epoch from
+    one permutation; entries past N in the last batch are masked."""
+    N, n = self._observation_count, chain["mb_size"]
+    per_epoch = -(-N // n)
+    if chain["draws"] == per_epoch:
+      perm = self._permutation(chain)
+      if chain["buffer"] is None:
+        chain["buffer"] = np.zeros(per_epoch * n, dtype=np.int64)
+      chain["buffer"][:N] = perm
+      chain["draws"] = 0
+    lo = n * chain["draws"]
+    sel = chain["buffer"][lo:lo + n].copy()
+    chain["draws"] += 1
+    return sel, np.arange(lo, lo + n) < N
+
   def _indices(self, chain):
+    N = self._observation_count
     if chain["type"] == "ordered":
       idx = np.arange(chain["mb_size"]) + chain["idx_offset"]
-      mask = idx < self._observation_count
-      if chain["idx_offset"] + chain["mb_size"] >= self._observation_count:
+      mask = idx < N
+      if chain["idx_offset"] + chain["mb_size"] >= N:
         chain["idx_offset"] = 0
       else:
         chain["idx_offset"] += chain["mb_size"]
-      return np.mod(idx, self._observation_count), mask
-    sel = chain["rng"].choice(np.arange(0, self._observation_count), size=chain["mb_size"],
-                              replace=True)
+      return np.mod(idx, N), mask
+    if chain["in_epochs"]:
+      return self._epochs(chain)
+    if chain["shuffle"]:
+      return self._shuffled(chain)
+    sel = chain["rng"].choice(np.arange(0, N), size=chain["mb_size"], replace=True)
     return sel, np.ones(chain["mb_size"], dtype=bool)
 
   def get_batches(self, chain_id):
+    """cache_size batches of mb_size observations: ({name: [cache, mb, ...]}, mask)."""
     chain = self._chains[chain_id]
     pairs = [self._indices(chain) for _ in range(chain["cache_size"])]
     idx = np.array([p[0] for p in pairs])
     mask = np.array([p[1] for p in pairs], dtype=bool)
-    out = {k: _to_device_array(v[idx], self.device) if v.ndim > 1 or True else None
-           for k, v in self._reference_data.items()}
+    out = {k: _to_device_array(v[idx], self.device) for k, v in self._reference_data.items()}
     return out, torch.as_tensor(mask, device=self.device)
 
 
