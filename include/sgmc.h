@@ -151,6 +151,13 @@ int sgmc_sghmc_position(float* theta, float* p, const float* inv_mass, const flo
 int sgmc_sghmc_momentum(float* p, const float* grad, const float* friction_arr, float friction,
                         const int64_t* leaf_table, int nblocks, const uint32_t* key_in,
                         uint32_t* key_out, float eps, void* stream);
+/* friction_leapfrog._body_fun momentum part with a noise model (integrator.py:626-650,
+ * adaption.fisher_information): p += -eps g; (key_out, sub) = split(key_in);
+ * p += sqrt(2 eps) (cb_diff_sqrt xi), cb_diff_sqrt per element. */
+int sgmc_sghmc_momentum_noise_model(float* p, const float* grad, const float* cb_diff_sqrt,
+                                    const int64_t* leaf_table, int nblocks,
+                                    const uint32_t* key_in, uint32_t* key_out, float eps,
+                                    void* stream);
 /* obabo O-step (_momentum_resampling, integrator.py:192-200) with noise key
  * noise_key (one of split(key, 3)), and the kinetic energy 0.5 p^T M^-1 p
  * (integrator.py:177-179) of the input momentum (ke_of_input != 0, used for

@@ -112,6 +112,8 @@ _SIGNATURES = {
     "sgmc_obabo_workspace_floats": (c_int64, [c_int]),
     "sgmc_obabo_o": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_float,
                              c_float, c_float, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sgmc_sghmc_momentum_noise_model": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                                c_void_p, c_void_p, c_float, c_void_p]),
     "sgmc_obabo_ba": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float,
                               c_void_p]),
     "sgmc_obabo_b": (c_int, [c_void_p, c_void_p, c_int64, c_float, c_void_p]),

@@ -369,13 +369,14 @@ def sghmc(potential_fn, data_loader, cache_size=512, batch_size=32, integration_
           friction=1.0, mass=None, first_step_size=0.05, last_step_size=0.001, burn_in=0,
           accepted_samples=1000, adapt_noise_model=False, diagonal_noise=True,
           save_to_numpy=True, progress_bar=True):
-  """alias.py:451-540."""
-  if adapt_noise_model:
-    raise NotImplementedError("adapt_noise_model=True (fisher_information) is not provided "
-                              "by this build")
+  """alias.py:451-540. adapt_noise_model=True estimates the gradient noise with the
+  empirical Fisher information of each minibatch (adaption.fisher_information)."""
   random_data = DT.random_reference_data(data_loader, cache_size, batch_size)
+  noise_model = (AD.fisher_information(minibatch_potential=potential_fn,
+                                       diagonal=diagonal_noise)
+                 if adapt_noise_model else None)
   integ = IG.friction_leapfrog(potential_fn, random_data, friction=friction, const_mass=mass,
-                               steps=integration_steps)
+                               steps=integration_steps, noise_model=noise_model)
   sched = _schedule(first_step_size, last_step_size, burn_in, accepted_samples)
   saving = IO.save(to_numpy=True) if save_to_numpy else None
   solver = SV.sgmc(integ)
@@ -384,8 +385,8 @@ def sghmc(potential_fn, data_loader, cache_size=512, batch_size=32, integration_
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
              data_keys=None, fused=None):
     scalar_fric = np.ndim(friction) == 0 and not isinstance(friction, (dict, list, tuple))
-    if (init_model_state is None and mass is None and scalar_fric and
-        _fused_eligible(potential_fn, data_loader, init_samples, fused)):
+    if (init_model_state is None and mass is None and scalar_fric and not adapt_noise_model
+        and _fused_eligible(potential_fn, data_loader, init_samples, fused)):
       return _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
                         "sghmc", keys, data_keys, steps=integration_steps,
                         friction=float(friction), to_numpy=save_to_numpy,

@@ -119,6 +119,25 @@ __global__ void __launch_bounds__(kNT) sghmc_position_kernel(float* __restrict__
   p[i] = pi + ((-eps) * c) * s;
 }
 
+// friction_leapfrog momentum step with a noise model (integrator.py:632-650):
+// p += (-eps) g; (key_out, sub) = split(key_in); p += sqrt(2 eps) (cb_diff_sqrt xi).
+__global__ void __launch_bounds__(kNT) sghmc_momentum_nm_kernel(float* __restrict__ p,
+                                                                const float* __restrict__ grad,
+                                                                const float* __restrict__ cb,
+                                                                const int64_t* __restrict__ table,
+                                                                const uint32_t* __restrict__ key_in,
+                                                                uint32_t* __restrict__ key_out,
+                                                                float neg_eps, float noise_scale) {
+  Key next, sub;
+  split2(load_key(key_in), next, sub);
+  if (blockIdx.x == 0 && threadIdx.x == 0) store_key(key_out, next);
+  const LeafBlock lb = leaf_block(table, sub);
+  for_each_noise(lb, [&](int64_t i, float xi) {
+    const float pn = p[i] + neg_eps * grad[i];
+    p[i] = pn + noise_scale * (cb[i] * xi);
+  });
+}
+
 // friction_leapfrog momentum step: p += (-eps) g; (key_out, sub) = split(key_in);
 // p += C * (sqrt(2 eps) xi).
 __global__ void __launch_bounds__(kNT) sghmc_momentum_kernel(float* __restrict__ p,
@@ -318,6 +337,21 @@ int sgmc_sghmc_momentum(float* p, const float* grad, const float* friction_arr, 
                      grad, friction_arr, friction, leaf_table, key_in, key_out, -eps,
                      noise_scale);
   return check_launch("sgmc_sghmc_momentum");
+}
+
+int sgmc_sghmc_momentum_noise_model(float* p, const float* grad, const float* cb_diff_sqrt,
+                                    const int64_t* leaf_table, int nblocks,
+                                    const uint32_t* key_in, uint32_t* key_out, float eps,
+                                    void* stream) {
+  SGMC_REQUIRE(p && grad && cb_diff_sqrt && leaf_table && key_in && key_out, SGMC_E_ARG,
+               "sgmc_sghmc_momentum_noise_model: null pointer");
+  SGMC_REQUIRE(key_in != key_out, SGMC_E_ARG,
+               "sgmc_sghmc_momentum_noise_model: keys must not alias");
+  if (nblocks <= 0) return SGMC_OK;
+  const float noise_scale = sqrtf(2.0f * eps);
+  hipLaunchKernelGGL(sghmc_momentum_nm_kernel, dim3(nblocks), dim3(kNT), 0, as_stream(stream),
+                     p, grad, cb_diff_sqrt, leaf_table, key_in, key_out, -eps, noise_scale);
+  return check_launch("sgmc_sghmc_momentum_noise_model");
 }
 
 int sgmc_obabo_o(float* p, const float* sqrt_mass, const float* inv_mass,

@@ -83,6 +83,8 @@ def random_tree(key, a):
 def _flat_mass(mass, lay, device):
   if mass is None:
     return None, None
+  if getattr(mass.inv, "ndim", 1) == 2 or getattr(mass.sqrt, "ndim", 1) == 2:
+    raise NotImplementedError("the step kernels take a diagonal mass")
   inv = lay.ravel(mass.inv.tensor, device)
   sq = lay.ravel(mass.sqrt.tensor, device)
   return inv.contiguous(), sq.contiguous()
@@ -162,11 +164,13 @@ def langevin_diffusion(potential_fn, batch_fn, adaption=None):
 
 def friction_leapfrog(potential_fn, batch_fn, steps=10, friction=0.25, const_mass=None,
                       noise_model=None):
-  """Original SGHMC leapfrog (integrator.py:563-765) without a noise model."""
-  if noise_model:
-    raise NotImplementedError("SGHMC noise models (adapt_noise_model=True) are not provided "
-                              "by this build")
+  """Original SGHMC leapfrog (integrator.py:563-765). With a noise model (e.g.
+  adaption.fisher_information) the momentum noise is sqrt(2 eps) (cb_diff_sqrt xi)
+  with cb_diff_sqrt from the model at the new position and gradient
+  (integrator.py:632-650); without one it is C (sqrt(2 eps) xi)."""
   init_data, get_data, _ = batch_fn
+  if noise_model:
+    init_nm, update_nm, get_nm = noise_model[:3]
   cmass = init_mass(T.tree_map(lambda x: T.as_tensor(x, "cuda"), const_mass)) \
       if const_mass else None
 
@@ -177,7 +181,7 @@ def friction_leapfrog(potential_fn, batch_fn, steps=10, friction=0.25, const_mas
     return LeapfrogState(positions=sample, momentum=sample,
                          potential=torch.zeros((), device=dev), model_state=init_model_state,
                          data_state=init_data(**(batch_kwargs or {})), key=key,
-                         extra_fields=None)
+                         extra_fields=init_nm(sample) if noise_model else None)
 
   def integrate(state, parameters, mass=None):
     dev = state.key.device
@@ -198,17 +202,30 @@ def friction_leapfrog(potential_fn, batch_fn, steps=10, friction=0.25, const_mas
               _lib.ptr(state.key), _lib.ptr(k_a), _lib.stream())
     eps = float(parameters.step_size)
     data_state, ms, U = state.data_state, state.model_state, torch.zeros((), device=dev)
+    extra = state.extra_fields
+    fr_flat = None
+    if noise_model:
+      fr_flat = fr_arr if fr_arr is not None else torch.full((lay.size,), fr, device=dev)
     for _ in range(int(steps)):
       _lib.call("sgmc_sghmc_position", _lib.ptr(theta), _lib.ptr(p), _lib.ptr(inv_m),
                 _lib.ptr(fr_arr), fr, lay.size, eps, _lib.stream())
       data_state, mini_batch = get_data(data_state, information=True)
+      ms_before = ms
       (U, ms), g = potential_fn.value_and_grad(lay.unravel(theta), mini_batch, state=ms)
-      _lib.call("sgmc_sghmc_momentum", _lib.ptr(p), _lib.ptr(g.contiguous()), _lib.ptr(fr_arr),
-                fr, _lib.ptr(table), nb, _lib.ptr(k_a), _lib.ptr(k_b), eps, _lib.stream())
+      g = g.contiguous()
+      if noise_model:
+        extra = update_nm(extra, theta, g, fr_flat, mini_batch=mini_batch, step_size=eps)
+        nm = get_nm(extra, theta, g, fr_flat, mini_batch=mini_batch, step_size=eps,
+                    model_state=ms_before)
+        cb = lay.ravel(nm.cb_diff_sqrt.tensor, dev).contiguous()
+        _lib.call("sgmc_sghmc_momentum_noise_model", _lib.ptr(p), _lib.ptr(g), _lib.ptr(cb),
+                  _lib.ptr(table), nb, _lib.ptr(k_a), _lib.ptr(k_b), eps, _lib.stream())
+      else:
+        _lib.call("sgmc_sghmc_momentum", _lib.ptr(p), _lib.ptr(g), _lib.ptr(fr_arr), fr,
+                  _lib.ptr(table), nb, _lib.ptr(k_a), _lib.ptr(k_b), eps, _lib.stream())
       k_a, k_b = k_b, k_a
     return LeapfrogState(positions=lay.unravel(theta), momentum=lay.unravel(p), potential=U,
-                         model_state=ms, data_state=data_state, key=k_a,
-                         extra_fields=state.extra_fields)
+                         model_state=ms, data_state=data_state, key=k_a, extra_fields=extra)
 
   def get_fn(state):
     return {"variables": state.positions, "energy": state.potential,

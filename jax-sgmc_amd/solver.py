@@ -16,6 +16,8 @@
 - amagold(integrator, full_potential_fn, full_data_map) (solver.py:301-432):
   reversible-leapfrog proposal, the amortized test log a = min(0, U - U' + energy)
   on the FP64 full-data potential, momentum flipped on rejection.
+Both MH solvers take ``mass_adaption`` (e.g. adaption.mass_matrix): the integrator
+runs with the adapted mass and the strategy is updated with every post-test position.
 """
 import math
 from typing import Any, NamedTuple
@@ -24,7 +26,15 @@ import torch
 
 from . import io as sgio
 from . import random as R
+from . import tree as T
 from .tree import tree_map
+
+
+def _on_device(sample):
+  """The sample's leaves on its GPU (the device the integrators run on)."""
+  dev = next((l.device for l in T.flatten(sample)[0]
+              if isinstance(l, torch.Tensor) and l.is_cuda), torch.device("cuda"))
+  return tree_map(lambda l: T.as_tensor(l, dev), sample)
 
 
 class AMAGOLDState(NamedTuple):
@@ -126,10 +136,10 @@ def parallel_tempering(integrator, sa_schedule=lambda n: 1 / n):
 
 
 def sggmc(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None):
-  if mass_adaption:
-    raise NotImplementedError("mass adaption is not provided by this build")
   init_i, update_i, get_i = integrator_fn
   init_full, full_map, _ = full_data_map
+  if mass_adaption:
+    init_m, update_m, get_m = mass_adaption[:3]
 
   def init(init_sample, key=None, initial_mass=None, full_data_kwargs=None, **kwargs):
     fds = init_full(**(full_data_kwargs or {}))
@@ -139,14 +149,16 @@ def sggmc(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None):
     key = R.PRNGKey(0) if key is None else key
     k = R.split(key)
     ist = init_i(init_sample, key=k[0].clone(), **kwargs)
+    mass_state = init_m(_on_device(init_sample), initial_mass) if mass_adaption else None
     z = torch.zeros((), device=k.device)
     return SGGMCState(full_data_state=fds, potential=U.to(torch.float64), integrator_state=ist,
-                      key=k[1].clone(), acceptance_ratio=(z, z, z), mass_state=None)
+                      key=k[1].clone(), acceptance_ratio=(z, z, z), mass_state=mass_state)
 
   def update(state, schedule):
     k = R.split(state.key)
     key, sub = k[0].clone(), k[1].clone()
-    prop = update_i(state.integrator_state, schedule, mass=None)
+    mass = get_m(state.mass_state) if mass_adaption else None
+    prop = update_i(state.integrator_state, schedule, mass=mass)
     U_new, (fds, new_ms) = full_potential_fn(prop.positions, state.full_data_state, full_map,
                                              state=prop.model_state)
     U_new = U_new.to(torch.float64)
@@ -166,10 +178,11 @@ def sggmc(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None):
                          potential=src.potential, data_state=prop.data_state, key=prop.key,
                          kinetic_energy_start=z, kinetic_energy_end=z.clone())
     ratio = torch.exp(log_alpha)
+    mass_state = update_m(state.mass_state, src.positions) if mass_adaption else None
     new = SGGMCState(full_data_state=fds, potential=U_keep, integrator_state=new_int, key=key,
                      acceptance_ratio=(ratio, torch.tensor(float(schedule.step_size)),
                                        prop.kinetic_energy_end - prop.kinetic_energy_start),
-                     mass_state=None)
+                     mass_state=mass_state)
     return new, {"acceptance_ratio": ratio}
 
   def get(state):
@@ -190,28 +203,31 @@ def amagold(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None)
   log u < min(0, U - U' + energy). A rejected proposal keeps the previous positions,
   potential and model state and flips the momentum; the integrator key and the data
   state come from the proposal either way."""
-  if mass_adaption:
-    raise NotImplementedError("mass adaption is not provided by this build")
   init_i, update_i, get_i = integrator_fn
   init_full, full_map, _ = full_data_map
+  if mass_adaption:
+    init_m, update_m, get_m = mass_adaption[:3]
 
   def init(init_sample, key=None, initial_mass=None, full_data_kwargs=None, **kwargs):
+    mass_state = init_m(_on_device(init_sample), initial_mass) if mass_adaption else None
+    mass = get_m(mass_state) if mass_adaption else None
     fds = init_full(**(full_data_kwargs or {}))
     ims = kwargs.get("init_model_state")
     U, (fds, new_ms) = full_potential_fn(init_sample, fds, full_map, state=ims)
     kwargs["init_model_state"] = new_ms
     key = R.PRNGKey(0) if key is None else key
     k = R.split(key)
-    ist = init_i(init_sample, key=k[0].clone(), **kwargs)
+    ist = init_i(init_sample, key=k[0].clone(), mass=mass, **kwargs)
     z = torch.zeros((), device=k.device)
     return AMAGOLDState(full_data_state=fds, potential=U.to(torch.float64),
                         integrator_state=ist, key=k[1].clone(), acceptance_ratio=(z, z),
-                        mass_state=None)
+                        mass_state=mass_state)
 
   def update(state, schedule):
     k = R.split(state.key)
     key, sub = k[0].clone(), k[1].clone()
-    prop = update_i(state.integrator_state, schedule, mass=None)
+    mass = get_m(state.mass_state) if mass_adaption else None
+    prop = update_i(state.integrator_state, schedule, mass=mass)
     U_new, (fds, new_ms) = full_potential_fn(prop.positions, state.full_data_state, full_map,
                                              state=prop.model_state)
     U_new = U_new.to(torch.float64)
@@ -229,9 +245,10 @@ def amagold(integrator_fn, full_potential_fn, full_data_map, mass_adaption=None)
                          model_state=ms, data_state=prop.data_state, key=prop.key,
                          extra_fields=prop.extra_fields)
     ratio = torch.exp(log_alpha)
+    mass_state = update_m(state.mass_state, src.positions) if mass_adaption else None
     new = AMAGOLDState(full_data_state=fds, potential=U_keep, integrator_state=new_int,
                        key=key, acceptance_ratio=(ratio, torch.tensor(float(schedule.step_size))),
-                       mass_state=None)
+                       mass_state=mass_state)
     return new, {"acceptance_ratio": ratio}
 
   def get(state):

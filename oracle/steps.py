@@ -94,11 +94,38 @@ def sgld(pot, theta0, iterations, eps, temperature, N, n, leaf_sizes,
   return res
 
 
+def fisher_noise(G, g, N, n, friction, eps, diagonal=True):
+  """adaption.fisher_information get (adaption.py:403-455): G [n, P] gradients of the
+  observations' likelihoods, g the potential's gradient. diff = G - g / N;
+  v = sum diff^2 / (n - 1) (outer products when not diagonal); b = 0.5 eps v.
+  Diagonal: c = friction - b with entries c <= 0 replaced by the smallest positive
+  entry; returns (sqrt(c), sqrt(friction - c)). Full: (U_cb sqrt(s_cb), U sqrt(s)) from
+  the SVDs of diag(friction) - b and of b."""
+  G = np.asarray(G, np.float64)
+  diff = G - np.asarray(g, np.float64) / N
+  e = float(F32(eps))
+  friction = np.broadcast_to(np.asarray(friction, np.float64), (G.shape[1],))
+  if diagonal:
+    b = 0.5 * e * (np.sum(diff * diff, axis=0) / (n - 1))
+    corr = friction - b
+    pos = corr[corr > 0]
+    smallest = pos.min() if pos.size else np.inf
+    c = np.where(corr <= 0, smallest, corr)
+    return np.sqrt(c), np.sqrt(friction - c)
+  b = 0.5 * e * (diff.T @ diff / (n - 1))
+  u_cb, s_cb, _ = np.linalg.svd(np.diag(friction) - b)
+  u, s_, _ = np.linalg.svd(b)
+  return u_cb @ np.sqrt(s_cb), u @ np.sqrt(s_)
+
+
 def sghmc(pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
-          key=None, dkey=None, inv_mass=None, sqrt_mass=None):
-  """friction_leapfrog.integrate (integrator.py:715-757) with _body_fun (603-666),
-  no noise model: momentum resampled each iteration, then ``steps`` x {position,
-  friction decay, gradient, C * sqrt(2 eps) noise}."""
+          key=None, dkey=None, inv_mass=None, sqrt_mass=None, obs_grads=None):
+  """friction_leapfrog.integrate (integrator.py:715-757) with _body_fun (603-666):
+  momentum resampled each iteration, then ``steps`` x {position, friction decay,
+  gradient, noise}. The noise is C * sqrt(2 eps) xi, or, with
+  ``obs_grads(theta, idx) -> [n, P]`` (the diagonal Fisher noise model,
+  adaption.py:371-457), sqrt(2 eps) (cb_diff_sqrt xi) with cb_diff_sqrt from
+  ``fisher_noise`` at the new position."""
   key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
   dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
   theta = np.asarray(theta0, np.float64).copy()
@@ -125,7 +152,11 @@ def sghmc(pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
       p = p + float(-e) * g
       key, sub = jr.split(key)
       xi = jr.random_tree_flat(sub, leaf_sizes).astype(np.float64)
-      p = p + C * (noise_scale * xi)
+      if obs_grads is None:
+        p = p + C * (noise_scale * xi)
+      else:
+        cb, _ = fisher_noise(obs_grads(theta, idx), g, N, n, C, e)
+        p = p + noise_scale * (cb * xi)
       out["idx"].append(idx)
     out["theta"].append(theta.copy())
     out["p"].append(p.copy())
@@ -329,17 +360,21 @@ def reversible_leapfrog_integrate(pot, theta, p, key, dkey, eps, steps, friction
 
 
 def amagold(pot, full_pot, theta0, iterations, eps, steps, friction, N, n, leaf_sizes,
-            key=None, dkey=None, inv_mass=None, sqrt_mass=None):
+            key=None, dkey=None, inv_mass=None, sqrt_mass=None, mass_burn_in=None):
   """solver.amagold (solver.py:301-432) over reversible_leapfrog: init splits the
   solver key (integrator key, MH key); every iteration splits the MH key, integrates,
   evaluates the full-data potential at the proposal and accepts iff
   log u < min(0, U_full - U_full' + energy). A rejected proposal keeps theta and U_full
-  and flips the momentum; keys and the data state advance either way."""
+  and flips the momentum; keys and the data state advance either way. With
+  ``mass_burn_in`` the diagonal mass adapts (adaption.mass_matrix with that burn_in,
+  updated with every post-test position): from then on M^-1 is the biased variance of
+  the first mass_burn_in post-test positions and M^{1/2} its inverse square root."""
   key = jr.PRNGKey(0) if key is None else np.asarray(key, np.uint32)
   dkey = jr.PRNGKey(0) if dkey is None else np.asarray(dkey, np.uint32)
   theta = np.asarray(theta0, np.float64).copy()
   eps = np.broadcast_to(np.asarray(eps, np.float32), (iterations,))
   U_full = full_pot(theta)
+  kept = []
   ikey, key = jr.split(key)
   p, ikey = reversible_leapfrog_init(theta, ikey, leaf_sizes, sqrt_mass)
   out = {"theta": [], "p": [], "accept": [], "log_alpha": [], "U_full": [], "energy": []}
@@ -356,6 +391,11 @@ def amagold(pot, full_pot, theta0, iterations, eps, steps, friction, N, n, leaf_
       theta, p, U_full = th_new, p_new, U_new
     else:
       p = -p
+    if mass_burn_in is not None:
+      kept.append(theta.copy())
+      if len(kept) == mass_burn_in:
+        var = np.var(np.asarray(kept), axis=0)
+        inv_mass, sqrt_mass = var, 1.0 / np.sqrt(var)
     for k_, val in (("theta", theta), ("p", p), ("accept", acc), ("log_alpha", log_alpha),
                     ("U_full", U_full), ("energy", energy)):
       out[k_].append(np.copy(val))
@@ -388,3 +428,20 @@ def adaptive_step_size(acceptance, burn_in, initial_step_size, stabilization_con
     if i < burn_in:
       x_bar = xb
   return np.asarray(out, np.float32)
+
+
+def mass_adaption(samples, burn_in, init_cov=None):
+  """adaption.mass_matrix, diagonal (adaption.py:296-369), over a sequence of samples:
+  returns (M^-1, M^{1/2}) after each update. Until burn_in updates: init_cov and
+  1 / sqrt(init_cov); from then on the biased sample variance of the first burn_in
+  samples and its inverse square root."""
+  samples = np.asarray(samples, np.float64)
+  cov = np.ones(samples.shape[1]) if init_cov is None else np.asarray(init_cov, np.float64)
+  out = []
+  inv, sq = cov, 1.0 / np.sqrt(cov)
+  for t in range(len(samples)):
+    if t + 1 == burn_in:
+      var = np.var(samples[:burn_in], axis=0)
+      inv, sq = var, 1.0 / np.sqrt(var)
+    out.append((inv.copy(), sq.copy()))
+  return out

@@ -205,3 +205,61 @@ def test_fused_amagold_graph_replay_matches_eager():
                                   e2.sample_trace(c).cpu().numpy())
     np.testing.assert_array_equal(e1.mh.acceptance_ratios(c).cpu().numpy(),
                                   e2.mh.acceptance_ratios(c).cpu().numpy())
+
+
+@pytest.mark.parametrize("fric", [0.5, "per-element"])
+def test_sghmc_noise_model_matches_oracle(fric):
+  """alias.sghmc(adapt_noise_model=True): diagonal Fisher noise model
+  (adaption.py:371-457) in the friction leapfrog, trajectory against the oracle."""
+  print("sghmc noise model", fric, flush=True)
+  X, y = logreg(N=1500, D=12, seed=9)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  C = 0.5 if fric == 0.5 else (0.3 + 0.05 * np.arange(12)).astype(np.float32)
+  friction = C if fric == 0.5 else torch.as_tensor(C, device=DEV)
+  its, steps = 12, 3
+  run = alias.sghmc(pot, loader, cache_size=1, batch_size=64, integration_steps=steps,
+                    friction=friction, first_step_size=0.01, last_step_size=0.002,
+                    accepted_samples=its, adapt_noise_model=True)
+  res = run(torch.zeros(12), iterations=its)[0]["samples"]
+  eps = SC.polynomial_table(0.01, 0.002, its)
+  fam = om.Family("logistic", X, y)
+  r = ost.sghmc(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0), np.zeros(12), its,
+                eps, steps, C, 1500, 64, [12],
+                obs_grads=lambda th, idx: om.logistic(th, X[idx], y[idx])[1])
+  got = np.asarray(res["variables"], np.float64)
+  ref = r["theta"]
+  tol = 1e-5 * np.abs(ref) + 1e-5 * np.max(np.abs(ref))
+  assert np.all(np.abs(got - ref) <= tol), np.max(np.abs(got - ref))
+
+
+def test_amagold_mass_adaption_matches_oracle():
+  """solver.amagold with adaption.mass_matrix (solver.py:301-432, adaption.py:296-369):
+  the diagonal mass switches to the post-test positions' variance at burn-in."""
+  print("amagold mass adaption", flush=True)
+  from jax_sgmc_amd import adaption as AD
+  from jax_sgmc_amd import integrator as IG
+  from jax_sgmc_amd import io as IO
+  from jax_sgmc_amd import solver as SV
+  X, y = logreg(N=1200, D=6, seed=10)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  full = PT.full_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  its, steps, fr, burn = 16, 3, 0.3, 6
+  integ = IG.reversible_leapfrog(pot, DT.random_reference_data(loader, 1, 64), steps, fr)
+  solver = SV.amagold(integ, full, DT.full_reference_data(loader, 1, 64),
+                      mass_adaption=AD.mass_matrix(burn_in=burn))
+  sched = SC.init_scheduler(step_size=SC.polynomial_step_size_first_last(0.02, 0.01))
+  mcmc = SV.mcmc(solver, sched, saving=IO.save(to_numpy=True))
+  x0 = 0.2 * torch.ones(6)
+  res = mcmc(solver[0](x0), iterations=its)[0]["samples"]
+  fam = om.Family("logistic", X, y)
+  r = ost.amagold(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
+                  lambda th: om.full_potential(fam, th, 1.0, 1200), 0.2 * np.ones(6), its,
+                  SC.polynomial_table(0.02, 0.01, its), steps, fr, 1200, 64, [6],
+                  mass_burn_in=burn)
+  got = np.asarray(res["variables"], np.float64)
+  ref = r["theta"]
+  assert r["accept"].any()
+  tol = 1e-4 * np.abs(ref) + 1e-5 * np.max(np.abs(ref))
+  assert np.all(np.abs(got - ref) <= tol), np.max(np.abs(got - ref))
