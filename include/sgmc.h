@@ -242,7 +242,10 @@ typedef struct sgmc_glm_chain {
 
 int sgmc_glm_dclass(int D);
 /* chains, block_chain (int32[nblocks], chain of each grid block), eps_table,
- * save_slot, iter and done (a zeroed unsigned) are device memory. uniform_dclass is
+ * save_slot, iter and done (a zeroed unsigned) are device memory. block_chain may be
+ * NULL for a uniform table: every chain then has nblocks / nchains blocks and chain c
+ * owns blocks [c nblocks / nchains, (c + 1) nblocks / nchains) (block_begin must say
+ * so). uniform_dclass is
  * the common dclass of all chains, or -1 for a mixed table. With advance_iter != 0 the
  * last chain to finish the launch's final leapfrog step (step == steps - 1, or every
  * SGLD launch) advances iter; a table split into several launches per step (one per

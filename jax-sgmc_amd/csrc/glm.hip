@@ -527,7 +527,8 @@ __global__ void __launch_bounds__(kNT) glm_full_kernel(FullArgs a) {
 
 struct FusedArgs {
   const sgmc_glm_chain* chains;
-  const int32_t* block_chain;
+  const int32_t* block_chain;   // grid block -> chain, or NULL with blocks_per_chain
+  int blocks_per_chain;         // uniform table: chain = block / blocks_per_chain
   const float* eps_table;
   const int32_t* save_slot;
   int32_t* iter;
@@ -666,26 +667,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   if (!arrive_last(ch.counter, ch.nblocks, flag)) return;
 
   // ---- last block of this chain: potential, gradient and the state update ----
+  // Loads first: each of the first nb threads takes its block's (lik sum, M2) record
+  // now; the reductions that need them wait until after the parameter update, whose
+  // own loads (column sums, theta, momentum) do not depend on them, so the whole
+  // finalizer pays one memory round trip instead of one per phase.
   const int nb = ch.nblocks;
   const float c = (float)(-ch.N) / (float)ch.n;
-  float sl = 0.0f;
-  for (int b = threadIdx.x; b < nb; b += kNT) sl += consume(g_partial + (int64_t)b * stride + 4 * D4);
-  const float sumlik = block_sum<kNT>(sl, lds_small);
-  const float mean = sumlik / (float)ch.n;
-  // var(lik) = (sum_b M2_b + cnt_b (mean_b - mean)^2) / n  (pairwise/Chan combination)
-  float var_sum = 0.0f;
-  if (ch.var) {
-    const int per_block = (ch.n + nb - 1) / nb;
-    float t = 0.0f;
-    for (int b = threadIdx.x; b < nb; b += kNT) {
-      const int lo = min(ch.n, b * per_block);
-      const int k = min(ch.n, lo + per_block) - lo;
-      if (k > 0) {
-        const float d = consume(g_partial + (int64_t)b * stride + 4 * D4) / (float)k - mean;
-        t += consume(g_partial + (int64_t)b * stride + 4 * D4 + 1) + (float)k * (d * d);
-      }
-    }
-    var_sum = block_sum<kNT>(t, lds_small);
+  const int per_block = (ch.n + nb - 1) / nb;
+  float rec_ls = 0.0f, rec_m2 = 0.0f;
+  if (threadIdx.x < nb) {
+    rec_ls = consume(g_partial + (int64_t)threadIdx.x * stride + 4 * D4);
+    if (ch.var) rec_m2 = consume(g_partial + (int64_t)threadIdx.x * stride + 4 * D4 + 1);
   }
 
   // Noise key of this step.
@@ -787,6 +779,28 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       if (slot >= 0 && last_step && ch.samples) g_samples[(int64_t)slot * D + j] = wn;
     }
   }
+  // var(lik) = (sum_b M2_b + cnt_b (mean_b - mean)^2) / n  (pairwise/Chan combination)
+  float sl = rec_ls;
+  for (int b = threadIdx.x + kNT; b < nb; b += kNT)
+    sl += consume(g_partial + (int64_t)b * stride + 4 * D4);
+  const float sumlik = block_sum<kNT>(sl, lds_small);
+  const float mean = sumlik / (float)ch.n;
+  float var_sum = 0.0f;
+  if (ch.var) {
+    float t = 0.0f;
+    for (int b = threadIdx.x; b < nb; b += kNT) {
+      const int lo = min(ch.n, b * per_block);
+      const int k = min(ch.n, lo + per_block) - lo;
+      if (k > 0) {
+        const bool held = b < kNT;
+        const float ls_b = held ? rec_ls : consume(g_partial + (int64_t)b * stride + 4 * D4);
+        const float m2_b = held ? rec_m2 : consume(g_partial + (int64_t)b * stride + 4 * D4 + 1);
+        const float d = ls_b / (float)k - mean;
+        t += m2_b + (float)k * (d * d);
+      }
+    }
+    var_sum = block_sum<kNT>(t, lds_small);
+  }
   lp = block_sum<kNT>(lp, lds_small);
   if (amg) edot = block_sum<kNT>(edot, lds_small);
   if (threadIdx.x == 0) {
@@ -812,9 +826,14 @@ __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   __shared__ float lds_small[kWaves];
   __shared__ float lds_lik[kLdsRows];
   __shared__ int flag;
-  const int chain = a.block_chain[blockIdx.x];
+  // A uniform table maps blocks to chains arithmetically, which saves the dependent
+  // load of the map before the chain descriptor can be read.
+  const int chain = a.blocks_per_chain > 0 ? (int)blockIdx.x / a.blocks_per_chain
+                                           : a.block_chain[blockIdx.x];
   if (chain < 0 || chain >= a.nchains) return;
-  const sgmc_glm_chain& ch = a.chains[chain];
+  // The chain descriptor is copied once: block-uniform, it lives in scalar registers
+  // instead of being re-read after every barrier and atomic.
+  const sgmc_glm_chain ch = a.chains[chain];
   const int bchain = blockIdx.x - ch.block_begin;
   if (bchain < 0 || bchain >= ch.nblocks) return;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
@@ -965,8 +984,11 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
                         const float* sqrt_mass, const float* eps_table, int table_len,
                         const int32_t* save_slot, int32_t* iter, unsigned* done,
                         int advance_iter, void* stream) {
-  SGMC_REQUIRE(chains && block_chain && eps_table && iter && done, SGMC_E_ARG,
+  SGMC_REQUIRE(chains && eps_table && iter && done, SGMC_E_ARG,
                "sgmc_glm_fused_step: null pointer");
+  SGMC_REQUIRE(block_chain || (nchains > 0 && nblocks % nchains == 0), SGMC_E_ARG,
+               "sgmc_glm_fused_step: without a block map nblocks must be a multiple of "
+               "nchains");
   SGMC_REQUIRE(nchains >= 1 && nblocks >= nchains && table_len >= 1, SGMC_E_ARG,
                "sgmc_glm_fused_step: bad sizes");
   SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
@@ -980,6 +1002,7 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   FusedArgs a;
   a.chains = chains;
   a.block_chain = block_chain;
+  a.blocks_per_chain = block_chain ? 0 : nblocks / nchains;
   a.eps_table = eps_table;
   a.save_slot = save_slot;
   a.iter = iter;

@@ -196,12 +196,16 @@ class FusedGLM:
     self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
     bc = np.repeat(order.astype(np.int32), self.nblocks_chain[order])
     self.block_chain = torch.as_tensor(bc, device=self.device)
+    # a uniform table in chain order needs no block map (the kernel divides)
+    self.uniform_blocks = bool(np.all(self.nblocks_chain == self.nblocks_chain[0]) and
+                               np.array_equal(order, np.arange(self.C)))
     self._arr = arr  # keep the host copy alive for inspection
   # ------------------------------------------------------------ launching ----
   def _launch(self, step):
     # A mixed-D table runs as one launch of the kernel with a block-uniform D-class
     # switch: per-class launches leave the device underfilled on small problems.
-    _lib.call("sgmc_glm_fused_step", _lib.ptr(self.table), self.C, _lib.ptr(self.block_chain),
+    bmap = None if self.uniform_blocks else _lib.ptr(self.block_chain)
+    _lib.call("sgmc_glm_fused_step", _lib.ptr(self.table), self.C, bmap,
               self.nblocks, self.uniform_dclass, self.family, self.sigma, self.has_prior,
               self.prior_scale, self.pot_temperature, self.mode, step, self.steps,
               self.friction, self.alpha, self.lmbd, _lib.ptr(self.inv_mass),
