@@ -281,10 +281,14 @@ __device__ __forceinline__ void step_pstate(const Step& s, PState& p, float gsum
     const float sg = (-k.eps) * g;
     const float sn = k.noise_scale * xi;
     if (s.rms) {
+      // G = 1 / (lambda + sqrt(v)) and sqrt(G) with the hardware reciprocal and square
+      // root (about 1 ulp each) instead of the correctly rounded sequences: this
+      // epilogue is bound by its per-parameter arithmetic, and the bf16 / fp8 GEMM path
+      // it serves is compared at a far looser tolerance.
       const float vn = s.alpha * p.aux + (1.0f - s.alpha) * (g * g);
       p.aux = vn;
-      const float G = 1.0f / (s.lmbd + sqrtf(vn));
-      p.w = w + ((0.0f + G * sg) + sqrtf(G) * sn);
+      const float G = __builtin_amdgcn_rcpf(s.lmbd + __builtin_amdgcn_sqrtf(vn));
+      p.w = w + ((0.0f + G * sg) + __builtin_amdgcn_sqrtf(G) * sn);
     } else {
       p.w = w + (sg + sn);
     }
