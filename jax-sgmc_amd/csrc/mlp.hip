@@ -661,26 +661,33 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     float lp = 0.0f, ke = 0.0f, wmax = 0.0f;
     if (EPI == EPI_UPDATE) {
       const Key lk = leaf_key(s, c, g.leaf);
-      // every load of the thread's 32 parameters first, then the arithmetic and stores
-      PState p0[16], p1[16];
+      // Four chunks of eight parameters: a chunk's loads go out before its arithmetic
+      // and stores. Small chunks keep the kernel within 128 registers, so four waves
+      // per SIMD share it and one wave's noise arithmetic hides another's memory time.
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-        p0[j] = load_pstate(s, base + i0);
-        p1[j] = load_pstate(s, base + i0 + (int64_t)half * g.N);
-      }
+      for (int j0 = 0; j0 < 16; j0 += 4) {
+        PState p0[4], p1[4];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-        const int64_t i1 = i0 + (int64_t)half * g.N;
-        uint32_t w0, w1;
-        bits_pair(lk, S, (uint32_t)i0, w0, w1);
-        step_pstate(s, p0[j], ab * acc0[j], bits_to_normal(w0), k, lp, ke);
-        step_pstate(s, p1[j], ab * acc1[j], bits_to_normal(w1), k, lp, ke);
-        store_pstate(s, c, g.leaf, base, i0, p0[j], wmax);
-        store_pstate(s, c, g.leaf, base, i1, p1[j], wmax);
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u;
+          const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+          p0[u] = load_pstate(s, base + i0);
+          p1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u;
+          const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+          const int64_t i1 = i0 + (int64_t)half * g.N;
+          uint32_t w0, w1;
+          bits_pair(lk, S, (uint32_t)i0, w0, w1);
+          step_pstate(s, p0[u], ab * acc0[j], bits_to_normal(w0), k, lp, ke);
+          step_pstate(s, p1[u], ab * acc1[j], bits_to_normal(w1), k, lp, ke);
+          store_pstate(s, c, g.leaf, base, i0, p0[u], wmax);
+          store_pstate(s, c, g.leaf, base, i1, p1[u], wmax);
+        }
       }
     } else {
       float w0v[16], w1v[16];
