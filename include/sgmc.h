@@ -272,6 +272,19 @@ int sgmc_glm_amagold_mh(const sgmc_glm_chain* chains, int nchains, const double*
                         int n_slots, const int32_t* save_slot, int table_len,
                         const int32_t* iter, void* stream);
 
+/* `iterations` iterations of a table whose chains all have one block (nblocks == 1,
+ * block c is chain c) in one launch: each block runs its chain's evaluations back to
+ * back (every leapfrog step of every iteration) with a barrier in between, no arrival
+ * counter and no launch per evaluation; step sizes, save slots and keys advance as
+ * with sgmc_glm_fused_step, and iter advances by `iterations`. Modes SGLD, SGLD_RMS
+ * and SGHMC (AMAGOLD tests every iteration between launches). */
+int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_dclass,
+                        int family, float sigma, int has_prior, float prior_scale,
+                        float pot_temperature, int mode, int steps, float friction, float alpha,
+                        float lmbd, const float* inv_mass, const float* sqrt_mass,
+                        const float* eps_table, int table_len, const int32_t* save_slot,
+                        int32_t* iter, unsigned* done, int iterations, void* stream);
+
 /* ---------------------------------------------------- replica exchange ----
  * parallel_tempering's swap step (solver.py:264-293, alias.py:122-207) for pairs of
  * chains of a fused chain table, run after the chains' update of iteration

@@ -99,6 +99,10 @@ _SIGNATURES = {
                                                c_int, c_void_p, c_int64, c_int, c_float, c_int,
                                                c_float, c_double, c_void_p, c_void_p, c_int64,
                                                c_int, c_void_p]),
+    "sgmc_glm_fused_loop": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_float,
+                                    c_float, c_int, c_int, c_float, c_float, c_float, c_void_p,
+                                    c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_int, c_void_p]),
     "sgmc_glm_amagold_mh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "sgmc_sgld_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int, c_void_p,

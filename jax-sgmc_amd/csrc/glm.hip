@@ -537,7 +537,9 @@ struct FusedArgs {
   const float* sqrt_mass;
   int table_len;
   int nchains;
-  int advance;             // the launch's last chain advances the iteration counter
+  int advance;             // the launch's last chain advances the iteration counter;
+                           // < 0: no counter bookkeeping (the loop kernel does it)
+  int iter_offset;         // iteration of this evaluation = *iter + iter_offset
   int mode;                // SGMC_MODE_*
   int step;                // leapfrog step within the iteration (SGHMC)
   int steps;
@@ -559,7 +561,7 @@ template <int LPR, int CPL>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_chain& ch,
                                            int bchain, float* lds_cols, float* lds_small,
                                            float* lds_lik, int* flag) {
-  const int it = max(0, min(a.iter[0], a.table_len - 1));
+  const int it = max(0, min(a.iter[0] + a.iter_offset, a.table_len - 1));
   const float eps = a.eps_table[it];
   const int D = ch.D;
   const int D4 = (D + 3) >> 2;
@@ -664,7 +666,13 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   }
   if (ch.lik)
     for (int j = threadIdx.x; j < cnt; j += kNT) g_lik[b0 + j] = lds_lik[j];
-  if (!arrive_last(ch.counter, ch.nblocks, flag)) return;
+  // A chain of one block needs no arrival counter: a barrier makes its records
+  // visible to its own threads.
+  if (ch.nblocks == 1) {
+    __syncthreads();
+  } else if (!arrive_last(ch.counter, ch.nblocks, flag)) {
+    return;
+  }
 
   // ---- last block of this chain: potential, gradient and the state update ----
   // Loads first: each of the first nb threads takes its block's (lik sum, M2) record
@@ -812,10 +820,53 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     store_key(ch.key, knext);
     store_key(ch.dkey, dnext);
     // Last chain of the launch advances the iteration counter (read by the next launch).
+    if (a.advance >= 0) {
+      const unsigned prev = atomicAdd(a.done, 1u);
+      if (prev == (unsigned)a.nchains - 1) {
+        atomicExch(a.done, 0u);
+        if (last_step && a.advance) atomicAdd(a.iter, 1);
+      }
+    }
+  }
+}
+
+// Many iterations in one launch for tables whose chains have one block each: block c
+// runs chain c's iterations back to back (every leapfrog step of every iteration),
+// reading its parameters and keys back from memory after a barrier, so no launch and
+// no arrival counter separates two evaluations. The last chain to finish advances the
+// iteration counter by the number of iterations.
+template <int DCLASS>
+__global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iterations) {
+  __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
+  __shared__ float lds_small[kWaves];
+  __shared__ float lds_lik[kLdsRows];
+  __shared__ int flag;
+  const int chain = blockIdx.x;
+  if (chain >= a.nchains) return;
+  const sgmc_glm_chain ch = a.chains[chain];
+  if (ch.nblocks != 1) return;
+  const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
+  FusedArgs b = a;
+  b.advance = -1;
+  for (int t = 0; t < iterations; ++t) {
+    b.iter_offset = t;
+    for (int st = 0; st < a.steps; ++st) {
+      b.step = st;
+      switch (dc) {
+        case 0: fused_body<4, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 1: fused_body<16, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 2: fused_body<64, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 3: fused_body<64, 2>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        default: fused_body<64, 4>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
     const unsigned prev = atomicAdd(a.done, 1u);
     if (prev == (unsigned)a.nchains - 1) {
       atomicExch(a.done, 0u);
-      if (last_step && a.advance) atomicAdd(a.iter, 1);
+      atomicAdd(a.iter, iterations);
     }
   }
 }
@@ -1022,6 +1073,7 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   a.pr = make_prior(has_prior, prior_scale);
   a.pot_temperature = pot_temperature;
   a.advance = advance_iter ? 1 : 0;
+  a.iter_offset = 0;
   hipStream_t s = as_stream(stream);
   dim3 grid(nblocks), block(kNT);
   switch (uniform_dclass) {
@@ -1033,6 +1085,61 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
     default: hipLaunchKernelGGL((glm_fused_kernel<-1>), grid, block, 0, s, a); break;
   }
   return check_launch("sgmc_glm_fused_step");
+}
+
+int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_dclass,
+                        int family, float sigma, int has_prior, float prior_scale,
+                        float pot_temperature, int mode, int steps, float friction, float alpha,
+                        float lmbd, const float* inv_mass, const float* sqrt_mass,
+                        const float* eps_table, int table_len, const int32_t* save_slot,
+                        int32_t* iter, unsigned* done, int iterations, void* stream) {
+  SGMC_REQUIRE(chains && eps_table && iter && done, SGMC_E_ARG,
+               "sgmc_glm_fused_loop: null pointer");
+  SGMC_REQUIRE(nchains >= 1 && table_len >= 1 && iterations >= 0, SGMC_E_ARG,
+               "sgmc_glm_fused_loop: bad sizes");
+  SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
+               "sgmc_glm_fused_loop: unknown family %d", family);
+  SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS || mode == SGMC_MODE_SGHMC,
+               SGMC_E_ARG, "sgmc_glm_fused_loop: mode %d needs a test between iterations",
+               mode);
+  SGMC_REQUIRE(steps >= 1, SGMC_E_ARG, "sgmc_glm_fused_loop: steps %d", steps);
+  SGMC_REQUIRE(pot_temperature != 0.0f, SGMC_E_ARG, "sgmc_glm_fused_loop: temperature is zero");
+  if (iterations == 0) return SGMC_OK;
+  FusedArgs a;
+  a.chains = chains;
+  a.block_chain = nullptr;
+  a.blocks_per_chain = 1;
+  a.eps_table = eps_table;
+  a.save_slot = save_slot;
+  a.iter = iter;
+  a.done = done;
+  a.inv_mass = inv_mass;
+  a.sqrt_mass = sqrt_mass;
+  a.table_len = table_len;
+  a.nchains = nchains;
+  a.mode = mode;
+  a.step = 0;
+  a.steps = mode == SGMC_MODE_SGHMC ? steps : 1;
+  a.friction = friction;
+  a.four_friction = (float)(4.0 * (double)friction);
+  a.alpha = alpha;
+  a.lmbd = lmbd;
+  a.L = make_lik(family, sigma);
+  a.pr = make_prior(has_prior, prior_scale);
+  a.pot_temperature = pot_temperature;
+  a.advance = -1;
+  a.iter_offset = 0;
+  hipStream_t s = as_stream(stream);
+  dim3 grid(nchains), block(kNT);
+  switch (uniform_dclass) {
+    case 0: hipLaunchKernelGGL((glm_loop_kernel<0>), grid, block, 0, s, a, iterations); break;
+    case 1: hipLaunchKernelGGL((glm_loop_kernel<1>), grid, block, 0, s, a, iterations); break;
+    case 2: hipLaunchKernelGGL((glm_loop_kernel<2>), grid, block, 0, s, a, iterations); break;
+    case 3: hipLaunchKernelGGL((glm_loop_kernel<3>), grid, block, 0, s, a, iterations); break;
+    case 4: hipLaunchKernelGGL((glm_loop_kernel<4>), grid, block, 0, s, a, iterations); break;
+    default: hipLaunchKernelGGL((glm_loop_kernel<-1>), grid, block, 0, s, a, iterations); break;
+  }
+  return check_launch("sgmc_glm_fused_loop");
 }
 
 }  // extern "C"

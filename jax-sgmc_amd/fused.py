@@ -217,6 +217,21 @@ class FusedGLM:
     for s in range(self.steps):
       self._launch(s)
 
+  def loop_capable(self):
+    """Chains of one block each and no test between iterations: many iterations can
+    run in one launch (``sgmc_glm_fused_loop``)."""
+    return (bool(np.all(self.nblocks_chain == 1)) and self.exchange is None and
+            self.mh is None and self.mode in (_lib.MODE_SGLD, _lib.MODE_SGLD_RMS,
+                                              _lib.MODE_SGHMC))
+
+  def _launch_loop(self, iterations):
+    _lib.call("sgmc_glm_fused_loop", _lib.ptr(self.table), self.C, self.uniform_dclass,
+              self.family, self.sigma, self.has_prior, self.prior_scale, self.pot_temperature,
+              self.mode, self.steps, self.friction, self.alpha, self.lmbd,
+              _lib.ptr(self.inv_mass), _lib.ptr(self.sqrt_mass), _lib.ptr(self.eps_table),
+              self.table_len, _lib.ptr(self.save_slot), _lib.ptr(self.iter),
+              _lib.ptr(self.done), int(iterations), _lib.stream())
+
   def iteration(self):
     """Advances every chain by one reference iteration (steps launches), followed by
     the replica-exchange step when one is attached."""
@@ -240,9 +255,22 @@ class FusedGLM:
     self._graph_iters = iterations_per_graph
     return g
 
-  def run(self, iterations, graph_iterations=0):
+  def run(self, iterations, graph_iterations=0, loop=None):
     """Runs ``iterations`` iterations (eagerly, or by replaying a captured graph of
-    ``graph_iterations`` iterations; the remainder runs eagerly)."""
+    ``graph_iterations`` iterations; the remainder runs eagerly). A table of
+    single-block chains with nothing between iterations runs them in loop launches of
+    up to 1000 iterations instead (``loop=False`` disables that)."""
+    if loop is None:
+      loop = self.loop_capable()
+    if loop:
+      if not self.loop_capable():
+        raise ValueError("loop launches need single-block chains and no exchange or MH")
+      done = 0
+      while done < iterations:
+        k = min(1000, iterations - done)
+        self._launch_loop(k)
+        done += k
+      return
     if graph_iterations and iterations >= graph_iterations:
       g = getattr(self, "_graph", None)
       if g is None or self._graph_iters != graph_iterations:

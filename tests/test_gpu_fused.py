@@ -199,3 +199,47 @@ def test_exchange_ladder_graph_equals_eager():
   assert torch.equal(a.theta_flat, b.theta_flat)
   assert torch.equal(exa.swaps, exb.swaps) and torch.equal(exa.ssq, exb.ssq)
   assert torch.equal(exa.samples, exb.samples)
+
+
+@pytest.mark.parametrize("mode", ["sgld", "sgld_rms", "sghmc"])
+def test_loop_launch_equals_per_evaluation_launches(mode):
+  """Single-block chains: many iterations in one loop launch give bit for bit the
+  chains of one launch per evaluation (saved samples, keys, iteration counter)."""
+  print("loop launch", mode, flush=True)
+  rng = np.random.default_rng(12)
+  tables = []
+  for _ in range(2):
+    chains = []
+    for c, (N, D, n) in enumerate([(500, 8, 32), (800, 33, 64), (300, 100, 17)]):
+      X = (rng.standard_normal((N, D)) / np.sqrt(D)).astype(np.float32) if not tables else \
+          tables[0][c][0]
+      y = (rng.random(N) < 0.5).astype(np.float32) if not tables else tables[0][c][1]
+      chains.append((X, y, D, n))
+    tables.append(chains)
+  its = 23
+  eps = np.linspace(0.01, 0.002, its).astype(np.float32)
+  engines = []
+  for _ in range(2):
+    spec = []
+    for c, (X, y, D, n) in enumerate(tables[0]):
+      Xp = np.zeros((X.shape[0], (D + 3) // 4 * 4), np.float32)
+      Xp[:, :D] = X
+      spec.append(dict(X=torch.as_tensor(Xp, device=DEV), y=torch.as_tensor(y, device=DEV), D=D,
+                       n=n,
+                       theta0=np.zeros(D, np.float32), key=(1, c), dkey=(2, c)))
+    eng = fused.FusedGLM(spec, "logistic", mode, eps, steps=3, friction=0.4,
+                      save_slots=np.arange(its, dtype=np.int32), n_slots=its,
+                      target_bytes=1 << 30)
+    assert eng.loop_capable()
+    engines.append(eng)
+  engines[0].run(its, loop=False)
+  engines[1].run(its, loop=True)
+  torch.cuda.synchronize()
+  a, b = engines
+  assert a.iteration_count() == b.iteration_count() == its
+  for c in range(3):
+    np.testing.assert_array_equal(a.sample_trace(c).cpu().numpy(), b.sample_trace(c).cpu().numpy())
+    np.testing.assert_array_equal(a.sample_potentials(c).cpu().numpy(),
+                                  b.sample_potentials(c).cpu().numpy())
+  np.testing.assert_array_equal(a.keys.cpu().numpy(), b.keys.cpu().numpy())
+  np.testing.assert_array_equal(a.dkeys.cpu().numpy(), b.dkeys.cpu().numpy())
