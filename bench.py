@@ -599,8 +599,8 @@ def main():
                   help="GEMM operands of the MLP workloads (sggmc-mlp default fp8, psgld-mlp bf16)")
   args = ap.parse_args()
   if args.gpus != 1:
-    print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-          "(chains shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+    print(f"bench.py: the benchmark runs on one GPU only; --gpus {args.gpus} is not supported "
+          "(chains would shard across GPUs without communication)", file=sys.stderr)
     sys.exit(2)
 
   if args.workload == "resgld-ladder":

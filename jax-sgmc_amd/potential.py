@@ -30,7 +30,7 @@ def _check_strategy(strategy):
   if strategy not in ("map", "vmap", "pmap"):
     raise NotImplementedError(f"Strategy {strategy} is unknown")
   if strategy == "pmap":
-    raise NotImplementedError("Strategy pmap needs several devices; this build runs on one "
+    raise NotImplementedError("Strategy pmap needs several devices; the library runs on one "
                               "GPU (use 'map' or 'vmap')")
 
 
