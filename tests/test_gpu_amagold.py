@@ -263,3 +263,54 @@ def test_amagold_mass_adaption_matches_oracle():
   assert r["accept"].any()
   tol = 1e-4 * np.abs(ref) + 1e-5 * np.max(np.abs(ref))
   assert np.all(np.abs(got - ref) <= tol), np.max(np.abs(got - ref))
+
+
+def _c2_data():
+  """The headline workload's data: X ~ N(0,1)/sqrt(512) [1e6, 512], y ~ Bernoulli."""
+  kx, kw, ku = R.split(R.PRNGKey(1), 3)
+  X = R.normal(kx, (1_000_000 * 512,)).reshape(1_000_000, 512)
+  X.mul_(1.0 / np.sqrt(512))
+  w = R.normal(kw, (512,))
+  y = (R.uniform(ku, (1_000_000,)) < torch.sigmoid(X @ w)).to(torch.float32)
+  return X, y, w
+
+
+def test_full_size_properties():
+  """At the benchmark's size (N = 1e6, D = 512, 64 chains): the one-pass multi-chain
+  full potential equals per-chain FP64 passes (independent float32 row evaluations,
+  both summed in FP64) to 1e-7 relative (a few float32 ulps per row; observed 7e-9),
+  and a frictionless fused AMAGOLD iteration's energy telescopes to
+  KE(p_start) - KE(p_end) (integrator.py:430-447)."""
+  print("full-size properties", flush=True)
+  from jax_sgmc_amd import fused as FU
+  X, y, w = _c2_data()
+  C = 64
+  thetas = (w[None, :] + 0.05 * R.normal(R.PRNGKey(9), (C * 512,)).reshape(C, 512)).contiguous()
+  nb = 256
+  ws = torch.zeros(int(L.query("sgmc_glm_full_chains_workspace_doubles", nb)),
+                   dtype=torch.float64, device=DEV)
+  U = torch.empty(C, dtype=torch.float64, device=DEV)
+  L.call("sgmc_glm_full_potential_chains", L.LIK_LOGISTIC, L.ptr(X), 512, L.ptr(y), 1_000_000,
+         512, L.ptr(thetas), 512, C, 1.0, 1, 1.0, 1.0, L.ptr(U), L.ptr(ws), ws.numel(), nb,
+         L.stream())
+  rpb = 1024
+  ws1 = torch.zeros(int(L.query("sgmc_glm_full_workspace_doubles", 1_000_000, rpb)),
+                    dtype=torch.float64, device=DEV)
+  for c in (0, 17, 63):
+    u1 = torch.empty(1, dtype=torch.float64, device=DEV)
+    L.call("sgmc_glm_full_potential", L.LIK_LOGISTIC, L.ptr(X), 512, L.ptr(y), 1_000_000, 512,
+           L.ptr(thetas[c]), 1.0, 1, 1.0, 1.0, L.ptr(u1), L.ptr(ws1), ws1.numel(), rpb,
+           L.stream())
+    assert abs(U[c].item() - u1.item()) <= 1e-7 * abs(u1.item()), (c, U[c].item(), u1.item())
+  th0 = w.cpu().numpy().astype(np.float32)
+  chains = [dict(X=X, y=y, D=512, n=4096, theta0=th0, dkey=(3, c)) for c in range(8)]
+  eng = FU.FusedGLM(chains, "logistic", "amagold", np.full(2, 2e-4, np.float32), steps=5,
+                    friction=0.0)
+  FU.AmagoldMH(eng, keys=[(4, c) for c in range(8)])
+  p_start = eng.mom_flat.clone().reshape(8, 512).double()
+  eng._launch_steps()
+  torch.cuda.synchronize()
+  p_end = eng.mom_flat.reshape(8, 512).double()
+  ke = 0.5 * (p_start * p_start).sum(1) - 0.5 * (p_end * p_end).sum(1)
+  np.testing.assert_allclose(eng.energy.double().cpu().numpy(), ke.cpu().numpy(), rtol=2e-4,
+                             atol=1e-3)
