@@ -42,6 +42,8 @@ extern "C" {
 #define SGMC_MODE_SGLD_RMS 1
 #define SGMC_MODE_SGHMC 2
 #define SGMC_MODE_AMAGOLD 3
+#define SGMC_MODE_OBABO 4
+#define SGMC_MODE_SGGMC 5
 
 const char* sgmc_last_error(void);
 int sgmc_abi_version(void);
@@ -205,6 +207,16 @@ int sgmc_amagold_momentum(float* p, const float* grad, const float* inv_mass,
  *   accumulates 0.5 eps (p + p') . M^-1 g, step 0 keeps theta_old / mom_old and the
  *   last step adds the final half position step. Samples are saved by
  *   sgmc_glm_amagold_mh.
+ *   SGMC_MODE_OBABO: 2 `steps` launches per iteration (obabo.integrate,
+ *   integrator.py:318-338, over _leapfrog_steps 203-273), `steps` counting leapfrog
+ *   steps; with (key', s1, s2) = split(key, 3), even launch 2k evaluates at theta and
+ *   applies O (noise of s1, energy[0] += 0.5 p.M^-1 p of the refreshed momentum), B
+ *   and A; odd launch 2k+1 applies B and O (noise of s2, energy[1] += 0.5 p.M^-1 p
+ *   before the refresh) and stores key'. U holds U0 after the even launch and
+ *   0.5 (U0 + U1) after the odd one. The noise scale uses the chain's temperature.
+ *   SGMC_MODE_SGGMC: OBABO proposals for sgmc_glm_sggmc_mh (solver.py:434-567): the
+ *   first launch also keeps theta_old, mom_old and energy[2] = U of the iteration's
+ *   start; samples are saved by the test.
  * It replaces solver.mcmc's per-chain sequential loop (solver.py:165-171, strategy
  * 'map') for chains that share a likelihood family. The step size of iteration t is
  * eps_table[t] with t = *iter (a device counter the launch advances after the last
@@ -235,9 +247,11 @@ typedef struct sgmc_glm_chain {
   int32_t nblocks;     /* grid blocks of this chain */
   int32_t dclass;      /* sgmc_glm_dclass(D) */
   float temperature;   /* schedule temperature of this chain (SGLD noise) */
-  float* energy;       /* [1] AMAGOLD: energy accumulated over the iteration, or NULL */
-  float* theta_old;    /* [D] AMAGOLD: position at the start of the iteration, or NULL */
-  float* mom_old;      /* [D] AMAGOLD: momentum at the start of the iteration, or NULL */
+  float* energy;       /* AMAGOLD: [1] energy accumulated over the iteration; OBABO: [2]
+                          kinetic energy start, end; SGGMC: [3] those and the start U;
+                          NULL in the other modes */
+  float* theta_old;    /* [D] AMAGOLD, SGGMC: position at the iteration's start, or NULL */
+  float* mom_old;      /* [D] AMAGOLD, SGGMC: momentum at the iteration's start, or NULL */
 } sgmc_glm_chain;
 
 int sgmc_glm_dclass(int D);
@@ -272,12 +286,29 @@ int sgmc_glm_amagold_mh(const sgmc_glm_chain* chains, int nchains, const double*
                         int n_slots, const int32_t* save_slot, int table_len,
                         const int32_t* iter, void* stream);
 
+/* SGGMC's Metropolis-Hastings test (solver.py:501-567) for every chain of a fused
+ * chain table stepped with SGMC_MODE_SGGMC, after the iteration's last launch
+ * (it = iter[0] - 1) and after U_prop[c], the full potential of chain c's proposal:
+ * (key', sub) = split(keys[c]); u = uniform(sub);
+ * log a = -1/T (U_prop - U_full + energy[1] - energy[0]) in FP64, T the chain's
+ * temperature, set to 0 unless log a <= 0 (so NaN accepts, as jnp.where does);
+ * accept iff log u < log a. Accepted: U_full[c] = U_prop[c]. Rejected: theta, mom and
+ * U return to theta_old, mom_old and energy[2]. The kinetic energies restart at zero
+ * either way. At save_slot[it] >= 0 the chain's samples get the kept theta and
+ * samples_U the kept U; ratio_samples[c][slot] = exp(log a), ke_samples[c][slot] =
+ * energy[1] - energy[0] of the proposal and U_samples[c][slot] the kept U_full
+ * (each of these three may be NULL). */
+int sgmc_glm_sggmc_mh(const sgmc_glm_chain* chains, int nchains, const double* U_prop,
+                      double* U_full, uint32_t* keys, float* ratio_samples, float* ke_samples,
+                      double* U_samples, int n_slots, const int32_t* save_slot, int table_len,
+                      const int32_t* iter, void* stream);
+
 /* `iterations` iterations of a table whose chains all have one block (nblocks == 1,
  * block c is chain c) in one launch: each block runs its chain's evaluations back to
  * back (every leapfrog step of every iteration) with a barrier in between, no arrival
  * counter and no launch per evaluation; step sizes, save slots and keys advance as
- * with sgmc_glm_fused_step, and iter advances by `iterations`. Modes SGLD, SGLD_RMS
- * and SGHMC (AMAGOLD tests every iteration between launches). */
+ * with sgmc_glm_fused_step, and iter advances by `iterations`. Modes SGLD, SGLD_RMS,
+ * SGHMC and OBABO (AMAGOLD and SGGMC test every iteration between launches). */
 int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_dclass,
                         int family, float sigma, int has_prior, float prior_scale,
                         float pot_temperature, int mode, int steps, float friction, float alpha,

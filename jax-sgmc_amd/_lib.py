@@ -28,6 +28,8 @@ MODE_SGLD = 0
 MODE_SGLD_RMS = 1
 MODE_SGHMC = 2
 MODE_AMAGOLD = 3
+MODE_OBABO = 4
+MODE_SGGMC = 5
 
 
 class GlmChain(ctypes.Structure):
@@ -105,6 +107,9 @@ _SIGNATURES = {
                                     c_int, c_void_p]),
     "sgmc_glm_amagold_mh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "sgmc_glm_sggmc_mh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                                  c_void_p]),
     "sgmc_sgld_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int, c_void_p,
                                c_void_p, c_float, c_float, c_float, c_float, c_void_p]),
     "sgmc_sghmc_resample": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,

@@ -12,10 +12,12 @@ and ``fused=`` (None: automatic).
 
 Execution: when the potential is a recognized GLM family with a Gaussian (or zero)
 prior, the data loader is device-resident, and the integrator is SGLD (with or
-without rms_prop) or SGHMC with unit mass and scalar friction, all chains run
-together in the fused multi-chain kernel (``fused.FusedGLM``); the result is the
-same chain by chain. Otherwise the chains run one after another through the
-modular kernels (solver.mcmc with strategy 'map', as in the reference).
+without rms_prop), SGHMC or OBABO with unit mass and scalar friction, all chains run
+together in the fused multi-chain kernel (``fused.FusedGLM``); AMAGOLD and SGGMC do
+too when the full potential is the same family, followed by one full-data pass for
+all chains and the Metropolis-Hastings kernel. The result is the same chain by chain.
+Otherwise the chains run one after another through the modular kernels (solver.mcmc
+with strategy 'map', as in the reference).
 """
 from functools import partial
 
@@ -184,9 +186,9 @@ def _run_fused_resgld(potential_fn, data_loader, batch_size, init_samples, itera
   return out
 
 
-def _amagold_fused_eligible(stochastic_potential_fn, full_potential_fn, data_loader,
-                            init_samples, mass, adaptive, fused):
-  """The fused AMAGOLD engine applies when the stochastic and the full potential are
+def _mh_fused_eligible(stochastic_potential_fn, full_potential_fn, data_loader,
+                       init_samples, mass, adaptive, fused):
+  """The fused AMAGOLD and SGGMC engines apply when the stochastic and the full potential are
   the same GLM family, prior and temperature over a DeviceNumpyDataLoader, the mass is
   the identity and the step-size schedule is fixed in advance."""
   if fused is False:
@@ -205,14 +207,15 @@ def _amagold_fused_eligible(stochastic_potential_fn, full_potential_fn, data_loa
   return ok
 
 
-def _run_fused_amagold(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
-                       steps, friction, keys, data_keys, to_numpy=True):
-  """alias.amagold on the fused engine: every chain's reversible-leapfrog iteration
-  runs in ``steps`` fused launches, then one pass over the data gives every proposal's
-  full potential and the MH kernel accepts or rejects each chain (solver.py:301-432)."""
+def _run_fused_mh(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                  mode, steps, friction, keys, data_keys, to_numpy=True):
+  """alias.amagold / alias.sggmc on the fused engine: every chain's proposal
+  (reversible leapfrog in ``steps`` launches, or OBABO in 2 ``steps``) runs for all
+  chains at once, then one pass over the data gives every proposal's full potential
+  and the MH kernel accepts or rejects each chain (solver.py:301-432, 434-567)."""
   init_fn = sched[0]
   st, _ = init_fn(iterations)
-  eps, _, keep = init_fn.tables(st)
+  eps, temp, keep = init_fn.tables(st)
   slots = np.full(iterations, -1, np.int32)
   kept = np.nonzero(keep)[0]
   slots[kept] = np.arange(kept.size, dtype=np.int32)
@@ -226,14 +229,15 @@ def _run_fused_amagold(potential_fn, data_loader, batch_size, init_samples, iter
     th0 = leaf.detach().cpu().numpy() if isinstance(leaf, torch.Tensor) else np.asarray(leaf)
     chains.append(dict(X=X, y=y, D=X.shape[1], n=batch_size,
                        theta0=np.asarray(th0, np.float32).reshape(-1),
-                       dkey=_key_words(data_keys[c] if data_keys else None)))
+                       dkey=_key_words(data_keys[c] if data_keys else None),
+                       temperature=float(temp[0]) if temp.size else 1.0))
   from . import fused as FU
-  eng = FU.FusedGLM(chains, lik.family, "amagold", eps, steps=steps, friction=friction,
+  eng = FU.FusedGLM(chains, lik.family, mode, eps, steps=steps, friction=friction,
                     sigma=lik.sigma, prior_scale=pscale, has_prior=bool(has_prior),
                     pot_temperature=potential_fn.temperature, save_slots=slots,
                     n_slots=max(1, kept.size))
-  mh = FU.AmagoldMH(eng, keys=[_key_words(keys[c]) for c in range(len(chains))]
-                    if keys else None)
+  test = FU.AmagoldMH if mode == "amagold" else FU.SggmcMH
+  mh = test(eng, keys=[_key_words(keys[c]) for c in range(len(chains))] if keys else None)
   eng.run(iterations)
   torch.cuda.synchronize()
   out = []
@@ -243,6 +247,9 @@ def _run_fused_amagold(potential_fn, data_loader, batch_size, init_samples, iter
                "energy": eng.sample_potentials(c)[:kept.size].clone(),
                "acceptance_ratio": mh.acceptance_ratios(c)[:kept.size].clone(),
                "step_size": torch.as_tensor(eps[kept]), "model_state": None}
+    if mode == "sggmc":
+      samples["kinetic_energy"] = mh.kinetic_energies(c)[:kept.size].clone()
+      samples["potential"] = mh.full_potentials(c)[:kept.size].clone()
     if to_numpy:
       samples = T.tree_map(lambda x: x.cpu().numpy() if isinstance(x, torch.Tensor) else x,
                            samples)
@@ -424,9 +431,17 @@ def obabo(potential_fn, data_loader, cache_size=512, batch_size=32, integration_
                             sched, "obabo", keys, data_keys, to_numpy=save_to_numpy,
                             steps=integration_steps, friction=float(friction),
                             result_key="energy", precision=precision)
+    if (init_model_state is None and mass is None and np.ndim(friction) == 0
+        and not isinstance(friction, (dict, list, tuple))
+        and _fused_eligible(potential_fn, data_loader, init_samples,
+                            None if fused is None else bool(fused))):
+      return _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
+                        "obabo", keys, data_keys, steps=integration_steps,
+                        friction=float(friction), to_numpy=save_to_numpy,
+                        result_key="energy")
     if fused:
-      raise ValueError("fused=True needs an MLPSoftmax potential over a DeviceNumpyDataLoader "
-                       "with identity mass")
+      raise ValueError("fused=True needs a GLM family or MLPSoftmax potential over a "
+                       "DeviceNumpyDataLoader with identity mass and scalar friction")
     states = []
     for c, s in enumerate(init_samples):
       kw = dict(init_model_state=init_model_state)
@@ -490,8 +505,8 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
   saving = IO.save(to_numpy=True) if save_to_numpy else None
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
-  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, fused=None,
-             precision="bf16"):
+  def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, data_keys=None,
+             fused=None, precision="bf16"):
     from . import mlp as ML
     if (fused and not adaptive_step_size and init_model_state is None and mass is None and
         isinstance(getattr(full_potential_fn, "likelihood", None), ML.MLPSoftmax) and
@@ -500,9 +515,19 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
                             iterations, sched, "sggmc", keys, None, to_numpy=save_to_numpy,
                             steps=integration_steps, friction=float(friction_coefficient),
                             result_key="energy", precision=precision)
+    if (init_model_state is None and np.ndim(friction_coefficient) == 0 and _mh_fused_eligible(
+        stochastic_potential_fn, full_potential_fn, data_loader, init_samples, mass,
+        adaptive_step_size, None if fused is None else bool(fused))):
+      return _run_fused_mh(stochastic_potential_fn, data_loader, batch_size, init_samples,
+                           iterations, sched, "sggmc", integration_steps,
+                           float(friction_coefficient), keys, data_keys,
+                           to_numpy=save_to_numpy)
     if fused:
-      raise ValueError("fused=True needs MLPSoftmax potentials over a DeviceNumpyDataLoader "
-                       "with identity mass and a fixed step-size schedule")
+      raise ValueError("fused=True needs matching GLM or MLPSoftmax potentials over a "
+                       "DeviceNumpyDataLoader with identity mass and a fixed step-size "
+                       "schedule")
+    if data_keys:
+      raise ValueError("data_keys is supported by the fused path only")
     states = []
     for c, s in enumerate(init_samples):
       kw = dict(init_model_state=init_model_state)
@@ -535,12 +560,12 @@ def amagold(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, data_keys=None,
              fused=None):
-    if init_model_state is None and _amagold_fused_eligible(
+    if init_model_state is None and _mh_fused_eligible(
         stochastic_potential_fn, full_potential_fn, data_loader, init_samples, mass,
         adaptive_step_size, fused):
-      return _run_fused_amagold(stochastic_potential_fn, data_loader, batch_size, init_samples,
-                                iterations, sched, integration_steps, float(friction), keys,
-                                data_keys, to_numpy=save_to_numpy)
+      return _run_fused_mh(stochastic_potential_fn, data_loader, batch_size, init_samples,
+                           iterations, sched, "amagold", integration_steps, float(friction),
+                           keys, data_keys, to_numpy=save_to_numpy)
     if data_keys:
       raise ValueError("data_keys is supported by the fused path only")
     states = []

@@ -576,8 +576,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   const Key key0 = load_key(ch.key);
   const bool hmc = a.mode == SGMC_MODE_SGHMC;
   const bool amg = a.mode == SGMC_MODE_AMAGOLD;
-  const bool has_mom = hmc || amg;
+  const bool ob_mh = a.mode == SGMC_MODE_SGGMC;
+  const bool ob = a.mode == SGMC_MODE_OBABO || ob_mh;
+  const bool has_mom = hmc || amg || ob;
+  const bool shift = hmc || amg;  // evaluation point theta + scale M^-1 p
   const bool resample = hmc && a.step == 0;
+  // OBABO: two launches per leapfrog step, the even one O B A, the odd one B O.
+  const int sub = a.step & 1;
+  const int launches = ob ? 2 * a.steps : a.steps;
   // Position step before this evaluation: eps, or eps/2 at the first step of an
   // AMAGOLD iteration (integrator.py:521-523, 401-408).
   const float half = 0.5f * eps;
@@ -606,7 +612,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         float v = 0.0f;
         if (c < D) {
           v = g_theta[c];
-          if (has_mom) {
+          if (shift) {
             float p;
             if (resample) {
               const float xi = leaf_normal(resample_leaf, (uint32_t)D, (uint32_t)c);
@@ -688,14 +694,31 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     if (ch.var) rec_m2 = consume(g_partial + (int64_t)threadIdx.x * stride + 4 * D4 + 1);
   }
 
-  // Noise key of this step.
-  Key knext, ksub;
-  split2(key_noise, knext, ksub);
-  const Key leaf = split_key(ksub, 1, 0);
-  // AMAGOLD samples are saved after the Metropolis-Hastings test, not here.
-  const int slot = (a.save_slot && !amg) ? a.save_slot[it] : -1;
-  const bool last_step = !has_mom || a.step == a.steps - 1;
+  // Noise key of this step. OBABO: (key', s1, s2) = split(key, 3) at both launches of
+  // a leapfrog step; the even one draws from s1, the odd one from s2 and stores key'.
+  Key knext, leaf;
+  if (ob) {
+    Key k0, s1, s2;
+    split3(key0, k0, s1, s2);
+    leaf = split_key(sub ? s2 : s1, 1, 0);
+    knext = sub ? k0 : key0;
+  } else {
+    Key ksub;
+    split2(key_noise, knext, ksub);
+    leaf = split_key(ksub, 1, 0);
+  }
+  // AMAGOLD and SGGMC samples are saved after the Metropolis-Hastings test, not here.
+  const int slot = (a.save_slot && !amg && !ob_mh) ? a.save_slot[it] : -1;
+  const bool last_step = !has_mom || a.step == launches - 1;
   const float Tn = ch.temperature;
+  // obabo O-step coefficients (integrator.py:192-200): sqrt(exp(-f eps)) and
+  // sqrt((1 - exp(-f eps)) T), float32.
+  float ob_decay = 0.0f, ob_noise = 0.0f;
+  if (ob) {
+    const float perm = expf((-a.friction) * eps);
+    ob_noise = sqrtf((1.0f - perm) * Tn);
+    ob_decay = sqrtf(perm);
+  }
   const bool rms = a.mode == SGMC_MODE_SGLD_RMS;
   // reversible_leapfrog momentum coefficients (integrator.py:420-437), float32.
   const float ef = eps * a.friction;
@@ -725,7 +748,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
       wv[h] = g_theta[j];
       pv[h] = (has_mom && !resample) ? g_mom[j] : 0.0f;
       imv[h] = (has_mom && a.inv_mass) ? g_inv_mass[j] : 1.0f;
-      smv[h] = ((resample || amg) && a.sqrt_mass) ? g_sqrt_mass[j] : 1.0f;
+      smv[h] = ((resample || amg || ob) && a.sqrt_mass) ? g_sqrt_mass[j] : 1.0f;
       vv[h] = rms ? g_v[j] : 0.0f;
     }
 #pragma unroll
@@ -743,7 +766,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
           p = pv[h];
         }
         sm = a.inv_mass ? imv[h] * p : p;
-        wj = wj + pos_scale * sm;  // evaluation point
+        if (shift) wj = wj + pos_scale * sm;  // evaluation point
       }
       float g = c * s[h];
       if (a.pr.has_prior) {
@@ -770,6 +793,26 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
         pn = pn + C * (sqrtf(2.0f * eps) * xi);
         g_mom[j] = pn;
         wn = wj;
+      } else if (ob) {
+        if (sub == 0) {
+          // O: p1 = decay p + noise (M^{1/2} xi); B: p2 = (-eps/2) g + p1;
+          // A: theta += eps M^-1 p2 (integrator.py:214-238).
+          const float p1 = ob_decay * p + ob_noise * (smv[h] * xi);
+          edot += p1 * (imv[h] * p1);
+          const float p2 = (-half) * g + p1;
+          if (ob_mh && a.step == 0) {
+            g_theta_old[j] = wv[h];
+            g_mom_old[j] = p;
+          }
+          g_mom[j] = p2;
+          wn = wj + eps * (imv[h] * p2);
+        } else {
+          // B: p3 = (-eps/2) g + p2; O with the second noise (integrator.py:244-262).
+          const float p3 = (-half) * g + p;
+          edot += p3 * (imv[h] * p3);
+          g_mom[j] = ob_decay * p3 + ob_noise * (smv[h] * xi);
+          wn = wj;
+        }
       } else {
         // p' = (((1 - eps f) p + (-eps) g) + sqrt(4 f eps) (M^{1/2} xi)) / (1 + eps f);
         // energy term (p + p') . (M^-1 g); final half position step after the last.
@@ -810,12 +853,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     var_sum = block_sum<kNT>(t, lds_small);
   }
   lp = block_sum<kNT>(lp, lds_small);
-  if (amg) edot = block_sum<kNT>(edot, lds_small);
+  if (amg || ob) edot = block_sum<kNT>(edot, lds_small);
   if (threadIdx.x == 0) {
     if (amg) ch.energy[0] = half * edot + (a.step == 0 ? 0.0f : ch.energy[0]);
+    if (ob) ch.energy[sub] = ch.energy[sub] + 0.5f * edot;
     const float Uc = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
-    ch.U[0] = Uc;
-    if (slot >= 0 && last_step && ch.samples_U) ch.samples_U[slot] = Uc;
+    // OBABO's potential is the mean of the step's two evaluations.
+    float Uk = Uc;
+    if (ob && sub) Uk = 0.5f * (ch.U[0] + Uc);
+    if (ob_mh && a.step == 0) ch.energy[2] = ch.U[0];
+    ch.U[0] = Uk;
+    if (slot >= 0 && last_step && ch.samples_U) ch.samples_U[slot] = Uk;
     if (ch.var) ch.var[0] = var_sum / (float)ch.n;
     store_key(ch.key, knext);
     store_key(ch.dkey, dnext);
@@ -848,9 +896,10 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   FusedArgs b = a;
   b.advance = -1;
+  const int launches = a.mode == SGMC_MODE_OBABO ? 2 * a.steps : a.steps;
   for (int t = 0; t < iterations; ++t) {
     b.iter_offset = t;
-    for (int st = 0; st < a.steps; ++st) {
+    for (int st = 0; st < launches; ++st) {
       b.step = st;
       switch (dc) {
         case 0: fused_body<4, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
@@ -1045,10 +1094,13 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
                "sgmc_glm_fused_step: unknown family %d", family);
   SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS ||
-               mode == SGMC_MODE_SGHMC || mode == SGMC_MODE_AMAGOLD,
+               mode == SGMC_MODE_SGHMC || mode == SGMC_MODE_AMAGOLD ||
+               mode == SGMC_MODE_OBABO || mode == SGMC_MODE_SGGMC,
                SGMC_E_ARG, "sgmc_glm_fused_step: unknown mode %d", mode);
-  SGMC_REQUIRE(steps >= 1 && step >= 0 && step < steps, SGMC_E_ARG,
-               "sgmc_glm_fused_step: step %d of %d", step, steps);
+  const int launches =
+      (mode == SGMC_MODE_OBABO || mode == SGMC_MODE_SGGMC) ? 2 * steps : steps;
+  SGMC_REQUIRE(steps >= 1 && step >= 0 && step < launches, SGMC_E_ARG,
+               "sgmc_glm_fused_step: launch %d of %d", step, launches);
   SGMC_REQUIRE(pot_temperature != 0.0f, SGMC_E_ARG, "sgmc_glm_fused_step: temperature is zero");
   FusedArgs a;
   a.chains = chains;
@@ -1099,7 +1151,8 @@ int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_d
                "sgmc_glm_fused_loop: bad sizes");
   SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
                "sgmc_glm_fused_loop: unknown family %d", family);
-  SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS || mode == SGMC_MODE_SGHMC,
+  SGMC_REQUIRE(mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS ||
+                   mode == SGMC_MODE_SGHMC || mode == SGMC_MODE_OBABO,
                SGMC_E_ARG, "sgmc_glm_fused_loop: mode %d needs a test between iterations",
                mode);
   SGMC_REQUIRE(steps >= 1, SGMC_E_ARG, "sgmc_glm_fused_loop: steps %d", steps);
@@ -1119,7 +1172,7 @@ int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_d
   a.nchains = nchains;
   a.mode = mode;
   a.step = 0;
-  a.steps = mode == SGMC_MODE_SGHMC ? steps : 1;
+  a.steps = (mode == SGMC_MODE_SGHMC || mode == SGMC_MODE_OBABO) ? steps : 1;
   a.friction = friction;
   a.four_friction = (float)(4.0 * (double)friction);
   a.alpha = alpha;

@@ -22,8 +22,13 @@ import torch
 from . import _lib
 
 MODES = {"sgld": _lib.MODE_SGLD, "sgld_rms": _lib.MODE_SGLD_RMS, "sghmc": _lib.MODE_SGHMC,
-         "amagold": _lib.MODE_AMAGOLD}
-_STEPPED = (_lib.MODE_SGHMC, _lib.MODE_AMAGOLD)
+         "amagold": _lib.MODE_AMAGOLD, "obabo": _lib.MODE_OBABO, "sggmc": _lib.MODE_SGGMC}
+_STEPPED = (_lib.MODE_SGHMC, _lib.MODE_AMAGOLD, _lib.MODE_OBABO, _lib.MODE_SGGMC)
+_OBABO = (_lib.MODE_OBABO, _lib.MODE_SGGMC)
+# floats of per-chain energy bookkeeping: AMAGOLD's energy; OBABO's kinetic energies at
+# the start and the end; SGGMC's as OBABO plus the minibatch potential of the start
+_ENERGY = {_lib.MODE_AMAGOLD: 1, _lib.MODE_OBABO: 2, _lib.MODE_SGGMC: 3}
+_KEEPS_START = (_lib.MODE_AMAGOLD, _lib.MODE_SGGMC)
 FAMILIES = {"linear_gaussian": _lib.LIK_LINEAR_GAUSSIAN, "logistic": _lib.LIK_LOGISTIC}
 NT = 256
 
@@ -60,6 +65,8 @@ class FusedGLM:
     self.family = FAMILIES[family] if isinstance(family, str) else int(family)
     self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
     self.steps = int(steps) if self.mode in _STEPPED else 1
+    # OBABO modes take two launches per leapfrog step (O B A, then B O)
+    self.launches = 2 * self.steps if self.mode in _OBABO else self.steps
     self.friction = float(friction)
     self.alpha, self.lmbd = float(alpha), float(lmbd)
     self.sigma, self.prior_scale = float(sigma), float(prior_scale)
@@ -121,10 +128,12 @@ class FusedGLM:
     self.theta_flat = torch.zeros(P, device=dev)
     self.mom_flat = torch.zeros(P, device=dev) if self.mode in _STEPPED else None
     self.v_flat = torch.ones(P, device=dev) if self.mode == _lib.MODE_SGLD_RMS else None
-    amg = self.mode == _lib.MODE_AMAGOLD
-    self.energy = torch.zeros(self.C, device=dev) if amg else None
-    self.theta_old_flat = torch.zeros(P, device=dev) if amg else None
-    self.mom_old_flat = torch.zeros(P, device=dev) if amg else None
+    self.energy_width = _ENERGY.get(self.mode, 0)
+    keeps = self.mode in _KEEPS_START
+    self.energy = (torch.zeros(self.C * self.energy_width, device=dev)
+                   if self.energy_width else None)
+    self.theta_old_flat = torch.zeros(P, device=dev) if keeps else None
+    self.mom_old_flat = torch.zeros(P, device=dev) if keeps else None
     for i, c in enumerate(ch):
       self.theta_flat[self.poff[i]:self.poff[i] + self.D[i]] = torch.as_tensor(
           np.asarray(c["theta0"], np.float32).reshape(-1), device=dev)
@@ -189,7 +198,8 @@ class FusedGLM:
       e.dclass = _lib.query("sgmc_glm_dclass", int(self.D[i]))
       e.temperature = float(c.get("temperature", 1.0))
       if self.energy is not None:
-        e.energy = self.energy.data_ptr() + fs * i
+        e.energy = self.energy.data_ptr() + fs * self.energy_width * i
+      if self.theta_old_flat is not None:
         e.theta_old = self.theta_old_flat.data_ptr() + fs * int(self.poff[i])
         e.mom_old = self.mom_old_flat.data_ptr() + fs * int(self.poff[i])
     raw = bytes(arr)
@@ -214,7 +224,7 @@ class FusedGLM:
               _lib.stream())
 
   def _launch_steps(self):
-    for s in range(self.steps):
+    for s in range(self.launches):
       self._launch(s)
 
   def loop_capable(self):
@@ -222,7 +232,7 @@ class FusedGLM:
     run in one launch (``sgmc_glm_fused_loop``)."""
     return (bool(np.all(self.nblocks_chain == 1)) and self.exchange is None and
             self.mh is None and self.mode in (_lib.MODE_SGLD, _lib.MODE_SGLD_RMS,
-                                              _lib.MODE_SGHMC))
+                                              _lib.MODE_SGHMC, _lib.MODE_OBABO))
 
   def _launch_loop(self, iterations):
     _lib.call("sgmc_glm_fused_loop", _lib.ptr(self.table), self.C, self.uniform_dclass,
@@ -233,8 +243,9 @@ class FusedGLM:
               _lib.ptr(self.done), int(iterations), _lib.stream())
 
   def iteration(self):
-    """Advances every chain by one reference iteration (steps launches), followed by
-    the replica-exchange step when one is attached."""
+    """Advances every chain by one reference iteration (steps launches, two per step in
+    the OBABO modes), followed by the replica-exchange step or the Metropolis-Hastings
+    test when one is attached."""
     self._launch_steps()
     if self.exchange is not None:
       self.exchange.launch()
@@ -293,6 +304,11 @@ class FusedGLM:
 
   def momentum(self, i):
     return self.mom_flat[self.poff[i]:self.poff[i] + self.D[i]]
+
+  def kinetic_energies(self, i):
+    """OBABO modes: chain i's (kinetic energy start, kinetic energy end) accumulators."""
+    w = self.energy_width
+    return self.energy[i * w:i * w + 2]
 
   def sample_trace(self, i):
     """[n_slots, D] samples saved for chain i."""
@@ -440,27 +456,28 @@ class ReplicaExchange:
     return self.samples_U[k * self.n_slots:(k + 1) * self.n_slots]
 
 
-class AmagoldMH:
-  """AMAGOLD's amortized Metropolis-Hastings test (solver.py:301-432) for a FusedGLM in
-  mode "amagold" whose chains share one dataset: after every iteration the full-data
-  potential of all proposals runs as one pass over X (``sgmc_glm_full_potential_chains``,
-  FP64 sums) and ``sgmc_glm_amagold_mh`` accepts or rejects each chain.
+class _SharedDataTest:
+  """Full-data Metropolis-Hastings test after every iteration of a FusedGLM whose
+  chains share one dataset: the full potential of all positions runs as one pass over
+  X (``sgmc_glm_full_potential_chains``, FP64 sums).
 
-  keys: one (solver key, integrator key) pair per chain as the reference's init derives
-  them (solver.py:340-352): the integrator key draws the initial momentum
-  sqrt_mass * xi (integrator.py:488-492) and the MH key starts the test stream.
-  """
+  keys: one key (two uint32 words) per chain, split as the reference's MH solvers do
+  at init (solver.py:340-352, 489-490): split(key)[0] starts the integrator,
+  split(key)[1] the test."""
+
+  mode = None
 
   def __init__(self, engine, keys=None, nblocks=None):
-    if engine.mode != _lib.MODE_AMAGOLD:
-      raise ValueError("AmagoldMH needs a FusedGLM in mode 'amagold'")
+    if engine.mode != self.mode:
+      raise ValueError(f"{type(self).__name__} needs a FusedGLM in mode "
+                       f"{[k for k, v in MODES.items() if v == self.mode][0]!r}")
     spec = engine.spec
     c0 = spec[0]
     for c in spec:
       if (c["X"].data_ptr() != c0["X"].data_ptr() or c["y"].data_ptr() != c0["y"].data_ptr()
           or int(c["D"]) != int(c0["D"]) or int(c.get("N", c["X"].shape[0])) !=
           int(c0.get("N", c0["X"].shape[0]))):
-        raise ValueError("AmagoldMH needs chains that share one dataset")
+        raise ValueError(f"{type(self).__name__} needs chains that share one dataset")
     from . import random as R
     dev = engine.device
     self.engine = engine
@@ -474,31 +491,19 @@ class AmagoldMH:
                                          self.nblocks)), dtype=torch.float64, device=dev)
     self.U_full = torch.zeros(C, dtype=torch.float64, device=dev)
     self.U_prop = torch.zeros(C, dtype=torch.float64, device=dev)
-    self.energy_kept = torch.zeros(C, device=dev)
     self.n_slots = engine.n_slots
     self.ratio_samples = torch.zeros(max(1, C * self.n_slots), device=dev)
-    # init (solver.py:340-352, integrator.py:484-492): (k_int, k_mh) = split(key);
-    # (k_int', sub) = split(k_int); p = sqrt_mass * random_tree(sub, theta0).
-    mh_words = np.zeros((C, 2), np.uint32)
-    int_words = np.zeros((C, 2), np.uint32)
-    lay_table = None
+    self.int_keys, mh_words = [], np.zeros((C, 2), np.uint32)
     for c in range(C):
       kw = (0, 0) if keys is None else tuple(int(v) for v in keys[c])
       k = R.split(R.key_from_words(kw, device=dev))
       mh_words[c] = R.key_words(k[1])
-      k_out = torch.empty(2, dtype=torch.int32, device=dev)
-      if lay_table is None:
-        from . import tree as T
-        lay_table = T.Layout([torch.zeros(self.D)]).leaf_table(dev, _lib.PAIRS_PER_BLOCK)
-      table, nb = lay_table
-      mom = engine.momentum(c)
-      _lib.call("sgmc_sghmc_resample", _lib.ptr(mom), _lib.ptr(engine.sqrt_mass),
-                _lib.ptr(table), nb, _lib.ptr(k[0].clone()), _lib.ptr(k_out), _lib.stream())
-      int_words[c] = R.key_words(k_out)
-    engine.keys.copy_(torch.as_tensor(int_words.view(np.int32).copy(), device=dev))
+      self.int_keys.append(k[0].clone())
     self.keys = torch.as_tensor(mh_words.view(np.int32).copy(), device=dev)
-    self._full(self.U_full)
-    engine.mh = self
+
+  def _set_integrator_keys(self, words):
+    self.engine.keys.copy_(torch.as_tensor(np.asarray(words, np.uint32).view(np.int32).copy(),
+                                           device=self.engine.device))
 
   def _full(self, out):
     eng = self.engine
@@ -508,6 +513,38 @@ class AmagoldMH:
               eng.pot_temperature, _lib.ptr(out), _lib.ptr(self.ws), self.ws.numel(),
               self.nblocks, _lib.stream())
 
+  def acceptance_ratios(self, i):
+    """[n_slots] exp(log a) of the test at each saved iteration of chain i."""
+    return self.ratio_samples[i * self.n_slots:(i + 1) * self.n_slots]
+
+
+class AmagoldMH(_SharedDataTest):
+  """AMAGOLD's amortized Metropolis-Hastings test (solver.py:301-432) for a FusedGLM in
+  mode "amagold" whose chains share one dataset, launched after every iteration
+  (``sgmc_glm_amagold_mh``). The integrator key draws the initial momentum
+  sqrt_mass * xi (integrator.py:488-492)."""
+
+  mode = _lib.MODE_AMAGOLD
+
+  def __init__(self, engine, keys=None, nblocks=None):
+    super().__init__(engine, keys, nblocks)
+    from . import random as R
+    from . import tree as T
+    dev = engine.device
+    self.energy_kept = torch.zeros(engine.C, device=dev)
+    # (k_int', sub) = split(k_int); p = sqrt_mass * random_tree(sub, theta0)
+    table, nb = T.Layout([torch.zeros(self.D)]).leaf_table(dev, _lib.PAIRS_PER_BLOCK)
+    int_words = np.zeros((engine.C, 2), np.uint32)
+    for c, k in enumerate(self.int_keys):
+      k_out = torch.empty(2, dtype=torch.int32, device=dev)
+      _lib.call("sgmc_sghmc_resample", _lib.ptr(engine.momentum(c)),
+                _lib.ptr(engine.sqrt_mass), _lib.ptr(table), nb, _lib.ptr(k), _lib.ptr(k_out),
+                _lib.stream())
+      int_words[c] = R.key_words(k_out)
+    self._set_integrator_keys(int_words)
+    self._full(self.U_full)
+    engine.mh = self
+
   def launch(self):
     eng = self.engine
     self._full(self.U_prop)
@@ -516,6 +553,39 @@ class AmagoldMH:
               _lib.ptr(self.ratio_samples), self.n_slots, _lib.ptr(eng.save_slot),
               eng.table_len, _lib.ptr(eng.iter), _lib.stream())
 
-  def acceptance_ratios(self, i):
-    """[n_slots] exp(log a) of the test at each saved iteration of chain i."""
-    return self.ratio_samples[i * self.n_slots:(i + 1) * self.n_slots]
+
+class SggmcMH(_SharedDataTest):
+  """SGGMC's Metropolis-Hastings test (solver.py:434-567) for a FusedGLM in mode
+  "sggmc" whose chains share one dataset: OBABO proposals from a zero initial momentum
+  (integrator.py:303), the full potential of every proposal, then
+  ``sgmc_glm_sggmc_mh``. Besides the acceptance ratios it keeps, per saved iteration,
+  the proposal's kinetic-energy difference and the kept full potential."""
+
+  mode = _lib.MODE_SGGMC
+
+  def __init__(self, engine, keys=None, nblocks=None):
+    super().__init__(engine, keys, nblocks)
+    from . import random as R
+    dev = engine.device
+    self._set_integrator_keys([R.key_words(k) for k in self.int_keys])
+    C = engine.C
+    self.ke_samples = torch.zeros(max(1, C * self.n_slots), device=dev)
+    self.U_samples = torch.zeros(max(1, C * self.n_slots), dtype=torch.float64, device=dev)
+    self._full(self.U_full)
+    engine.mh = self
+
+  def launch(self):
+    eng = self.engine
+    self._full(self.U_prop)
+    _lib.call("sgmc_glm_sggmc_mh", _lib.ptr(eng.table), eng.C, _lib.ptr(self.U_prop),
+              _lib.ptr(self.U_full), _lib.ptr(self.keys), _lib.ptr(self.ratio_samples),
+              _lib.ptr(self.ke_samples), _lib.ptr(self.U_samples), self.n_slots,
+              _lib.ptr(eng.save_slot), eng.table_len, _lib.ptr(eng.iter), _lib.stream())
+
+  def kinetic_energies(self, i):
+    """[n_slots] KE_end - KE_start of the proposal tested at each saved iteration."""
+    return self.ke_samples[i * self.n_slots:(i + 1) * self.n_slots]
+
+  def full_potentials(self, i):
+    """[n_slots] full potential kept after the test at each saved iteration (FP64)."""
+    return self.U_samples[i * self.n_slots:(i + 1) * self.n_slots]

@@ -245,7 +245,8 @@ def sggmc(pot, full_pot, theta0, iterations, eps, steps, friction, N, n, leaf_si
         leaf_sizes, ones, ones)
     U_new = full_pot(th_new)
     log_alpha = -1.0 / float(temperature) * (U_new - U_full + ke - ks)
-    log_alpha = min(log_alpha, 0.0)
+    # jnp.where(log_alpha <= 0, log_alpha, 0.0) (solver.py:526): NaN becomes 0
+    log_alpha = log_alpha if log_alpha <= 0.0 else 0.0
     u = jr.uniform(sub)
     acc = float(np.log(np.float64(u))) < log_alpha
     if acc:

@@ -139,8 +139,9 @@ def test_sghmc_logistic_matches_oracle(fused):
     _close(res[c]["samples"]["energy"], r["U"], f"chain {c} energy")
 
 
-def test_obabo_matches_oracle():
-  print("obabo parity", flush=True)
+@pytest.mark.parametrize("fused", [True, False])
+def test_obabo_matches_oracle(fused):
+  print("obabo parity fused", fused, flush=True)
   X, y = logreg(N=3000, D=24)
   loader = DT.DeviceNumpyDataLoader(x=X, y=y)
   pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
@@ -148,7 +149,7 @@ def test_obabo_matches_oracle():
   run = alias.obabo(pot, loader, cache_size=1, batch_size=128, integration_steps=3,
                     friction=2.0, first_step_size=5e-3, last_step_size=1e-3,
                     accepted_samples=its)
-  res = run(torch.zeros(24), iterations=its)[0]
+  res = run(torch.zeros(24), iterations=its, fused=fused)[0]
   eps = SC.polynomial_table(5e-3, 1e-3, its)
   fam = om.Family("logistic", X, y)
   r = ost.obabo(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0), np.zeros(24), its,

@@ -130,6 +130,25 @@ def test_amagold_rejection_flips_momentum_and_keeps_theta():
       np.testing.assert_array_equal(r["p"][t], -r["p"][t - 1])
 
 
+def test_nan_log_ratio_follows_each_solver():
+  """A NaN full potential at the proposal: sggmc's jnp.where(log_alpha <= 0, log_alpha,
+  0.0) (solver.py:526) turns it into log a = 0 and accepts; amagold's
+  jnp.where(log_alpha > 0.0, 0.0, log_alpha) (solver.py:386) keeps NaN and rejects."""
+  X, y, fam = _linreg()
+  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
+  calls = []
+
+  def full(th):
+    calls.append(1)
+    return om.full_potential(fam, th, 1.0, 60) if len(calls) == 1 else float("nan")
+
+  r = ost.sggmc(pot, full, np.zeros(3), 4, np.full(4, 0.01, np.float32), 2, 1.0, 60, 8, [3])
+  assert r["accept"].all() and np.all(r["log_alpha"] == 0.0)
+  calls.clear()
+  r = ost.amagold(pot, full, np.zeros(3), 4, 0.01, 2, 0.25, 60, 8, [3])
+  assert not r["accept"].any() and np.all(np.isnan(r["log_alpha"]))
+
+
 def test_dual_averaging_fixed_point_and_burn_in():
   eps0 = 0.05
   steps = ost.adaptive_step_size(np.full(200, 0.25, np.float32), 100, eps0, 10, 0.75, 0.05,
