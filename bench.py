@@ -447,14 +447,18 @@ def run_sgld_linreg(args):
   print(json.dumps(out), flush=True)
 
 
-AMG_STEPS, AMG_FRICTION, AMG_EPS = 5, 0.25, (1e-4, 5e-5)
+# Full-data MH workloads on the headline's data: leapfrog steps per test, friction and
+# the polynomial step-size range.
+MH_SETTINGS = {"amagold": (5, 0.25, (1e-4, 5e-5)), "sggmc": (5, 1.0, (1e-4, 5e-5))}
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
-def amagold_cpu_baseline(X_host, y_host, theta0, keys, dkeys, eps, target_s=12.0):
-  """The oracle's C port of the same AMAGOLD loop (reversible leapfrog, FP64 full
+def mh_cpu_baseline(kind, X_host, y_host, theta0, keys, dkeys, eps, target_s=12.0):
+  """The oracle's C port of the same MH loop (reversible leapfrog or OBABO, FP64 full
   potential over all rows, MH), OpenMP over chains."""
   from oracle import cport
+  from oracle import jaxrandom as jr
+  steps, fric, _ = MH_SETTINGS[kind]
   threads = max(1, min(16, os.cpu_count() or 1))
   C = min(CHAINS, threads)
   Xh = np.ascontiguousarray(X_host, np.float32)
@@ -466,52 +470,59 @@ def amagold_cpu_baseline(X_host, y_host, theta0, keys, dkeys, eps, target_s=12.0
     kk = np.zeros((C, 2), np.uint32)
     mk = np.zeros((C, 2), np.uint32)
     for c in range(C):
-      p0, ki, km = cport.amagold_init(th[c], np.asarray(keys[c], np.uint32))
-      mom[c], kk[c], mk[c] = p0, ki, km
+      if kind == "amagold":
+        mom[c], kk[c], mk[c] = cport.amagold_init(th[c], np.asarray(keys[c], np.uint32))
+      else:
+        kk[c], mk[c] = jr.split(np.asarray(keys[c], np.uint32))
     dd = np.array(dkeys[:C], np.uint32).copy()
     U = np.full(C, cport.full_potential(1, Xh, yh, th[0]))
+    run = cport.amagold if kind == "amagold" else cport.sggmc
     t0 = time.perf_counter()
-    cport.amagold(1, Xh, yh, BATCH, th, mom, kk, mk, dd, U, eps[:iters], AMG_STEPS,
-                  AMG_FRICTION, threads=threads)
+    run(1, Xh, yh, BATCH, th, mom, kk, mk, dd, U, eps[:iters], steps, fric, threads=threads)
     return time.perf_counter() - t0
 
   t1 = go(1)
   iters = int(max(1, min(500, target_s / max(t1, 1e-3))))
   dt = go(iters)
+  name = "AMAGOLD" if kind == "amagold" else "SGGMC"
   return {"value": C * iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
           "sample": f"{C} chains x {iters} iterations of the same workload (oracle/cport.c "
-                    f"AMAGOLD logistic with its FP64 full-data potential, OpenMP over chains, "
+                    f"{name} logistic with its FP64 full-data potential, OpenMP over chains, "
                     f"{dt:.1f} s)"}
 
 
-def run_amagold(args):
-  """AMAGOLD on the headline's logistic data (N=1e6, D=512, batch 4096), 64 chains, 5
-  reversible-leapfrog steps per amortized MH test: per iteration 5 fused step launches,
-  one FP64 full-data potential of all 64 proposals (fp32 matrix cores, X read once) and
-  the MH kernel, captured in HIP graphs of 10 iterations."""
+def run_mh(args, kind):
+  """AMAGOLD or SGGMC on the headline's logistic data (N=1e6, D=512, batch 4096), 64
+  chains, 5 leapfrog steps (reversible leapfrog, or OBABO with two minibatches per
+  step) per full-data MH test: per iteration 5 (10) fused step launches, one FP64
+  full-data potential of all 64 proposals (fp32 matrix cores, X read once) and the MH
+  kernel, captured in HIP graphs of 10 iterations."""
   import torch
   import __graft_entry__
   __graft_entry__.load()
   from jax_sgmc_amd import _lib as L, fused, random as R, scheduler
   L.require_gpu()
+  steps, fric, eps_range = MH_SETTINGS[kind]
+  name = "AMAGOLD" if kind == "amagold" else "SGGMC"
   log("generating data")
   X, y, w_true = make_data(torch, R)
   torch.cuda.synchronize()
   K, W, G = args.steps, args.warmup, 10
   total = K + W + 2 * G
-  eps = scheduler.polynomial_table(AMG_EPS[0], AMG_EPS[1], total)
+  eps = scheduler.polynomial_table(eps_range[0], eps_range[1], total)
   keys, dkeys = chain_keys(R, CHAINS)
   # Chains start at the data-generating weights, inside the posterior's bulk: from a
   # distant start the stochastic-gradient energy error rejects nearly every proposal.
   theta0 = w_true.cpu().numpy().astype(np.float32)
   chains = [dict(X=X, y=y, D=D, n=BATCH, theta0=theta0, dkey=tuple(dkeys[c]))
             for c in range(CHAINS)]
-  eng = fused.FusedGLM(chains, "logistic", "amagold", eps, steps=AMG_STEPS,
-                       friction=AMG_FRICTION, prior_scale=1.0,
-                       save_slots=np.arange(total, dtype=np.int32), n_slots=total)
+  eng = fused.FusedGLM(chains, "logistic", kind, eps, steps=steps, friction=fric,
+                       prior_scale=1.0, save_slots=np.arange(total, dtype=np.int32),
+                       n_slots=total)
   tuned = eng.autotune(reps=2)
   log(f"autotune: {tuned}")
-  mh = fused.AmagoldMH(eng, keys=[tuple(k) for k in keys])
+  test = fused.AmagoldMH if kind == "amagold" else fused.SggmcMH
+  mh = test(eng, keys=[tuple(k) for k in keys])
   g = eng.capture(G)
   for _ in range(max(1, W // G)):
     g.replay()
@@ -547,17 +558,19 @@ def run_amagold(args):
   full_flops = 2.0 * N_OBS * D * CHAINS
   achieved = full_flops / (full_us / 1e6) / 1e12
   step_bytes = CHAINS * BYTES_PER_CHAIN_GRAD
-  out = {"metric": "chain-steps/sec, AMAGOLD logistic regression (fused, 1 MI355X)",
+  proposal = ("reversible-leapfrog steps" if kind == "amagold" else
+              "OBABO steps (two minibatch gradients each)")
+  out = {"metric": f"chain-steps/sec, {name} logistic regression (fused, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
          "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
          "vs_baseline": None, "dtype": "fp32",
          "data": "synthetic (jax-compatible Threefry draws from PRNGKey(1); random-init chains)",
-         "config": {"workload": "AMAGOLD logistic regression, N=1e6, D=512, batch 4096, "
-                                "64 chains, 5 reversible-leapfrog steps per MH test, "
-                                "friction 0.25, FP64 full-data potential every iteration",
+         "config": {"workload": f"{name} logistic regression, N=1e6, D=512, batch 4096, "
+                                f"64 chains, {steps} {proposal} per MH test, "
+                                f"friction {fric}, FP64 full-data potential every iteration",
                     "chains": CHAINS, "N": N_OBS, "D": D, "batch": BATCH,
-                    "leapfrog_steps": AMG_STEPS, "friction": AMG_FRICTION,
-                    "step_size": f"polynomial {AMG_EPS[0]} -> {AMG_EPS[1]}",
+                    "leapfrog_steps": steps, "step_launches": eng.launches, "friction": fric,
+                    "step_size": f"polynomial {eps_range[0]} -> {eps_range[1]}",
                     "graph_iterations": G, "grid_blocks": eng.nblocks,
                     "target_bytes_per_block": tuned["target_bytes"],
                     "full_potential_blocks": mh.nblocks},
@@ -568,13 +581,14 @@ def run_amagold(args):
                       "kernel": "glm_full_chains_kernel<2, logistic> "
                                 "(sgmc_glm_full_potential_chains)",
                       "flops_per_launch": full_flops, "avg_launch_us": full_us},
-         "step_kernel": {"kernel": "glm_fused_kernel<3> (AMAGOLD mode)", "avg_launch_us": step_us,
+         "step_kernel": {"kernel": f"glm_fused_kernel<3> ({name} mode)",
+                         "avg_launch_us": step_us,
                          "achieved_GBs": step_bytes / (step_us / 1e6) / 1e9,
                          "bytes_per_launch": step_bytes}}
   if not args.no_cpu:
     log("CPU baseline (oracle C port)")
-    out["cpu_baseline"] = amagold_cpu_baseline(X.cpu().numpy(), y.cpu().numpy(), theta0, keys,
-                                               dkeys, eps)
+    out["cpu_baseline"] = mh_cpu_baseline(kind, X.cpu().numpy(), y.cpu().numpy(), theta0, keys,
+                                          dkeys, eps)
     out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
   else:
     out["cpu_baseline"] = None
@@ -593,7 +607,7 @@ def main():
                        "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
   ap.add_argument("--workload", default="sghmc-logistic",
                   choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp", "sggmc-mlp",
-                           "sgld-linreg", "amagold-logistic"])
+                           "sgld-linreg", "amagold-logistic", "sggmc-logistic"])
   ap.add_argument("--chains", type=int, default=0, help="chains of the sggmc-mlp workload")
   ap.add_argument("--precision", choices=["bf16", "fp8"], default=None,
                   help="GEMM operands of the MLP workloads (sggmc-mlp default fp8, psgld-mlp bf16)")
@@ -612,7 +626,9 @@ def main():
   if args.workload == "sgld-linreg":
     return run_sgld_linreg(args)
   if args.workload == "amagold-logistic":
-    return run_amagold(args)
+    return run_mh(args, "amagold")
+  if args.workload == "sggmc-logistic":
+    return run_mh(args, "sggmc")
   import torch
   if args.copy_calibration:
     n = 128 << 20

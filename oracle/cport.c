@@ -404,6 +404,103 @@ int cport_amagold(int family, const float* X, int64_t ldx, const float* y, int64
   return 0;
 }
 
+/* SGGMC (unit mass, solver.py:434-567 over obabo, integrator.py:138-346): per
+ * iteration split the MH key, run `steps` OBABO steps from (theta, p) with both kinetic
+ * energies starting at zero, evaluate the full potential of the proposal;
+ * log a = -(U' - U_full + KE_end - KE_start) / T, replaced by 0 unless <= 0; accept iff
+ * log u < log a, otherwise theta and p keep their values. theta/mom [C][D],
+ * keys/mh_keys/dkeys [C][2] and U_full [C] advance in place; accepts [C][iters]
+ * (nullable) records the decisions. */
+int cport_sggmc(int family, const float* X, int64_t ldx, const float* y, int64_t N, int D,
+                int n, int C, float* theta, float* mom, uint32_t* keys, uint32_t* mh_keys,
+                uint32_t* dkeys, double* U_full, const float* eps, int iters, int steps,
+                float friction, float temperature, float sigma, float prior_scale,
+                int8_t* accepts, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int c = 0; c < C; ++c) {
+    float* w = theta + (int64_t)c * D;
+    float* p = mom + (int64_t)c * D;
+    float* th = (float*)malloc(sizeof(float) * D);
+    float* q = (float*)malloc(sizeof(float) * D);
+    float* g = (float*)malloc(sizeof(float) * D);
+    float* xi = (float*)malloc(sizeof(float) * D);
+    double* acc = (double*)malloc(sizeof(double) * D);
+    int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * n);
+    key_t2 key = {keys[2 * c], keys[2 * c + 1]}, mkey = {mh_keys[2 * c], mh_keys[2 * c + 1]};
+    key_t2 dkey = {dkeys[2 * c], dkeys[2 * c + 1]};
+    for (int t = 0; t < iters; ++t) {
+      key_t2 mnext, msub;
+      split2(mkey, &mnext, &msub);
+      mkey = mnext;
+      const float e = eps[t];
+      const float half = 0.5f * e;
+      const float perm = expf((-friction) * e);
+      const float a_noise = sqrtf((1.0f - perm) * temperature);
+      const float a_decay = sqrtf(perm);
+      memcpy(th, w, sizeof(float) * D);
+      memcpy(q, p, sizeof(float) * D);
+      float ks = 0.0f, ke = 0.0f;
+      for (int s = 0; s < steps; ++s) {
+        const key_t2 k0 = split_key(key, 3, 0), s1 = split_key(key, 3, 1),
+                     s2 = split_key(key, 3, 2);
+        key = k0;
+        leaf_normals(s1, (uint32_t)D, xi);
+        double k = 0.0;
+        for (int j = 0; j < D; ++j) {
+          q[j] = a_decay * q[j] + a_noise * xi[j];
+          k += (double)q[j] * q[j];
+        }
+        ks = ks + 0.5f * (float)k;
+        draw_indices(&dkey, (uint32_t)n, (uint32_t)N, idx);
+        potential(family, X, ldx, y, N, D, n, idx, th, sigma, prior_scale, g, acc);
+        for (int j = 0; j < D; ++j) {
+          q[j] = (-half) * g[j] + q[j];
+          th[j] = th[j] + e * q[j];
+        }
+        draw_indices(&dkey, (uint32_t)n, (uint32_t)N, idx);
+        potential(family, X, ldx, y, N, D, n, idx, th, sigma, prior_scale, g, acc);
+        k = 0.0;
+        for (int j = 0; j < D; ++j) {
+          q[j] = (-half) * g[j] + q[j];
+          k += (double)q[j] * q[j];
+        }
+        ke = ke + 0.5f * (float)k;
+        leaf_normals(s2, (uint32_t)D, xi);
+        for (int j = 0; j < D; ++j) q[j] = a_decay * q[j] + a_noise * xi[j];
+      }
+      const double U_new = full_potential_c(family, X, ldx, y, N, D, th, sigma, prior_scale);
+      double log_alpha = (-1.0 / (double)temperature) * (U_new - U_full[c] + (double)ke -
+                                                        (double)ks);
+      if (!(log_alpha <= 0.0)) log_alpha = 0.0;
+      uint32_t fb = (bits_word(msub, 1u, 0u) >> 9) | 0x3F800000u;
+      float u;
+      memcpy(&u, &fb, 4);
+      u = u - 1.0f;
+      if (u < 0.0f) u = 0.0f;
+      const int ok = log((double)u) < log_alpha;
+      if (ok) {
+        memcpy(w, th, sizeof(float) * D);
+        memcpy(p, q, sizeof(float) * D);
+        U_full[c] = U_new;
+      }
+      if (accepts) accepts[(int64_t)c * iters + t] = (int8_t)ok;
+    }
+    keys[2 * c] = key.k0;
+    keys[2 * c + 1] = key.k1;
+    mh_keys[2 * c] = mkey.k0;
+    mh_keys[2 * c + 1] = mkey.k1;
+    dkeys[2 * c] = dkey.k0;
+    dkeys[2 * c + 1] = dkey.k1;
+    free(th);
+    free(q);
+    free(g);
+    free(xi);
+    free(acc);
+    free(idx);
+  }
+  return 0;
+}
+
 double cport_full_potential(int family, const float* X, int64_t ldx, const float* y, int64_t N,
                             int D, const float* w, float sigma, float prior_scale) {
   return full_potential_c(family, X, ldx, y, N, D, w, sigma, prior_scale);

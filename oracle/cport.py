@@ -88,6 +88,25 @@ def amagold(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, step
   return acc.astype(bool)
 
 
+def sggmc(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, steps, friction,
+          temperature=1.0, sigma=1.0, prior_scale=1.0, threads=1):
+  """Runs len(eps) SGGMC iterations (OBABO proposals, full-data MH test) for
+  theta.shape[0] chains in place; returns the accept decisions [C, iterations]."""
+  lib = load()
+  X = np.ascontiguousarray(X, np.float32)
+  y = np.ascontiguousarray(y, np.float32)
+  eps = np.ascontiguousarray(eps, np.float32)
+  C, D = theta.shape
+  acc = np.zeros((C, eps.size), np.int8)
+  lib.cport_sggmc(ctypes.c_int(family), _p(X), ctypes.c_int64(X.shape[1]), _p(y),
+                  ctypes.c_int64(X.shape[0]), ctypes.c_int(D), ctypes.c_int(n),
+                  ctypes.c_int(C), _p(theta), _p(mom), _p(keys), _p(mh_keys), _p(dkeys),
+                  _p(U_full), _p(eps), ctypes.c_int(eps.size), ctypes.c_int(steps),
+                  ctypes.c_float(friction), ctypes.c_float(temperature), ctypes.c_float(sigma),
+                  ctypes.c_float(prior_scale), _p(acc), ctypes.c_int(threads))
+  return acc.astype(bool)
+
+
 def full_potential(family, X, y, theta, sigma=1.0, prior_scale=1.0):
   """FP64 full-data potential of one position (the C port's MH reference value)."""
   lib = load()

@@ -91,3 +91,28 @@ def test_cport_amagold_matches_numpy_oracle():
   np.testing.assert_array_equal(acc[0], r["accept"])
   np.testing.assert_allclose(theta[0], r["theta"][-1], rtol=1e-5, atol=1e-6)
   np.testing.assert_allclose(mom[0], r["p"][-1], rtol=1e-4, atol=1e-5)
+
+
+def test_cport_sggmc_matches_numpy_oracle():
+  """SGGMC (OBABO proposals + full-data MH) in the C port against the numpy oracle:
+  same accept/reject decisions, positions and momenta within float32 rounding."""
+  N, D, n, its, steps, fr = 600, 6, 32, 25, 3, 1.0
+  X, y = _data(N, D, "logistic", seed=3)
+  fam = om.Family("logistic", X, y)
+  eps = np.full(its, 0.02, np.float32)
+  r = ost.sggmc(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
+                lambda th: om.full_potential(fam, th, 1.0, N), np.zeros(D), its, eps, steps,
+                fr, N, n, [D], key=jr.PRNGKey(5), dkey=jr.PRNGKey(6))
+  assert r["accept"].any() and (~r["accept"]).any()
+  k_int, k_mh = jr.split(jr.PRNGKey(5))
+  theta = np.zeros((1, D), np.float32)
+  mom = np.zeros((1, D), np.float32)
+  keys = np.array([k_int], np.uint32)
+  mh_keys = np.array([k_mh], np.uint32)
+  dkeys = np.array([jr.PRNGKey(6)], np.uint32)
+  U = np.array([om.full_potential(fam, np.zeros(D), 1.0, N)])
+  acc = cport.sggmc(1, X, y, n, theta, mom, keys, mh_keys, dkeys, U, eps, steps, fr)
+  np.testing.assert_array_equal(acc[0], r["accept"])
+  np.testing.assert_allclose(theta[0], r["theta"][-1], rtol=1e-5, atol=1e-6)
+  # the C port's per-row likelihoods are float32 (summed in FP64)
+  np.testing.assert_allclose(U[0], r["U_full"][-1], rtol=1e-7)
