@@ -34,6 +34,7 @@ EPS_FIRST, EPS_LAST = 1e-2, 1e-3
 # Algorithmic HBM bytes of one gradient evaluation of one chain: n rows of 4D bytes,
 # n labels and n indices, theta and momentum read and written.
 BYTES_PER_CHAIN_GRAD = BATCH * (4 * D + 8) + 16 * D
+TABLES = 2  # chain tables of the headline run, one stream each
 
 
 def log(msg):
@@ -63,20 +64,27 @@ def chain_keys(R, C):
   return keys, dkeys
 
 
-def run_fused(torch, fused, R, X, y, K, W, graph_iters=10):
+def run_fused(torch, fused, R, X, y, K, W, graph_iters=10, target_bytes=0):
+  """The 64 chains as TABLES chain tables on separate streams (fused.TableGroup): each
+  table's launch prologue and finalizer tail overlap the other table's row gather.
+  Every chain gives what it gives in one table with the same blocks per chain."""
   C = CHAINS
   keys, dkeys = chain_keys(R, C)
   eps = step_sizes(K + W + graph_iters)
   chains = [dict(X=X, y=y, D=D, n=BATCH, theta0=np.zeros(D, np.float32), key=tuple(keys[c]),
                  dkey=tuple(dkeys[c])) for c in range(C)]
-  eng = fused.FusedGLM(chains, "logistic", "sghmc", eps, steps=LEAPFROG, friction=FRICTION,
-                       prior_scale=1.0)
-  tuned = eng.autotune(reps=2)
+  grp = fused.TableGroup.split(chains, TABLES, "logistic", "sghmc", eps, steps=LEAPFROG,
+                               friction=FRICTION, prior_scale=1.0)
+  if target_bytes:
+    # a fixed grid (counter collection serializes launches, which misleads the tuner)
+    tuned = grp.autotune(candidates=[target_bytes], reps=1)
+  else:
+    tuned = grp.autotune(reps=2)
   log(f"autotune: {tuned}")
-  g = eng.capture(graph_iters)
-  # warmup (graph replays) then the timed region, bracketed by synchronizations
-  for _ in range(max(1, W // graph_iters)):
-    g.replay()
+  grp.capture(graph_iters)
+  # warmup (graph replays) then the timed region, bracketed by synchronizations; the
+  # group forks from and joins into the current stream, whose events bracket it
+  grp.run(max(1, W // graph_iters) * graph_iters, graph_iters)
   torch.cuda.synchronize()
   reps = max(1, K // graph_iters)
   stream = torch.cuda.current_stream()
@@ -84,16 +92,15 @@ def run_fused(torch, fused, R, X, y, K, W, graph_iters=10):
   e1 = torch.cuda.Event(enable_timing=True)
   t0 = time.perf_counter()
   e0.record(stream)
-  for _ in range(reps):
-    g.replay()
+  grp.run(reps * graph_iters, graph_iters)
   e1.record(stream)
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   ms = e0.elapsed_time(e1)
   steps = reps * graph_iters
-  ok = bool(torch.isfinite(eng.theta_flat).all().item())
-  return eng, dict(ms=ms, wall=wall, steps=steps, launches=steps * LEAPFROG, finite=ok,
-                   nblocks=eng.nblocks, tuned=tuned)
+  ok = all(bool(torch.isfinite(e.theta_flat).all().item()) for e in grp.engines)
+  return grp, dict(ms=ms, wall=wall, steps=steps, launches=steps * LEAPFROG, finite=ok,
+                   nblocks=grp.nblocks, tuned=tuned)
 
 
 def run_plain(torch, L, R, T, X, y, iters, key, dkey):
@@ -609,6 +616,9 @@ def main():
                   choices=["sghmc-logistic", "resgld-ladder", "psgld-mlp", "sggmc-mlp",
                            "sgld-linreg", "amagold-logistic", "sggmc-logistic"])
   ap.add_argument("--chains", type=int, default=0, help="chains of the sggmc-mlp workload")
+  ap.add_argument("--target-bytes", type=int, default=0,
+                  help="headline: fixed bytes-per-block target instead of the autotuner "
+                       "(for counter collection, which serializes launches)")
   ap.add_argument("--precision", choices=["bf16", "fp8"], default=None,
                   help="GEMM operands of the MLP workloads (sggmc-mlp default fp8, psgld-mlp bf16)")
   args = ap.parse_args()
@@ -650,12 +660,13 @@ def main():
   torch.cuda.synchronize()
   K, W = args.steps, args.warmup
   log(f"fused run: {CHAINS} chains, {K} timed iterations")
-  eng, fr = run_fused(torch, fused, R, X, y, K, W)
+  eng, fr = run_fused(torch, fused, R, X, y, K, W, target_bytes=args.target_bytes)
   chain_steps = CHAINS * fr["steps"]
   value = chain_steps / (fr["ms"] / 1e3)
-  avg_launch_s = fr["ms"] / 1e3 / fr["launches"]
-  bytes_launch = CHAINS * BYTES_PER_CHAIN_GRAD
-  achieved = bytes_launch / avg_launch_s / 1e9
+  # one leapfrog step of all chains = one launch per table, the tables concurrent
+  step_s = fr["ms"] / 1e3 / fr["launches"]
+  bytes_step = CHAINS * BYTES_PER_CHAIN_GRAD
+  achieved = bytes_step / step_s / 1e9
   out = {
       "metric": METRIC, "value": value, "unit": "chain-steps/s", "n_gpus": 1,
       "steps": fr["steps"], "warmup": W, "ms_per_step": fr["ms"] / fr["steps"],
@@ -666,7 +677,8 @@ def main():
                  "chains": CHAINS, "N": N_OBS, "D": D, "batch": BATCH,
                  "leapfrog_steps": LEAPFROG, "friction": FRICTION,
                  "step_size": f"polynomial {EPS_FIRST} -> {EPS_LAST}",
-                 "grid_blocks": fr["nblocks"], "graph_iterations": 10,
+                 "tables": TABLES, "grid_blocks_per_table": fr["nblocks"],
+                 "graph_iterations": 10,
                  "target_bytes_per_block": fr["tuned"]["target_bytes"]},
       "gradient_evals_per_s": value * LEAPFROG,
   }
@@ -674,11 +686,13 @@ def main():
   tp = os.path.join(ROOT, "profiles", "traffic_glm_fused_sghmc_logistic.json")
   if os.path.exists(tp):
     with open(tp) as f:
-      traffic = json.load(f).get("hbm_bytes_per_launch")
+      traffic = json.load(f).get("hbm_bytes_per_step")
   out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "glm_fused_kernel<3> (sgmc_glm_fused_step)",
-                     "bytes_per_launch": bytes_launch, "avg_launch_us": avg_launch_s * 1e6}
+                     "kernel": f"glm_fused_kernel<3> (sgmc_glm_fused_step), {TABLES} tables "
+                               f"of {CHAINS // TABLES} chains on {TABLES} streams",
+                     "bytes_per_step": bytes_step, "launches_per_step": TABLES,
+                     "avg_step_us": step_s * 1e6}
 
   keys, dkeys = chain_keys(R, CHAINS)
   if not args.no_plain:

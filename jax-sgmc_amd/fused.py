@@ -367,6 +367,149 @@ class FusedGLM:
     torch.cuda.synchronize()
 
 
+class TableGroup:
+  """Several chain tables advanced side by side, each on its own HIP stream.
+
+  A fused launch begins with a prologue (index draw, evaluation point) before its row
+  gather saturates HBM and ends with a tail in which only each chain's last-arriving
+  block works (the reduction records and the update with its noise). Tables on
+  separate streams fill one table's prologue and tail with another table's row gather,
+  so the device stays on the gather for nearly the whole step. Chains stay
+  independent: a chain gives bit for bit what it gives in a single table with the same
+  blocks per chain.
+
+  engines: FusedGLM tables without an exchange or MH test attached. Results are indexed
+  over the group's chains in table order.
+  """
+
+  def __init__(self, engines):
+    engines = list(engines)
+    if not engines:
+      raise ValueError("TableGroup needs at least one table")
+    for e in engines:
+      if e.exchange is not None or e.mh is not None:
+        raise ValueError("tables of a TableGroup run without exchange or MH tests")
+    self.engines = engines
+    self.streams = [torch.cuda.Stream(device=e.device) for e in engines]
+    self.offsets = np.concatenate([[0], np.cumsum([e.C for e in engines])]).astype(np.int64)
+    self.C = int(self.offsets[-1])
+    self._graphs = None
+    self._graph_iters = 0
+
+  @classmethod
+  def split(cls, chains, tables, *args, **kwargs):
+    """``tables`` FusedGLM tables over consecutive, near-equal slices of ``chains``
+    (the other arguments as for FusedGLM)."""
+    k = max(1, min(int(tables), len(chains)))
+    cut = np.linspace(0, len(chains), k + 1).round().astype(int)
+    return cls([FusedGLM(chains[a:b], *args, **kwargs) for a, b in zip(cut[:-1], cut[1:])])
+
+  @property
+  def nblocks(self):
+    return [e.nblocks for e in self.engines]
+
+  def _locate(self, i):
+    t = int(np.searchsorted(self.offsets, i, side="right") - 1)
+    return self.engines[t], i - int(self.offsets[t])
+
+  def _fork(self):
+    cur = torch.cuda.current_stream()
+    for s in self.streams:
+      s.wait_stream(cur)
+
+  def _join(self):
+    cur = torch.cuda.current_stream()
+    for s in self.streams:
+      cur.wait_stream(s)
+
+  def _each(self, fn):
+    self._fork()
+    for e, s in zip(self.engines, self.streams):
+      with torch.cuda.stream(s):
+        fn(e)
+    self._join()
+
+  def iteration(self):
+    """One iteration of every table, the tables concurrently."""
+    self._each(lambda e: e.iteration())
+
+  def capture(self, iterations_per_graph=1):
+    """One HIP graph of ``iterations_per_graph`` iterations per table, replayed on
+    the table's stream."""
+    self._graphs = [e.capture(iterations_per_graph) for e in self.engines]
+    self._graph_iters = iterations_per_graph
+    return self._graphs
+
+  def run(self, iterations, graph_iterations=0):
+    """Runs ``iterations`` iterations of every table (by replaying graphs of
+    ``graph_iterations`` iterations when given; the remainder eagerly). The current
+    stream waits for all tables."""
+    rest = iterations
+    if graph_iterations and iterations >= graph_iterations:
+      if self._graphs is None or self._graph_iters != graph_iterations:
+        self.capture(graph_iterations)
+      reps, rest = divmod(iterations, graph_iterations)
+      self._fork()
+      for _ in range(reps):
+        for g, s in zip(self._graphs, self.streams):
+          with torch.cuda.stream(s):
+            g.replay()
+      self._join()
+    for _ in range(rest):
+      self.iteration()
+
+  def autotune(self, candidates=None, reps=3):
+    """Picks one bytes-per-block target for every table by timing their launches run
+    concurrently; restores the chain states afterwards."""
+    if candidates is None:
+      base = max(e.target_bytes for e in self.engines)
+      candidates = sorted({max(16 * 1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4, 8, 16)})
+    snaps = [e._snapshot() for e in self.engines]
+    best = None
+    for tb in candidates:
+      for e in self.engines:
+        e.target_bytes = tb
+        e._build_table()
+      self._each(lambda e: e._launch_steps())
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      for _ in range(reps):
+        self._each(lambda e: e._launch_steps())
+      torch.cuda.synchronize()
+      dt = (time.perf_counter() - t0) / reps
+      if best is None or dt < best[0]:
+        best = (dt, tb)
+    for e, snap in zip(self.engines, snaps):
+      e.target_bytes = best[1]
+      e._build_table()
+      e._restore(snap)
+    self._graphs = None
+    self.tuned = {"target_bytes": best[1], "seconds_per_iteration": best[0],
+                  "candidates": list(candidates), "tables": len(self.engines)}
+    return self.tuned
+
+  def theta(self, i=None):
+    if i is None:
+      return [self.theta(j) for j in range(self.C)]
+    e, j = self._locate(i)
+    return e.theta(j)
+
+  def momentum(self, i):
+    e, j = self._locate(i)
+    return e.momentum(j)
+
+  def sample_trace(self, i):
+    e, j = self._locate(i)
+    return e.sample_trace(j)
+
+  def sample_potentials(self, i):
+    e, j = self._locate(i)
+    return e.sample_potentials(j)
+
+  def iteration_count(self):
+    return min(e.iteration_count() for e in self.engines)
+
+
 class ReplicaExchange:
   """parallel_tempering's swap step (solver.py:264-293) between pairs of chains of a
   FusedGLM, launched after every iteration (``sgmc_resgld_swap``).

@@ -243,3 +243,35 @@ def test_loop_launch_equals_per_evaluation_launches(mode):
                                   b.sample_potentials(c).cpu().numpy())
   np.testing.assert_array_equal(a.keys.cpu().numpy(), b.keys.cpu().numpy())
   np.testing.assert_array_equal(a.dkeys.cpu().numpy(), b.dkeys.cpu().numpy())
+
+
+@pytest.mark.parametrize("tables,graph", [(2, 0), (3, 4)])
+def test_table_group_equals_one_table(tables, graph):
+  """Chains split over tables on separate streams give bit for bit the single table's
+  positions, momenta and saved samples (same bytes-per-block target, so every chain
+  keeps its block count and reduction order); graph replays included."""
+  print("table group", tables, "graph", graph, flush=True)
+  its = 8
+  eps = np.full(its, 2e-3, np.float32)
+  sizes = [(3000, 40, 256), (5000, 128, 512), (2500, 12, 64), (4000, 256, 300),
+           (3000, 40, 128), (6000, 64, 700)]
+  chains = []
+  for c, (N, D, n) in enumerate(sizes):
+    X, y = _data(N, D, "logistic", 20 + c)
+    chains.append(dict(X=_dev(X), y=torch.as_tensor(y, device=DEV), D=D, n=n,
+                       theta0=np.full(D, 0.01 * c, np.float32), key=(3, c), dkey=(c, 5)))
+  kw = dict(steps=3, friction=0.7, target_bytes=64 * 1024,
+            save_slots=np.arange(its, dtype=np.int32), n_slots=its)
+  one = fused.FusedGLM(chains, "logistic", "sghmc", eps, **kw)
+  grp = fused.TableGroup.split(chains, tables, "logistic", "sghmc", eps, **kw)
+  assert len(grp.engines) == tables and grp.C == len(chains)
+  assert any(nb > 1 for nb in one.nblocks_chain)
+  one.run(its)
+  grp.run(its, graph_iterations=graph)
+  torch.cuda.synchronize()
+  assert grp.iteration_count() == one.iteration_count() == its
+  for c in range(len(chains)):
+    assert torch.equal(grp.theta(c), one.theta(c)), c
+    assert torch.equal(grp.momentum(c), one.momentum(c)), c
+    assert torch.equal(grp.sample_trace(c), one.sample_trace(c)), c
+    assert torch.equal(grp.sample_potentials(c), one.sample_potentials(c)), c
