@@ -451,7 +451,37 @@ def run_sgld_linreg(args):
                       "frac": bytes_step / (ms / 1e3 / iters) / 1e9 / HBM_PEAK_GBS,
                       "traffic": None, "kernel": "glm_loop_kernel (latency bound: one block)",
                       "bytes_per_chain_step": bytes_step}}
+  if not args.no_cpu:
+    log("CPU baseline (oracle C port, one chain)")
+    out["cpu_baseline"] = linreg_cpu_baseline(X.cpu().numpy(), y.cpu().numpy(), eps)
+    out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+  else:
+    out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
+
+
+def linreg_cpu_baseline(X_host, y_host, eps_table, target_s=10.0):
+  """The oracle's C port of the same single-chain SGLD loop on one host core (a single
+  chain has nothing to spread over more)."""
+  from oracle import cport
+  N, Dl = X_host.shape
+
+  def go(iters):
+    theta = np.zeros((1, Dl), np.float32)
+    keys = np.zeros((1, 2), np.uint32)
+    dkeys = np.zeros((1, 2), np.uint32)
+    eps = np.resize(np.asarray(eps_table, np.float32), iters)
+    t0 = time.perf_counter()
+    cport.sgld(0, X_host, y_host, 128, theta, keys, dkeys, eps, sigma=0.5,
+               prior_scale=float(np.sqrt(10.0)), threads=1)
+    return time.perf_counter() - t0
+
+  t1 = go(2000)
+  iters = int(max(2000, min(2_000_000, target_s * 2000 / max(t1, 1e-6))))
+  dt = go(iters)
+  return {"value": iters / dt, "unit": "chain-steps/s", "cores": 1, "kind": "port",
+          "sample": f"1 chain x {iters} iterations of the same workload (oracle/cport.c "
+                    f"SGLD linear regression, {dt:.1f} s)"}
 
 
 # Full-data MH workloads on the headline's data: leapfrog steps per test, friction and

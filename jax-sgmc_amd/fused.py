@@ -378,17 +378,17 @@ class TableGroup:
   independent: a chain gives bit for bit what it gives in a single table with the same
   blocks per chain.
 
-  engines: FusedGLM tables without an exchange or MH test attached. Results are indexed
-  over the group's chains in table order.
+  engines: FusedGLM tables. A replica exchange or MH test attached to a table acts on
+  that table's chains only and runs on its stream after each of its iterations. Results
+  are indexed over the group's chains in table order.
   """
 
   def __init__(self, engines):
     engines = list(engines)
     if not engines:
       raise ValueError("TableGroup needs at least one table")
-    for e in engines:
-      if e.exchange is not None or e.mh is not None:
-        raise ValueError("tables of a TableGroup run without exchange or MH tests")
+    if len({str(e.device) for e in engines}) != 1:
+      raise ValueError("the tables of a TableGroup share one device")
     self.engines = engines
     self.streams = [torch.cuda.Stream(device=e.device) for e in engines]
     self.offsets = np.concatenate([[0], np.cumsum([e.C for e in engines])]).astype(np.int64)
