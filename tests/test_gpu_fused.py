@@ -275,3 +275,21 @@ def test_table_group_equals_one_table(tables, graph):
     assert torch.equal(grp.momentum(c), one.momentum(c)), c
     assert torch.equal(grp.sample_trace(c), one.sample_trace(c)), c
     assert torch.equal(grp.sample_potentials(c), one.sample_potentials(c)), c
+
+
+def test_table_group_with_exchanges_equals_tables_alone():
+  """Two ladder tables, each with its own replica exchange, advanced as a table group
+  on two streams (graph replays) equal the same tables run one after the other."""
+  print("table group with exchanges", flush=True)
+  grouped = [_ladder(R_=4), _ladder(R_=5, D=40, N=900)]
+  alone = [_ladder(R_=4), _ladder(R_=5, D=40, N=900)]
+  grp = fused.TableGroup([e for e, _ in grouped])
+  grp.run(12, graph_iterations=6)
+  for e, _ in alone:
+    e.run(12)
+  torch.cuda.synchronize()
+  for (ga, exa), (gb, exb) in zip(grouped, alone):
+    assert torch.equal(ga.theta_flat, gb.theta_flat)
+    assert torch.equal(exa.swaps, exb.swaps) and torch.equal(exa.ssq, exb.ssq)
+    assert torch.equal(exa.samples, exb.samples)
+    assert torch.equal(exa.step, exb.step) and int(exa.step.sum()) > 0
