@@ -487,7 +487,6 @@ def linreg_cpu_baseline(X_host, y_host, eps_table, target_s=10.0):
 # Full-data MH workloads on the headline's data: leapfrog steps per test, friction and
 # the polynomial step-size range.
 MH_SETTINGS = {"amagold": (5, 0.25, (1e-4, 5e-5)), "sggmc": (5, 1.0, (1e-4, 5e-5))}
-FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
 def mh_cpu_baseline(kind, X_host, y_host, theta0, keys, dkeys, eps, target_s=12.0):
@@ -593,7 +592,10 @@ def run_mh(args, kind):
   step_us = avg_us(lambda: eng._launch(1))
   value = CHAINS * iters / (ms / 1e3)
   full_flops = 2.0 * N_OBS * D * CHAINS
-  achieved = full_flops / (full_us / 1e6) / 1e12
+  # the split pass reads X once (4 D + 4 bytes per row) and runs three fp16 products per
+  # multiply-add on the matrix cores (about 80 us of matrix time): bound by the X stream
+  full_bytes = N_OBS * (4 * D + 4)
+  achieved = full_bytes / (full_us / 1e6) / 1e9
   step_bytes = CHAINS * BYTES_PER_CHAIN_GRAD
   proposal = ("reversible-leapfrog steps" if kind == "amagold" else
               "OBABO steps (two minibatch gradients each)")
@@ -612,12 +614,13 @@ def run_mh(args, kind):
                     "target_bytes_per_block": tuned["target_bytes"],
                     "full_potential_blocks": mh.nblocks},
          "mean_acceptance_ratio": accept_mean,
-         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
-                      "traffic": None,
-                      "kernel": "glm_full_chains_kernel<2, logistic> "
-                                "(sgmc_glm_full_potential_chains)",
-                      "flops_per_launch": full_flops, "avg_launch_us": full_us},
+         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                      "kernel": "glm_full_chains_split_kernel<2, logistic> "
+                                "(sgmc_glm_full_potential_chains_split)",
+                      "bytes_per_launch": full_bytes, "avg_launch_us": full_us,
+                      "algorithmic_TFLOPs": full_flops / (full_us / 1e6) / 1e12,
+                      "fp16_matrix_TFLOPs_executed": 3 * full_flops / (full_us / 1e6) / 1e12},
          "step_kernel": {"kernel": f"glm_fused_kernel<3> ({name} mode)",
                          "avg_launch_us": step_us,
                          "achieved_GBs": step_bytes / (step_us / 1e6) / 1e9,

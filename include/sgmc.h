@@ -124,6 +124,19 @@ int sgmc_glm_full_potential_chains(int family, const float* X, int64_t ldx, cons
                                    int nchains, float sigma, int has_prior, float prior_scale,
                                    double temperature, double* out_U, double* workspace,
                                    int64_t workspace_doubles, int nblocks, void* stream);
+/* The same on the fp16 matrix cores. Each scaled value becomes a hi + lo pair of fp16
+ * halves (2^-22 relative) and a product is lo hi + hi lo + hi hi with fp32
+ * accumulation, so the row dot products hold about 1e-6 relative (float32 products:
+ * 6e-8) for a fifth of the matrix time; sums stay FP64. X is scaled by 2^x_exp, chosen
+ * once for the dataset so that max |X| 2^x_exp < 2^14 (for example 14 - the frexp
+ * exponent of max |X|; -64 <= x_exp <= 64); each chain's position is scaled by its own
+ * power of two from its largest finite entry. */
+int sgmc_glm_full_potential_chains_split(int family, const float* X, int64_t ldx,
+                                         const float* y, int64_t N, int D, const float* thetas,
+                                         int64_t ld_theta, int nchains, float sigma,
+                                         int has_prior, float prior_scale, double temperature,
+                                         int x_exp, double* out_U, double* workspace,
+                                         int64_t workspace_doubles, int nblocks, void* stream);
 
 /* ------------------------------------------------------ step updates ----
  * Elementwise integrator updates over a flattened pytree; noise drawn in registers

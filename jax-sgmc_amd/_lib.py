@@ -101,6 +101,10 @@ _SIGNATURES = {
                                                c_int, c_void_p, c_int64, c_int, c_float, c_int,
                                                c_float, c_double, c_void_p, c_void_p, c_int64,
                                                c_int, c_void_p]),
+    "sgmc_glm_full_potential_chains_split": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                                     c_int, c_void_p, c_int64, c_int, c_float,
+                                                     c_int, c_float, c_double, c_int, c_void_p,
+                                                     c_void_p, c_int64, c_int, c_void_p]),
     "sgmc_glm_fused_loop": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_float,
                                     c_float, c_int, c_int, c_float, c_float, c_float, c_void_p,
                                     c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,

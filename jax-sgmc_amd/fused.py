@@ -14,6 +14,7 @@ Chains may differ in dataset, N, D (<= 1024), minibatch size and temperature
 results equal the same chain run alone (chains are independent; every chain owns
 its keys and its reduction order is fixed by its own block count).
 """
+import math
 import time
 
 import numpy as np
@@ -606,11 +607,14 @@ class _SharedDataTest:
 
   keys: one key (two uint32 words) per chain, split as the reference's MH solvers do
   at init (solver.py:340-352, 489-490): split(key)[0] starts the integrator,
-  split(key)[1] the test."""
+  split(key)[1] the test. full_potential: "split" (default) takes the row dot products
+  on the fp16 matrix cores as hi + lo halves (about 1e-6 relative per dot product,
+  ``sgmc_glm_full_potential_chains_split``), "fp32" on the fp32 matrix cores (6e-8);
+  both sum in FP64."""
 
   mode = None
 
-  def __init__(self, engine, keys=None, nblocks=None):
+  def __init__(self, engine, keys=None, nblocks=None, full_potential="split"):
     if engine.mode != self.mode:
       raise ValueError(f"{type(self).__name__} needs a FusedGLM in mode "
                        f"{[k for k, v in MODES.items() if v == self.mode][0]!r}")
@@ -628,6 +632,13 @@ class _SharedDataTest:
     self.N = int(c0.get("N", c0["X"].shape[0]))
     self.D = int(c0["D"])
     self.ld_theta = int(engine.poff[1] - engine.poff[0])
+    if full_potential not in ("split", "fp32"):
+      raise ValueError("full_potential must be 'split' or 'fp32'")
+    self.split = full_potential == "split"
+    # one scale for the dataset: max |X| 2^x_exp < 2^14
+    amax = float(self.X[:self.N, :self.D].abs().max().item()) if self.split else 0.0
+    self.x_exp = 14 - math.frexp(amax)[1] if math.isfinite(amax) and amax > 0 else 0
+    self.x_exp = max(-64, min(64, self.x_exp))
     C = engine.C
     self.nblocks = int(nblocks or torch.cuda.get_device_properties(dev).multi_processor_count)
     self.ws = torch.zeros(int(_lib.query("sgmc_glm_full_chains_workspace_doubles",
@@ -650,11 +661,14 @@ class _SharedDataTest:
 
   def _full(self, out):
     eng = self.engine
-    _lib.call("sgmc_glm_full_potential_chains", eng.family, _lib.ptr(self.X),
-              int(self.X.stride(0)), _lib.ptr(self.y), self.N, self.D, _lib.ptr(eng.theta_flat),
-              self.ld_theta, eng.C, eng.sigma, eng.has_prior, eng.prior_scale,
-              eng.pot_temperature, _lib.ptr(out), _lib.ptr(self.ws), self.ws.numel(),
-              self.nblocks, _lib.stream())
+    head = (eng.family, _lib.ptr(self.X), int(self.X.stride(0)), _lib.ptr(self.y), self.N,
+            self.D, _lib.ptr(eng.theta_flat), self.ld_theta, eng.C, eng.sigma, eng.has_prior,
+            eng.prior_scale, eng.pot_temperature)
+    tail = (_lib.ptr(out), _lib.ptr(self.ws), self.ws.numel(), self.nblocks, _lib.stream())
+    if self.split:
+      _lib.call("sgmc_glm_full_potential_chains_split", *head, self.x_exp, *tail)
+    else:
+      _lib.call("sgmc_glm_full_potential_chains", *head, *tail)
 
   def acceptance_ratios(self, i):
     """[n_slots] exp(log a) of the test at each saved iteration of chain i."""
@@ -669,8 +683,8 @@ class AmagoldMH(_SharedDataTest):
 
   mode = _lib.MODE_AMAGOLD
 
-  def __init__(self, engine, keys=None, nblocks=None):
-    super().__init__(engine, keys, nblocks)
+  def __init__(self, engine, keys=None, nblocks=None, full_potential="split"):
+    super().__init__(engine, keys, nblocks, full_potential)
     from . import random as R
     from . import tree as T
     dev = engine.device
@@ -706,8 +720,8 @@ class SggmcMH(_SharedDataTest):
 
   mode = _lib.MODE_SGGMC
 
-  def __init__(self, engine, keys=None, nblocks=None):
-    super().__init__(engine, keys, nblocks)
+  def __init__(self, engine, keys=None, nblocks=None, full_potential="split"):
+    super().__init__(engine, keys, nblocks, full_potential)
     from . import random as R
     dev = engine.device
     self._set_integrator_keys([R.key_words(k) for k in self.int_keys])

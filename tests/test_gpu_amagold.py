@@ -4,6 +4,8 @@ momentum and positions bit for bit; the energy within float32 summation error), 
 alias trajectory and its accept/reject decisions against the oracle, the reference's
 Kolmogorov-Smirnov fixture (tests/test_alias.py:165-182), and the dual-averaging step
 size (scheduler.py:376-444) on both MH solvers."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -280,7 +282,8 @@ def test_full_size_properties():
   full potential equals per-chain FP64 passes (independent float32 row evaluations,
   both summed in FP64) to 1e-7 relative (a few float32 ulps per row; observed 7e-9),
   and a frictionless fused AMAGOLD iteration's energy telescopes to
-  KE(p_start) - KE(p_end) (integrator.py:430-447)."""
+  KE(p_start) - KE(p_end) (integrator.py:430-447). The split-fp16 pass meets the same
+  bound (observed 7e-9)."""
   print("full-size properties", flush=True)
   from jax_sgmc_amd import fused as FU
   X, y, w = _c2_data()
@@ -293,6 +296,11 @@ def test_full_size_properties():
   L.call("sgmc_glm_full_potential_chains", L.LIK_LOGISTIC, L.ptr(X), 512, L.ptr(y), 1_000_000,
          512, L.ptr(thetas), 512, C, 1.0, 1, 1.0, 1.0, L.ptr(U), L.ptr(ws), ws.numel(), nb,
          L.stream())
+  Us = torch.empty(C, dtype=torch.float64, device=DEV)
+  x_exp = 14 - math.frexp(float(X.abs().max()))[1]
+  L.call("sgmc_glm_full_potential_chains_split", L.LIK_LOGISTIC, L.ptr(X), 512, L.ptr(y),
+         1_000_000, 512, L.ptr(thetas), 512, C, 1.0, 1, 1.0, 1.0, x_exp, L.ptr(Us), L.ptr(ws),
+         ws.numel(), nb, L.stream())
   rpb = 1024
   ws1 = torch.zeros(int(L.query("sgmc_glm_full_workspace_doubles", 1_000_000, rpb)),
                     dtype=torch.float64, device=DEV)
@@ -302,6 +310,8 @@ def test_full_size_properties():
            L.ptr(thetas[c]), 1.0, 1, 1.0, 1.0, L.ptr(u1), L.ptr(ws1), ws1.numel(), rpb,
            L.stream())
     assert abs(U[c].item() - u1.item()) <= 1e-7 * abs(u1.item()), (c, U[c].item(), u1.item())
+    # fp16 hi + lo halves: dot products good to ~1e-6, the sum over 1e6 rows to ~1e-8
+    assert abs(Us[c].item() - u1.item()) <= 1e-7 * abs(u1.item()), (c, Us[c].item(), u1.item())
   th0 = w.cpu().numpy().astype(np.float32)
   chains = [dict(X=X, y=y, D=512, n=4096, theta0=th0, dkey=(3, c)) for c in range(8)]
   eng = FU.FusedGLM(chains, "logistic", "amagold", np.full(2, 2e-4, np.float32), steps=5,

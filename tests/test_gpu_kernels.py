@@ -3,6 +3,8 @@
 RNG words, indices and normals must match bit for bit; potentials and gradients
 within 1e-5 relative (atol floor 1e-5 * max|x|) of the float64 oracle.
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -227,33 +229,48 @@ def test_glm_full_potential_f64(family, N, D):
   assert abs(out.item() - ref) <= 1e-6 * (abs(ref) + N), (out.item(), ref)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("family", [L.LIK_LINEAR_GAUSSIAN, L.LIK_LOGISTIC])
 @pytest.mark.parametrize("N,D,C", [(1, 3, 1), (3001, 37, 33), (2500, 512, 64), (777, 1000, 70),
                                    (4097, 8, 2)])
-def test_glm_full_potential_chains(family, N, D, C):
-  """Many chains in one pass over X: every chain's full potential against the FP64
-  oracle, and bit-equal to a second call (fixed reduction order)."""
-  print("glm_full_chains", family, N, D, C, flush=True)
+def test_glm_full_potential_chains(family, N, D, C, split):
+  """Many chains in one pass over X (fp32 matrix cores, or fp16 hi + lo halves): every
+  chain's full potential against the FP64 oracle, bit-equal to a second call (fixed
+  reduction order); one chain with a NaN entry (and, for the split kernel, one scaled
+  far from the others) leaves the other chains' values untouched."""
+  print("glm_full_chains", family, N, D, C, "split", split, flush=True)
   X, y = _glm_data(N, D, family, seed=5)
   thetas = (0.2 * np.random.default_rng(6).standard_normal((C, D))).astype(np.float32)
+  if C > 2:
+    thetas[1] *= 1e4    # its own power-of-two scale
   Xd, ldx = _padded(X)
   yd = torch.as_tensor(y, device=DEV)
   ld_theta = D + 3
   th = torch.zeros((C, ld_theta), device=DEV)
   th[:, :D] = torch.as_tensor(thetas, device=DEV)
+  if C > 1:
+    th[C - 1, D - 1] = float("nan")
   nblocks = 96
   ws = torch.zeros(int(L.query("sgmc_glm_full_chains_workspace_doubles", nblocks)),
                    dtype=torch.float64, device=DEV)
+  x_exp = 14 - math.frexp(float(np.max(np.abs(X))))[1]
   outs = []
   for _ in range(2):
     out = torch.empty(C, dtype=torch.float64, device=DEV)
-    L.call("sgmc_glm_full_potential_chains", family, L.ptr(Xd), ldx, L.ptr(yd), N, D, L.ptr(th),
-           ld_theta, C, 0.5, 1, 1.0, 2.0, L.ptr(out), L.ptr(ws), ws.numel(), nblocks, L.stream())
+    head = (family, L.ptr(Xd), ldx, L.ptr(yd), N, D, L.ptr(th), ld_theta, C, 0.5, 1, 1.0, 2.0)
+    tail = (L.ptr(out), L.ptr(ws), ws.numel(), nblocks, L.stream())
+    if split:
+      L.call("sgmc_glm_full_potential_chains_split", *head, x_exp, *tail)
+    else:
+      L.call("sgmc_glm_full_potential_chains", *head, *tail)
     outs.append(out.cpu().numpy())
   np.testing.assert_array_equal(outs[0], outs[1])
   kind = "linear_gaussian" if family == L.LIK_LINEAR_GAUSSIAN else "logistic"
   fam = om.Family(kind, X, y, sigma=0.5)
   for c in range(C):
+    if C > 1 and c == C - 1:
+      assert np.isnan(outs[0][c])
+      continue
     ref = om.full_potential(fam, thetas[c], 1.0, mb=N, temperature=2.0)
     assert abs(outs[0][c] - ref) <= 1e-6 * (abs(ref) + N), (c, outs[0][c], ref)
 
