@@ -82,6 +82,10 @@ def _fused_eligible(potential_fn, data_loader, init_samples, fused):
   return ok
 
 
+# chains from which alias runs of multi-block chains use two tables on two streams
+_TABLE_SPLIT_CHAINS = 16
+
+
 def _key_words(k):
   if k is None:
     return (0, 0)
@@ -112,10 +116,19 @@ def _run_fused(potential_fn, data_loader, batch_size, init_samples, iterations, 
                        dkey=_key_words(data_keys[c] if data_keys else None),
                        temperature=float(temp[0]) if temp.size else 1.0))
   from . import fused as FU
-  eng = FU.FusedGLM(chains, lik.family, mode, eps, steps=steps, friction=friction, alpha=alpha,
-                    lmbd=lmbd, sigma=lik.sigma, prior_scale=pscale, has_prior=bool(has_prior),
-                    pot_temperature=potential_fn.temperature, save_slots=slots,
-                    n_slots=max(1, kept.size))
+  kw = dict(steps=steps, friction=friction, alpha=alpha, lmbd=lmbd, sigma=lik.sigma,
+            prior_scale=pscale, has_prior=bool(has_prior),
+            pot_temperature=potential_fn.temperature, save_slots=slots,
+            n_slots=max(1, kept.size))
+  total = sum(int(c["n"]) * 4 * FU._align4(int(c["D"])) for c in chains)
+  tb = max(64 * 1024, total // 2048)
+  multi = any(FU.blocks_for(int(c["n"]), int(c["D"]), tb) > 1 for c in chains)
+  if len(chains) >= _TABLE_SPLIT_CHAINS and multi:
+    # many multi-block chains: two tables on two streams, so that one table's launch
+    # tail overlaps the other's row gather (same results chain by chain)
+    eng = FU.TableGroup.split(chains, 2, lik.family, mode, eps, **kw)
+  else:
+    eng = FU.FusedGLM(chains, lik.family, mode, eps, **kw)
   eng.run(iterations)
   torch.cuda.synchronize()
   out = []

@@ -307,3 +307,27 @@ def test_sgld_mlp_engine_matches_generic_path(precision):
     d = np.linalg.norm(a[k] - b[k])
     print(k, d, d_ref, flush=True)
     assert d <= tol * d_ref + 1e-6, (k, d, d_ref)
+
+
+def test_sghmc_many_chains_two_tables_match_generic_path():
+  """alias.sghmc with 16 multi-block chains runs as two chain tables on two streams;
+  every chain equals the same chain through the generic kernels."""
+  print("sghmc 16 chains, two tables", flush=True)
+  X, y = logreg(N=4000, D=64, seed=11)
+  loader = DT.DeviceNumpyDataLoader(x=X, y=y)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  run = alias.sghmc(pot, loader, cache_size=1, batch_size=1024, integration_steps=3,
+                    friction=0.5, first_step_size=2e-3, last_step_size=1e-3, accepted_samples=5)
+  C = 16
+  # the case takes the two-table path: enough chains, several blocks per chain
+  tb = max(64 * 1024, C * 1024 * 4 * 64 // 2048)
+  assert C >= alias._TABLE_SPLIT_CHAINS and FU.blocks_for(1024, 64, tb) > 1
+  keys = [R.PRNGKey(100 + c, DEV) for c in range(C)]
+  dkeys = [R.PRNGKey(200 + c, DEV) for c in range(C)]
+  inits = [torch.as_tensor(0.02 * np.random.default_rng(c).standard_normal(64).astype(np.float32))
+           for c in range(C)]
+  a = run(*inits, iterations=6, keys=keys, data_keys=dkeys, fused=True)
+  b = run(*inits, iterations=6, keys=keys, data_keys=dkeys, fused=False)
+  for c in range(C):
+    _close(a[c]["samples"]["variables"], b[c]["samples"]["variables"], f"chain {c}")
+    _close(a[c]["samples"]["energy"], b[c]["samples"]["energy"], f"energy {c}")
