@@ -46,11 +46,13 @@ def _within(got, ref, what, rtol=1e-5):
   assert np.all(err <= tol), f"{what}: max err {err.max():.3g}, max tol {tol.max():.3g}"
 
 
-@pytest.mark.parametrize("family", ["logistic", "linear_gaussian"])
-def test_fused_obabo_state_matches_oracle(family):
+@pytest.mark.parametrize("family,mass", [("logistic", False), ("linear_gaussian", False),
+                                         ("logistic", True)])
+def test_fused_obabo_state_matches_oracle(family, mass):
   """Positions, momenta, potentials and both kinetic-energy accumulators of two chains
-  with their own keys after every iteration, against the oracle's obabo."""
-  print("fused obabo vs oracle", family, flush=True)
+  with their own keys after every iteration, against the oracle's obabo (with a shared
+  diagonal mass: M^{1/2} in both O steps, M^-1 in the A step and the energies)."""
+  print("fused obabo vs oracle", family, "mass", mass, flush=True)
   if family == "logistic":
     X, y = logreg()
     fam = om.Family("logistic", X, y)
@@ -66,10 +68,20 @@ def test_fused_obabo_state_matches_oracle(family):
   dkeys = [(3, 0), (0, 9)]
   chains = [dict(X=Xd, y=yd, D=D, n=n, theta0=np.full(D, 0.1 * c, np.float32), key=keys[c],
                  dkey=dkeys[c]) for c in range(2)]
-  eng = FU.FusedGLM(chains, family, "obabo", eps, steps=steps, friction=fr, sigma=0.5)
+  inv_m = sqrt_m = None
+  mk = {}
+  if mass:
+    inv = np.linspace(0.5, 2.0, D).astype(np.float32)
+    sq = (1.0 / np.sqrt(inv)).astype(np.float32)
+    inv_m = torch.as_tensor(inv, device=DEV)
+    sqrt_m = torch.as_tensor(sq, device=DEV)
+    mk = dict(inv_mass=inv.astype(np.float64), sqrt_mass=sq.astype(np.float64))
+  eng = FU.FusedGLM(chains, family, "obabo", eps, steps=steps, friction=fr, sigma=0.5,
+                    inv_mass=inv_m, sqrt_mass=sqrt_m)
   refs = [ost.obabo(lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0),
                     np.full(D, 0.1 * c), its, eps, steps, fr, N, n, [D],
-                    key=np.asarray(keys[c], np.uint32), dkey=np.asarray(dkeys[c], np.uint32))
+                    key=np.asarray(keys[c], np.uint32), dkey=np.asarray(dkeys[c], np.uint32),
+                    **mk)
           for c in range(2)]
   for t in range(its):
     eng.iteration()
