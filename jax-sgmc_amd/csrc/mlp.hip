@@ -661,19 +661,29 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     float lp = 0.0f, ke = 0.0f, wmax = 0.0f;
     if (EPI == EPI_UPDATE) {
       const Key lk = leaf_key(s, c, g.leaf);
-      // Four chunks of eight parameters: a chunk's loads go out before its arithmetic
-      // and stores. Small chunks keep the kernel within 128 registers, so four waves
-      // per SIMD share it and one wave's noise arithmetic hides another's memory time.
+      // Four chunks of eight parameters, software-pipelined: the next chunk's loads go
+      // out before this chunk's noise arithmetic and stores (each parameter is read and
+      // written by this thread only), so its memory time overlaps the arithmetic.
+      PState p0[4], p1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = 8 * (u >> 2) + 4 * (lane >> 5) + (u & 3);
+        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+        p0[u] = load_pstate(s, base + i0);
+        p1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+      }
 #pragma unroll
       for (int j0 = 0; j0 < 16; j0 += 4) {
-        PState p0[4], p1[4];
+        PState q0[4], q1[4];
+        if (j0 + 4 < 16) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int j = j0 + u;
-          const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-          const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-          p0[u] = load_pstate(s, base + i0);
-          p1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + 4 + u;
+            const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
+            q0[u] = load_pstate(s, base + i0);
+            q1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+          }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -687,6 +697,13 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
           step_pstate(s, p1[u], ab * acc1[j], bits_to_normal(w1), k, lp, ke);
           store_pstate(s, c, g.leaf, base, i0, p0[u], wmax);
           store_pstate(s, c, g.leaf, base, i1, p1[u], wmax);
+        }
+        if (j0 + 4 < 16) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            p0[u] = q0[u];
+            p1[u] = q1[u];
+          }
         }
       }
     } else {
