@@ -1,10 +1,11 @@
 """Headline benchmark: chain-steps/s of the fused multi-chain SGHMC step on one MI355X.
 
-Workload (the BASELINE config that fits one GPU and carries the metric): SGHMC
-(friction_leapfrog, 5 leapfrog steps, friction 0.5) on Bayesian logistic regression,
-X ~ N(0,1)/sqrt(D) float32 [N=1,000,000, D=512], y ~ Bernoulli(sigmoid(X w*)),
-w* ~ N(0, I), prior N(0, I), minibatch 4096, 64 independent chains advanced together.
-A step is one reference iteration for every chain (5 gradient evaluations each).
+Workload (the multi-chain configuration that fits one GPU): SGHMC (friction_leapfrog,
+5 leapfrog steps, friction 0.5) on Bayesian logistic regression, X ~ N(0,1)/sqrt(D)
+float32 [N=1,000,000, D=512], y ~ Bernoulli(sigmoid(X w*)), w* ~ N(0, I), prior
+N(0, I), minibatch 4096, 64 independent chains advanced together as two chain tables
+on two HIP streams. A step is one reference iteration for every chain (5 gradient
+evaluations each).
 Data is synthetic, drawn on the GPU with the library's jax-compatible Threefry
 normal/uniform from PRNGKey(1).
 
