@@ -83,3 +83,54 @@ def test_struct_layouts_match_the_header(tmp_path):
     assert c[(cname, "size")] == ctypes.sizeof(cls), cname
     for field, _ in cls._fields_:
       assert c[(cname, field)] == getattr(cls, field).offset, (cname, field)
+
+
+def _prototypes():
+  """name -> (return type, [parameter types]) parsed from the header."""
+  text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+  out = {}
+  for m in re.finditer(r"^\s*((?:const\s+)?[A-Za-z_][\w\s\*]*?)\b(sgmc_\w+)\s*\(([^;]*?)\)\s*;",
+                       text, flags=re.M | re.S):
+    ret, name, params = m.group(1).strip(), m.group(2), m.group(3).strip()
+    types = []
+    if params and params != "void":
+      for p in params.split(","):
+        p = " ".join(p.split())
+        # drop the parameter name (the last identifier), keep the type and its stars
+        types.append(re.sub(r"\b\w+$", "", p).strip())
+    out[name] = (ret, types)
+  return out
+
+
+def _kind_c(t):
+  if "*" in t:
+    return "ptr"
+  t = t.replace("const", "").strip()
+  return {"int": "i32", "int32_t": "i32", "unsigned": "u32", "uint32_t": "u32",
+          "int64_t": "i64", "float": "f32", "double": "f64"}[t]
+
+
+def _kind_ctypes(c):
+  if c in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(c, ctypes._Pointer):
+    return "ptr"
+  return {ctypes.c_int: "i32", ctypes.c_int32: "i32", ctypes.c_uint32: "u32",
+          ctypes.c_int64: "i64", ctypes.c_float: "f32", ctypes.c_double: "f64"}[c]
+
+
+def test_binding_argument_types_match_the_header():
+  """Every ctypes signature has the header prototype's arity and, parameter by
+  parameter, the same kind (pointer, 32/64-bit integer, float, double): a mismatch
+  would pass arguments in the wrong registers without any error."""
+  L = _binding()
+  sigs = dict(L._SIGNATURES)
+  sigs.update(L.EXTRA_SIGNATURES)
+  protos = _prototypes()
+  assert set(protos) == set(header_functions())
+  bad = []
+  for name, (ret, params) in protos.items():
+    restype, argtypes = sigs[name]
+    if [_kind_c(p) for p in params] != [_kind_ctypes(a) for a in argtypes]:
+      bad.append((name, params, [a.__name__ for a in argtypes]))
+    if ret != "void" and _kind_c(ret) != _kind_ctypes(restype):
+      bad.append((name, ret, restype.__name__))
+  assert not bad, bad
