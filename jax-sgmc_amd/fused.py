@@ -12,7 +12,9 @@ host input.
 Chains may differ in dataset, N, D (<= 1024), minibatch size and temperature
 (tempering ladders); they share the family, the prior and the integrator. Per-chain
 results equal the same chain run alone (chains are independent; every chain owns
-its keys and its reduction order is fixed by its own block count).
+its keys and its reduction order is fixed by its own block count). TableGroup runs
+several tables side by side on their own streams; ReplicaExchange, AmagoldMH and
+SggmcMH attach a swap or Metropolis-Hastings step to a table.
 """
 import math
 import time
@@ -55,6 +57,10 @@ class FusedGLM:
   Optional per-chain keys: N (observations, default X.shape[0]) and save (default
   True; False leaves the chain out of the sample buffer).
   keep_lik: also keep the per-row likelihoods of the last evaluation in ``lik``.
+  mode: "sgld", "sgld_rms" (one launch per iteration), "sghmc", "amagold" (``steps``
+  launches), "obabo", "sggmc" (two launches per leapfrog step); "amagold" and "sggmc"
+  expect their Metropolis-Hastings test (AmagoldMH, SggmcMH) to be attached.
+  inv_mass / sqrt_mass: a diagonal mass shared by all chains (device float32 [D]).
   """
 
   def __init__(self, chains, family, mode, eps_table, steps=1, friction=1.0, alpha=0.9,
