@@ -367,7 +367,9 @@ def run_sggmc_mlp(args):
   keys = R.key_words(R.split(R.PRNGKey(0), C))
   dkeys = R.key_words(R.split(R.PRNGKey(2), C))
   K, W = max(1, args.steps), args.warmup
-  eps = scheduler.polynomial_table(1e-4, 1e-4, K + W + 2)
+  # a constant step size (a polynomial table with equal first and last step sizes has
+  # NaN coefficients, as the reference's schedule does)
+  eps = np.full(K + W + 2, 1e-4, np.float32)
   prec = args.precision or "fp8"
   eng = M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys, prior_scale=1.0,
                    integrator="sggmc", friction=1.0, steps=steps, precision=prec)

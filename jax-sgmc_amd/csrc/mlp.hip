@@ -132,12 +132,36 @@ struct Step {
 };
 
 // Records max |v| of a wavefront for tensor t of chain c (every lane of the wavefront
-// calls this).
+// calls this). The atomic is skipped when the recorded maximum already covers the
+// wavefront's: thousands of wavefronts updating one word otherwise serialize at the L2.
+// A stale read is never above the true maximum, so no needed update is skipped.
 __device__ __forceinline__ void q_note_amax(const Step& s, int c, int t, float a) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) a = fmaxf(a, __shfl_xor(a, m, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0 && a > 0.0f)
-    atomicMax(reinterpret_cast<unsigned*>(s.q + (int64_t)c * kQWords + 8 + t), __float_as_uint(a));
+  if ((threadIdx.x & (kWave - 1)) == 0 && a > 0.0f) {
+    unsigned* slot = reinterpret_cast<unsigned*>(s.q + (int64_t)c * kQWords + 8 + t);
+    const unsigned cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__float_as_uint(a) > cur) atomicMax(slot, __float_as_uint(a));
+  }
+}
+
+// Block-wide version (every thread of the block calls it): one check and at most one
+// atomic per block, for kernels whose wavefronts are too short to carry their own.
+__device__ __forceinline__ void q_note_amax_block(const Step& s, int c, int t, float a,
+                                                  float* red) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) a = fmaxf(a, __shfl_xor(a, m, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) b = fmaxf(b, red[i]);
+    if (b > 0.0f) {
+      unsigned* slot = reinterpret_cast<unsigned*>(s.q + (int64_t)c * kQWords + 8 + t);
+      const unsigned cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__float_as_uint(b) > cur) atomicMax(slot, __float_as_uint(b));
+    }
+  }
 }
 
 // Shadow copy of an updated weight: bf16, or e4m3 at the next weight scale (W1, W2).
@@ -750,6 +774,7 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
 // splits added in order; four consecutive columns per thread.
 template <int EPI, bool F8>
 __global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
+  __shared__ float red[GT / kWave];
   const int c = blockIdx.y;
   const int64_t e = ((int64_t)blockIdx.x * GT + threadIdx.x) * 4;
   const float* qc = F8 ? s.q + (int64_t)c * kQWords : nullptr;
@@ -782,7 +807,7 @@ __global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
       put_act(out, (int64_t)m * g.ldo + n + t, v, q8, qs);
     }
   }
-  if (q8) q_note_amax(s, c, g.tout, amax);
+  if (q8) q_note_amax_block(s, c, g.tout, amax, red);
 }
 
 // ------------------------------------------------------------------ head ------
