@@ -261,7 +261,48 @@ def run_ladder(args):
                       "kernel": "glm_fused_kernel<-1> (mixed D classes) + swap_kernel",
                       "bytes_per_iteration": bytes_it},
          "exchanges": int(ex.swaps.sum().item()), "tests": int(ex.step.sum().item())}
+  if not args.no_cpu:
+    log("CPU baseline (oracle C port per chain)")
+    out["cpu_baseline"] = ladder_cpu_baseline(probs, temps, keys, dkeys, eps)
+    out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+  else:
+    out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
+
+
+def ladder_cpu_baseline(probs, temps, keys, dkeys, eps, target_s=10.0):
+  """The oracle's C port of SGLD for every chain of the ladders, one chain per task on a
+  thread pool (ctypes releases the interpreter lock during the call). The swap tests,
+  one per pair every 50 iterations, are a negligible share of the work and are left
+  out."""
+  from concurrent.futures import ThreadPoolExecutor
+  from oracle import cport
+  threads = max(1, min(16, os.cpu_count() or 1))
+  Xs = [np.ascontiguousarray(X.cpu().numpy(), np.float32) for X, _ in probs]
+  ys = [np.ascontiguousarray(y.cpu().numpy(), np.float32) for _, y in probs]
+  chains = [(k, j) for k in range(len(probs)) for j in range(len(temps))]
+
+  def one(c, iters):
+    k, j = chains[c]
+    th = np.zeros((1, Xs[k].shape[1]), np.float32)
+    kk = np.array([keys[c]], np.uint32)
+    dd = np.array([dkeys[c]], np.uint32)
+    cport.sgld(1, Xs[k], ys[k], LADDER_BATCH, th, kk, dd, np.resize(eps, iters),
+               temperature=temps[j], threads=1)
+
+  def go(iters):
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+      list(ex.map(lambda c: one(c, iters), range(len(chains))))
+    return time.perf_counter() - t0
+
+  t1 = go(50)  # long enough that the pool's start-up does not dominate
+  iters = int(max(50, min(20000, target_s * 50 / max(t1, 1e-6))))
+  dt = go(iters)
+  return {"value": len(chains) * iters / dt, "unit": "chain-steps/s", "cores": threads,
+          "kind": "port",
+          "sample": f"all {len(chains)} chains x {iters} iterations (oracle/cport.c SGLD, one "
+                    f"chain per task on {threads} threads, {dt:.1f} s; the swap tests left out)"}
 
 
 def mlp_data(torch, R, M):
@@ -348,7 +389,38 @@ def run_mlp(args):
                     "path": "generic torch fp32 potential (vmap + autograd) with the "
                             "alias.sgld rms_prop loop"}
     out["fused_over_plain"] = value / plain
+  if not args.no_cpu:
+    log("CPU baseline (numpy oracle, one chain)")
+    out["cpu_baseline"] = mlp_cpu_baseline(X.cpu().numpy(), labels.cpu().numpy(), th0[0], eps)
+    out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+  else:
+    out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
+
+
+def mlp_cpu_baseline(X_host, labels_host, theta0, eps, target_s=10.0):
+  """The numpy oracle's pSGLD (float64 MLP forward / backward, jax-compatible noise) on
+  one chain of the same workload; numpy's BLAS threads do the matrix products."""
+  from oracle import model as om
+  from oracle import steps as ost
+  D_in, H1, H2 = MLP_DIMS
+  fam = om.Family("mlp", X_host, labels_host, dims=MLP_DIMS + (10,))
+  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
+  leaves = [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10]
+
+  def go(iters):
+    t0 = time.perf_counter()
+    ost.sgld(pot, np.asarray(theta0, np.float64), iters, eps[:iters], 1.0, MLP_N, MLP_BATCH,
+             leaves, rms=(0.9, 1e-5))
+    return time.perf_counter() - t0
+
+  t1 = go(1)
+  iters = int(max(1, min(len(eps), target_s / max(t1, 1e-6))))
+  dt = go(iters)
+  threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+  return {"value": iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
+          "sample": f"1 chain x {iters} iterations (oracle/model.py + oracle/steps.py pSGLD "
+                    f"in float64 numpy, {dt:.1f} s)"}
 
 
 def run_sggmc_mlp(args):
