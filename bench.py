@@ -478,7 +478,42 @@ def run_sggmc_mlp(args):
                       "kernel": "all kernels of one SGGMC iteration (GEMM flops)",
                       "flops_per_chain_step": flops},
          "accepted": eng.accepted.tolist()}
+  if not args.no_cpu:
+    log("CPU baseline (numpy oracle, sampled)")
+    out["cpu_baseline"] = sggmc_mlp_cpu_baseline(X.cpu().numpy(), labels.cpu().numpy(), th0[0],
+                                                 dims, n, steps)
+    out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+  else:
+    out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
+
+
+def sggmc_mlp_cpu_baseline(X_host, labels_host, theta0, dims, n, steps, sample_steps=2):
+  """The numpy oracle's SGGMC iteration on one chain, timed on a bounded sample: two
+  OBABO steps (four minibatch gradients and their noise) and one full-data potential,
+  scaled to an iteration of ``steps`` OBABO steps plus the full-data pass."""
+  from oracle import model as om
+  from oracle import steps as ost
+  D_in, H1, H2 = dims
+  fam = om.Family("mlp", X_host, labels_host, dims=tuple(dims) + (10,))
+  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
+  leaves = [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10]
+  th = np.asarray(theta0, np.float64)
+  ones = np.ones(th.size)
+  t0 = time.perf_counter()
+  ost.obabo_integrate(pot, th, np.zeros(th.size), np.array([0, 1], np.uint32),
+                      np.array([0, 2], np.uint32), np.float32(1e-4), 1.0, 1.0, sample_steps,
+                      MLP_N, n, leaves, ones, ones)
+  t_steps = time.perf_counter() - t0
+  t0 = time.perf_counter()
+  om.full_potential(fam, th, 1.0, 5000)
+  t_full = time.perf_counter() - t0
+  per_iter = t_steps * steps / sample_steps + t_full
+  threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+  return {"value": 1.0 / per_iter, "unit": "chain-steps/s", "cores": threads, "kind": "port",
+          "sample": f"1 chain: {sample_steps} OBABO steps ({t_steps:.1f} s) and one full-data "
+                    f"potential ({t_full:.1f} s) of the numpy oracle in float64, scaled to an "
+                    f"iteration of {steps} steps ({per_iter:.0f} s)"}
 
 
 def run_sgld_linreg(args):
