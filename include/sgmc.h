@@ -268,6 +268,18 @@ typedef struct sgmc_glm_chain {
 } sgmc_glm_chain;
 
 int sgmc_glm_dclass(int D);
+/* Host-side check of a chain table before it is uploaded (no reference counterpart:
+ * solver.py:162-171 runs chains one by one and has no table). chains and block_chain
+ * are HOST copies. Checks every chain's pointers, D, dclass, n, N and row alignment,
+ * that its block range [block_begin, block_begin + nblocks) lies in the grid and holds
+ * at most 4096 rows per block, that block_chain maps each range to its chain (or,
+ * without a map, that chain c owns blocks [c nblocks / nchains, ...)), and that the
+ * ranges add up to nblocks. loop != 0 checks a table for sgmc_glm_fused_loop (one
+ * block per chain, block c for chain c). sgmc_glm_fused_step and sgmc_glm_fused_loop
+ * assume a table that passed: their kernels read the map without bounds checks.
+ * Returns SGMC_E_ARG with the first violation in sgmc_last_error(). */
+int sgmc_glm_check_table(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,
+                         int nblocks, int loop);
 /* chains, block_chain (int32[nblocks], chain of each grid block), eps_table,
  * save_slot, iter and done (a zeroed unsigned) are device memory. block_chain may be
  * NULL for a uniform table: every chain then has nblocks / nchains blocks and chain c

@@ -105,6 +105,7 @@ _SIGNATURES = {
                                                      c_int, c_void_p, c_int64, c_int, c_float,
                                                      c_int, c_float, c_double, c_int, c_void_p,
                                                      c_void_p, c_int64, c_int, c_void_p]),
+    "sgmc_glm_check_table": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int]),
     "sgmc_glm_fused_loop": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_float,
                                     c_float, c_int, c_int, c_float, c_float, c_float, c_void_p,
                                     c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,

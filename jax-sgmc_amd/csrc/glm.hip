@@ -889,10 +889,9 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   __shared__ float lds_small[kWaves];
   __shared__ float lds_lik[kLdsRows];
   __shared__ int flag;
+  // The grid has one block per chain and every chain one block (sgmc_glm_check_table).
   const int chain = blockIdx.x;
-  if (chain >= a.nchains) return;
   const sgmc_glm_chain ch = a.chains[chain];
-  if (ch.nblocks != 1) return;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   FusedArgs b = a;
   b.advance = -1;
@@ -928,14 +927,14 @@ __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   __shared__ int flag;
   // A uniform table maps blocks to chains arithmetically, which saves the dependent
   // load of the map before the chain descriptor can be read.
+  // The block map and the chains' block ranges agree (sgmc_glm_check_table checks the
+  // table on the host before it is uploaded), so every block belongs to a chain.
   const int chain = a.blocks_per_chain > 0 ? (int)blockIdx.x / a.blocks_per_chain
                                            : a.block_chain[blockIdx.x];
-  if (chain < 0 || chain >= a.nchains) return;
   // The chain descriptor is copied once: block-uniform, it lives in scalar registers
   // instead of being re-read after every barrier and atomic.
   const sgmc_glm_chain ch = a.chains[chain];
   const int bchain = blockIdx.x - ch.block_begin;
-  if (bchain < 0 || bchain >= ch.nblocks) return;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   switch (dc) {
     case 0: fused_body<4, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
@@ -1076,6 +1075,60 @@ int sgmc_glm_full_potential(int family, const float* X, int64_t ldx, const float
 }
 
 int sgmc_glm_dclass(int D) { return dclass_of(D); }
+
+int sgmc_glm_check_table(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,
+                         int nblocks, int loop) {
+  SGMC_REQUIRE(chains && nchains >= 1, SGMC_E_ARG, "sgmc_glm_check_table: no chains");
+  if (loop) {
+    SGMC_REQUIRE(nblocks == nchains, SGMC_E_ARG,
+                 "sgmc_glm_check_table: a loop launch has one block per chain");
+  } else {
+    SGMC_REQUIRE(nblocks >= nchains, SGMC_E_ARG, "sgmc_glm_check_table: %d blocks for %d chains",
+                 nblocks, nchains);
+    SGMC_REQUIRE(block_chain || nblocks % nchains == 0, SGMC_E_ARG,
+                 "sgmc_glm_check_table: without a block map nblocks must be a multiple of "
+                 "nchains");
+  }
+  const int per = (!loop && !block_chain) ? nblocks / nchains : 1;
+  int64_t covered = 0;
+  for (int c = 0; c < nchains; ++c) {
+    const sgmc_glm_chain& ch = chains[c];
+    SGMC_REQUIRE(ch.X && ch.y && ch.theta && ch.key && ch.dkey && ch.U && ch.partial &&
+                     ch.counter,
+                 SGMC_E_ARG, "sgmc_glm_check_table: chain %d has a null pointer", c);
+    SGMC_REQUIRE(ch.D >= 1 && ch.D <= kMaxD && ch.dclass == dclass_of(ch.D), SGMC_E_ARG,
+                 "sgmc_glm_check_table: chain %d: D=%d or dclass %d", c, ch.D, ch.dclass);
+    SGMC_REQUIRE(ch.n >= 1 && ch.N >= 1 && ch.N <= 2147483647LL, SGMC_E_ARG,
+                 "sgmc_glm_check_table: chain %d: n=%d, N=%lld", c, ch.n, (long long)ch.N);
+    SGMC_REQUIRE(ch.ldx % 4 == 0 && ch.ldx >= 4 * ((ch.D + 3) / 4) &&
+                     ((uintptr_t)ch.X & 15) == 0,
+                 SGMC_E_ARG, "sgmc_glm_check_table: chain %d: rows not 16-byte aligned", c);
+    SGMC_REQUIRE(ch.nblocks >= 1 && ch.block_begin >= 0 &&
+                     (int64_t)ch.block_begin + ch.nblocks <= nblocks,
+                 SGMC_E_ARG, "sgmc_glm_check_table: chain %d: blocks [%d, %d) outside the grid",
+                 c, ch.block_begin, ch.block_begin + ch.nblocks);
+    // a block stages the likelihoods of its ceil(n / nblocks) rows in LDS
+    SGMC_REQUIRE((ch.n + ch.nblocks - 1) / ch.nblocks <= kLdsRows, SGMC_E_ARG,
+                 "sgmc_glm_check_table: chain %d: %d rows over %d blocks exceed %d rows per "
+                 "block", c, ch.n, ch.nblocks, kLdsRows);
+    if (loop || !block_chain) {
+      SGMC_REQUIRE(ch.nblocks == per && ch.block_begin == c * per, SGMC_E_ARG,
+                   "sgmc_glm_check_table: chain %d must own blocks [%d, %d)", c, c * per,
+                   (c + 1) * per);
+    } else {
+      for (int b = ch.block_begin; b < ch.block_begin + ch.nblocks; ++b)
+        SGMC_REQUIRE(block_chain[b] == c, SGMC_E_ARG,
+                     "sgmc_glm_check_table: block %d maps to chain %d, not %d", b,
+                     block_chain[b], c);
+    }
+    covered += ch.nblocks;
+  }
+  // Every chain's range maps to it and the ranges add up to the grid: the ranges tile
+  // the grid and every map entry names its owner.
+  SGMC_REQUIRE(covered == nblocks, SGMC_E_ARG,
+               "sgmc_glm_check_table: chains own %lld of %d blocks", (long long)covered, nblocks);
+  return SGMC_OK;
+}
 
 int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,
                         int nblocks, int uniform_dclass, int family, float sigma, int has_prior,

@@ -40,12 +40,16 @@ def _align4(x):
   return (x + 3) // 4 * 4
 
 
+# Rows per block whose likelihoods a fused block stages in LDS (kLdsRows in glm.hip).
+LDS_ROWS = 4096
+
+
 def blocks_for(n, D, target_bytes):
   """Grid blocks for one chain: about target_bytes of rows per block, at least 32
-  rows per block."""
+  rows per block, at most LDS_ROWS rows per block."""
   rows_bytes = n * 4 * _align4(D)
   nb = int(round(rows_bytes / max(1, target_bytes)))
-  return max(1, min(nb, (n + 31) // 32))
+  return max(1, -(-n // LDS_ROWS), min(nb, (n + 31) // 32))
 
 
 class FusedGLM:
@@ -209,13 +213,18 @@ class FusedGLM:
       if self.theta_old_flat is not None:
         e.theta_old = self.theta_old_flat.data_ptr() + fs * int(self.poff[i])
         e.mom_old = self.mom_old_flat.data_ptr() + fs * int(self.poff[i])
-    raw = bytes(arr)
-    self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
     bc = np.repeat(order.astype(np.int32), self.nblocks_chain[order])
-    self.block_chain = torch.as_tensor(bc, device=self.device)
     # a uniform table in chain order needs no block map (the kernel divides)
     self.uniform_blocks = bool(np.all(self.nblocks_chain == self.nblocks_chain[0]) and
                                np.array_equal(order, np.arange(self.C)))
+    # The kernels trust the table: check it on the host before it goes to the device.
+    bc_host = None if self.uniform_blocks else (ctypes.c_int32 * self.nblocks)(*bc.tolist())
+    _lib.call("sgmc_glm_check_table", arr, self.C, bc_host, self.nblocks, 0)
+    if self.loop_capable():
+      _lib.call("sgmc_glm_check_table", arr, self.C, None, self.nblocks, 1)
+    raw = bytes(arr)
+    self.table = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+    self.block_chain = torch.as_tensor(bc, device=self.device)
     self._arr = arr  # keep the host copy alive for inspection
   # ------------------------------------------------------------ launching ----
   def _launch(self, step):
