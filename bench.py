@@ -326,7 +326,7 @@ def run_mlp(args):
   L.require_gpu()
   D_in, H1, H2 = MLP_DIMS
   X, labels = mlp_data(torch, R, M)
-  C = MLP_CHAINS
+  C = args.chains or MLP_CHAINS
   th0 = np.stack([M.init_params(10 + c, D_in, H1, H2) for c in range(C)])
   keys = R.key_words(R.split(R.PRNGKey(0), C))
   dkeys = R.key_words(R.split(R.PRNGKey(2), C))
@@ -762,7 +762,7 @@ def main():
   ap.add_argument("--target-bytes", type=int, default=0,
                   help="headline: fixed bytes-per-block target instead of the autotuner "
                        "(for counter collection, which serializes launches)")
-  ap.add_argument("--precision", choices=["bf16", "fp8"], default=None,
+  ap.add_argument("--precision", choices=["fp32", "bf16", "fp8"], default=None,
                   help="GEMM operands of the MLP workloads (sggmc-mlp default fp8, psgld-mlp bf16)")
   args = ap.parse_args()
   if args.gpus != 1:

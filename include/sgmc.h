@@ -393,7 +393,7 @@ typedef struct sgmc_mlp_state {
   int32_t D_in, H1, H2, n, C;
   float* theta;            /* [C][P] */
   float* v;                /* [C][P] rms_prop second moment, or NULL */
-  uint16_t* shadow;        /* [C][P] bf16 copy of theta */
+  uint16_t* shadow;        /* [C][P] bf16 copy of theta (NULL allowed for precision 2) */
   int64_t P;
   uint32_t* key;           /* [C][2] */
   uint32_t* dkey;          /* [C][2] */
@@ -438,7 +438,9 @@ typedef struct sgmc_mlp_state {
    * scaling -- inputs, first hidden layer and the two backward activations, and the
    * W1 / W2 shadows -- each evaluation's amax setting the next scales (q[c][0..5]
    * current scales of X, h1, dA2, dA1, W1, W2; q[c][6..7] the scales of the next W1 /
-   * W2 shadows; q[c][8..13] amax bits). precision 0: bf16. */
+   * W2 shadows; q[c][8..13] amax bits). precision 0: bf16. precision 2: float32
+   * operands on the fp32 matrix instruction (v_mfma_f32_32x32x2_f32), float32
+   * activations, the weights read from theta (shadow may be NULL). */
   int32_t precision;
   uint8_t* shadow8;        /* [C][P] e4m3 copy of W1, W2 (precision 1) */
   float* q;                /* [C][16] scale words (precision 1) */

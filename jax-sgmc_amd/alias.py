@@ -10,6 +10,14 @@ data/numpy_loader.py:124), so chains started from the same sample coincide. As a
 extension, ``run_fn`` accepts ``keys=`` and ``data_keys=`` (one jax key per chain)
 and ``fused=`` (None: automatic).
 
+Execution of an MLP classifier (likelihood ``mlp.MLPSoftmax`` with a Gaussian or no
+prior, over a DeviceNumpyDataLoader, identity mass): alias.sgld, obabo and sggmc run
+every chain on the MLP engine (``mlp.FusedMLP``: hand-written forward / backward GEMMs on
+the matrix cores with the update fused in) unless ``fused=False``; ``precision=``
+selects its GEMM operands: "fp32" (default), "bf16" or "fp8". Parameters, momenta,
+noise and updates are float32 in every precision. ``fused=False`` runs the generic torch
+path (autograd over the likelihood callable).
+
 Execution: when the potential is a recognized GLM family with a Gaussian (or zero)
 prior, the data loader is device-resident, and the integrator is SGLD (with or
 without rms_prop), SGHMC or OBABO with unit mass and scalar friction, all chains run
@@ -299,9 +307,9 @@ def _mlp_eligible(potential_fn, data_loader, init_samples, batch_size, mass=None
 
 def _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations, sched,
                    integrator, keys, data_keys, to_numpy=True, rms=False, alpha=0.9, lmbd=1e-5,
-                   steps=1, friction=1.0, result_key="likelihood", precision="bf16"):
-  """alias.sgld / obabo / sggmc on the MLP engine (bf16 or fp8 GEMM operands, float32
-  state): every chain advances in the same launches."""
+                   steps=1, friction=1.0, result_key="likelihood", precision="fp32"):
+  """alias.sgld / obabo / sggmc on the MLP engine (fp32, bf16 or fp8 GEMM operands,
+  float32 state): every chain advances in the same launches."""
   from . import mlp as ML
   init_fn = sched[0]
   st, _ = init_fn(iterations)
@@ -361,8 +369,8 @@ def sgld(potential_fn, data_loader, cache_size=512, batch_size=32, first_step_si
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
-             data_keys=None, fused=None, precision="bf16"):
-    if (fused and init_model_state is None and
+             data_keys=None, fused=None, precision="fp32"):
+    if (fused is not False and init_model_state is None and
         _mlp_eligible(potential_fn, data_loader, init_samples, batch_size)):
       return _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations,
                             sched, "sgld", keys, data_keys, to_numpy=save_to_numpy,
@@ -437,8 +445,9 @@ def obabo(potential_fn, data_loader, cache_size=512, batch_size=32, integration_
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None,
-             data_keys=None, fused=None, precision="bf16"):
-    if (fused and init_model_state is None and np.ndim(friction) == 0 and
+             data_keys=None, fused=None, precision="fp32"):
+    if (fused is not False and init_model_state is None and np.ndim(friction) == 0 and
+        not isinstance(friction, (dict, list, tuple)) and
         _mlp_eligible(potential_fn, data_loader, init_samples, batch_size, mass)):
       return _run_fused_mlp(potential_fn, data_loader, batch_size, init_samples, iterations,
                             sched, "obabo", keys, data_keys, to_numpy=save_to_numpy,
@@ -519,13 +528,14 @@ def sggmc(stochastic_potential_fn, full_potential_fn, data_loader, cache_size=51
   mcmc = SV.mcmc(solver, sched, strategy="map", saving=saving)
 
   def run_fn(*init_samples, init_model_state=None, iterations=1000, keys=None, data_keys=None,
-             fused=None, precision="bf16"):
+             fused=None, precision="fp32"):
     from . import mlp as ML
-    if (fused and not adaptive_step_size and init_model_state is None and mass is None and
+    if (fused is not False and not adaptive_step_size and init_model_state is None and
+        mass is None and np.ndim(friction_coefficient) == 0 and
         isinstance(getattr(full_potential_fn, "likelihood", None), ML.MLPSoftmax) and
         _mlp_eligible(stochastic_potential_fn, data_loader, init_samples, batch_size)):
       return _run_fused_mlp(stochastic_potential_fn, data_loader, batch_size, init_samples,
-                            iterations, sched, "sggmc", keys, None, to_numpy=save_to_numpy,
+                            iterations, sched, "sggmc", keys, data_keys, to_numpy=save_to_numpy,
                             steps=integration_steps, friction=float(friction_coefficient),
                             result_key="energy", precision=precision)
     if (init_model_state is None and np.ndim(friction_coefficient) == 0 and _mh_fused_eligible(

@@ -21,7 +21,12 @@
 //   small    W3, b1, b2, b3 gradients and updates; U; key and data-key advance
 // c = -N / (n T) carries the potential's scaling into the backward pass.
 //
-// GEMM tiling (gfx950, v_mfma_f32_32x32x16_bf16): a block computes 64 rows x 128
+// Precisions: bf16 GEMM operands (v_mfma_f32_32x32x16_bf16), OCP e4m3 with per-tensor
+// scales (v_mfma_f32_32x32x16_fp8_fp8), or float32 (v_mfma_f32_32x32x2_f32: activations
+// stored in float32, weights read from theta itself, no shadow); fp32 accumulation in
+// all three.
+//
+// GEMM tiling (gfx950): a block computes 64 rows x 128
 // columns; its 64 rows are two bands of 32 rows M/2 apart, so each lane's two
 // accumulators hold the element pairs (i, i + size/2) that one Threefry block of
 // random_bits serves (integrator.py:119-135 noise). Four wavefronts own 32 columns
@@ -40,6 +45,14 @@ namespace mlp {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// GEMM operand precision (template argument) and its element bytes.
+enum Prec { P_BF16 = 0, P_FP8 = 1, P_F32 = 2 };
+template <int PREC>
+constexpr int elem_bytes() {
+  return PREC == P_FP8 ? 1 : PREC == P_BF16 ? 2 : 4;
+}
 
 constexpr int GT = 256;
 constexpr int BM = 64;
@@ -79,18 +92,19 @@ struct Step {
   int D_in, H1, H2, n, C;
   float* theta;            // [C][P]
   float* v;                // [C][P] or null (rms_prop)
-  uint16_t* shadow;        // [C][P] bf16
+  uint16_t* shadow;        // [C][P] bf16 (null for float32 GEMMs)
   int64_t P;
   int64_t off[NLEAF];      // leaf offsets
   int64_t size[NLEAF];     // leaf sizes
   uint32_t* key;           // [C][2]
   uint32_t* dkey;          // [C][2]
-  uint16_t* Xb;            // [C][n][D_in]
+  // activations: bf16; e4m3 except h2 (bf16) in fp8 mode; float32 in fp32 mode
+  void* Xb;                // [C][n][D_in]
   int32_t* lab;            // [C][n]
-  uint16_t* h1;            // [C][n][H1]
-  uint16_t* h2;            // [C][n][H2]
-  uint16_t* da2;           // [C][n][H2]
-  uint16_t* da1;           // [C][n][H1]
+  void* h1;                // [C][n][H1]
+  void* h2;                // [C][n][H2]
+  void* da2;               // [C][n][H2]
+  void* da1;               // [C][n][H1]
   float* dL;               // [C][n][10]
   float* lik;              // [C][n]
   float* prior_part;       // [C][nparts]
@@ -127,6 +141,7 @@ struct Step {
   int64_t row0;
   // fp8 GEMM operands (Xb, h1, da2, da1 and the W1 / W2 shadows in e4m3)
   int fp8;
+  int fp32;                // float32 activations and GEMM operands (weights from theta)
   float* q;                // [C][kQWords]
   uint8_t* shadow8;        // [C][P] (W1 and W2 leaves)
 };
@@ -167,6 +182,7 @@ __device__ __forceinline__ void q_note_amax_block(const Step& s, int c, int t, f
 // Shadow copy of an updated weight: bf16, or e4m3 at the next weight scale (W1, W2).
 __device__ __forceinline__ void write_shadow(const Step& s, int c, int leaf, int64_t base,
                                              int64_t i, float wn, float& wmax) {
+  if (s.fp32) return;  // the float32 GEMMs read theta itself
   if (s.fp8 && leaf < 2) {
     s.shadow8[base + i] = f2q(wn * s.q[(int64_t)c * kQWords + 6 + leaf]);
     wmax = fmaxf(wmax, fabsf(wn));
@@ -397,8 +413,15 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
       *reinterpret_cast<uint32_t*>(dst + f) = v;
     }
     q_note_amax(s, c, Q_X, amax);
+  } else if (s.fp32) {
+    float* dst = reinterpret_cast<float*>(s.Xb) + row * s.D_in;
+    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
+      float4 x = *reinterpret_cast<const float4*>(src + f);
+      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(dst + f) = x;
+    }
   } else {
-    uint16_t* dst = s.Xb + row * s.D_in;
+    uint16_t* dst = reinterpret_cast<uint16_t*>(s.Xb) + row * s.D_in;
     for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
       float4 x = *reinterpret_cast<const float4*>(src + f);
       if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -411,8 +434,8 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
 }
 
 // ------------------------------------------------------------------ GEMM ------
-// Operands are bf16, or e4m3 (F8) with per-tensor scales: element pointers are byte
-// addresses of 2- or 1-byte elements, strides are in elements.
+// Operands are bf16, e4m3 (P_FP8) with per-tensor scales, or float32 (P_F32): element
+// pointers are byte addresses of 2-, 1- or 4-byte elements, strides are in elements.
 struct Gemm {
   const void* A;
   int64_t lda, sA;   // sA: chain stride in elements
@@ -430,8 +453,8 @@ struct Gemm {
   int splits;        // split-K factor (grid z = chains x splits)
   float* part;       // EPI_PARTIAL: [splits][C][M][N] float32 partial products
   int C;
-  int ta, tb;        // F8: scale slots of A and B
-  int tout;          // F8: scale slot of an e4m3 output, or -1 for a bf16 output
+  int ta, tb;        // P_FP8: scale slots of A and B
+  int tout;          // P_FP8: scale slot of an e4m3 output, or -1 for a bf16 output
 };
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -466,6 +489,12 @@ __device__ __forceinline__ void transpose8x8(const u32x4 (&in)[8], u32x4 (&out)[
   }
 }
 
+// 4x4 transpose of 32-bit elements: in[i] holds 4 consecutive columns of row i.
+__device__ __forceinline__ void transpose8x8(const u32x4 (&in)[4], u32x4 (&out)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) out[j] = u32x4{in[0][j], in[1][j], in[2][j], in[3][j]};
+}
+
 // 8x8 transpose of bytes: in[i] holds 8 consecutive columns of row i.
 __device__ __forceinline__ void transpose8x8(const u32x2 (&in)[8], u32x2 (&out)[8]) {
 #pragma unroll
@@ -485,16 +514,19 @@ __device__ __forceinline__ void transpose8x8(const u32x2 (&in)[8], u32x2 (&out)[
 
 // Loads of one operand tile (ROWS x BK elements of EB bytes, at k0) into registers and
 // their stores to LDS. KC: the source is k-contiguous (row r at grow(r) * ld);
-// otherwise it is row-contiguous (k-row k at k * ld) and each thread moves one 8x8
-// block, transposed in registers. grow maps tile rows to source rows.
+// otherwise it is row-contiguous (k-row k at k * ld) and each thread moves one TBxTB
+// block (8x8 of 1- or 2-byte elements, 4x4 of 4-byte ones), transposed in registers.
+// grow maps tile rows to source rows (contiguous within a block).
 template <int ROWS, int BK, bool KC, int EB>
 struct Tile {
   static constexpr int EPC = 16 / EB;                 // elements per chunk
   static constexpr int CH = BK / EPC;                 // chunks per row
   static constexpr int ROWB = BK * EB;                // bytes per LDS row
-  static constexpr int NR = KC ? ROWS * CH / GT : 8;  // registers per thread
-  static constexpr int BLOCKS = ROWS * BK / 64;       // 8x8 blocks (transposed case)
-  using R = typename std::conditional<KC || EB == 2, u32x4, u32x2>::type;
+  static constexpr int TB = EB == 4 ? 4 : 8;          // transposed block edge
+  static constexpr int NR = KC ? ROWS * CH / GT : TB;  // registers per thread
+  static constexpr int BLOCKS = ROWS * BK / (TB * TB);  // blocks (transposed case)
+  using R = typename std::conditional<KC || EB >= 2, u32x4, u32x2>::type;
+  static_assert(KC || BLOCKS <= GT, "transposed tile: one block per thread at most");
   static_assert(!KC || (ROWS * CH) % GT == 0, "k-contiguous tile: whole chunks per thread");
   R r[NR];
 
@@ -514,9 +546,9 @@ struct Tile {
       }
     } else {
       if (!(BLOCKS >= GT || t < BLOCKS)) return;
-      const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
+      const int r8 = (t % (ROWS / TB)) * TB, k8 = (t / (ROWS / TB)) * TB;
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < TB; ++i)
         r[i] = *reinterpret_cast<const R*>(src + ((int64_t)(k0 + k8 + i) * ld + grow(r8)) * EB);
     }
   }
@@ -532,30 +564,51 @@ struct Tile {
       }
     } else {
       if (!(BLOCKS >= GT || t < BLOCKS)) return;
-      const int r8 = (t % (ROWS / 8)) * 8, k8 = (t / (ROWS / 8)) * 8;
-      R o[8];
+      const int r8 = (t % (ROWS / TB)) * TB, k8 = (t / (ROWS / TB)) * TB;
+      R o[TB];
       transpose8x8(r, o);
       const int kb = k8 * EB;  // byte offset of the block's k range in a row
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < TB; ++j)
         *reinterpret_cast<R*>(lds + off(r8 + j, kb >> 4) + (kb & 15)) = o[j];
     }
   }
 };
 
-// One 32x32x16 MFMA step: a and b are read at chunk/offset of rows ra and rb.
-template <int BK, bool F8>
+// k-values one mfma_step consumes: 16 (one 32x32x16 instruction per band), or 8 for
+// float32 (four 32x32x2 instructions per band).
+template <int PREC>
+constexpr int k_step() {
+  return PREC == P_F32 ? 8 : 16;
+}
+
+// One MFMA step: a and b are read at chunk/offset of rows ra and rb. bf16 / e4m3: lanes
+// 0-31 hold k .. k + 7 and lanes 32-63 k + 8 .. k + 15 of one 32x32x16 instruction.
+// float32: each lane reads one 16-byte chunk (4 k-values; lanes 32-63 the next chunk)
+// and feeds its elements to four 32x32x2 instructions, instruction u taking k + u from
+// lanes 0-31 and k + 4 + u from lanes 32-63 (the same k for A and B, so every product
+// of the 8 k-values enters once).
+template <int BK, int PREC>
 __device__ __forceinline__ void mfma_step(const uint8_t* As, const uint8_t* Bs, int kk, int lane,
                                           int w, f32x16& acc0, f32x16& acc1) {
-  constexpr int EB = F8 ? 1 : 2;
+  constexpr int EB = elem_bytes<PREC>();
   using TA = Tile<BM, BK, true, EB>;
-  const int k = kk + 8 * (lane >> 5);   // this lane's 8 consecutive k
+  const int k = kk + (k_step<PREC>() / 2) * (lane >> 5);   // this lane's first k
   const int kb = k * EB;
   const int ra0 = lane & 31, ra1 = 32 + (lane & 31), rb0 = w * 32 + (lane & 31);
   const int o0 = TA::off(ra0, kb >> 4) + (kb & 15);
   const int o1 = TA::off(ra1, kb >> 4) + (kb & 15);
   const int ob = TA::off(rb0, kb >> 4) + (kb & 15);
-  if constexpr (F8) {
+  if constexpr (PREC == P_F32) {
+    const f32x4 a0 = *reinterpret_cast<const f32x4*>(As + o0);
+    const f32x4 a1 = *reinterpret_cast<const f32x4*>(As + o1);
+    const f32x4 b = *reinterpret_cast<const f32x4*>(Bs + ob);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u], b[u], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u], b[u], acc1, 0, 0, 0);
+    }
+  } else if constexpr (PREC == P_FP8) {
     const long a0 = *reinterpret_cast<const long*>(As + o0);
     const long a1 = *reinterpret_cast<const long*>(As + o1);
     const long b = *reinterpret_cast<const long*>(Bs + ob);
@@ -570,10 +623,13 @@ __device__ __forceinline__ void mfma_step(const uint8_t* As, const uint8_t* Bs, 
   }
 }
 
-// ReLU mask of an activation element (bf16 or e4m3): positive and nonzero.
-template <bool F8>
+// ReLU mask of an activation element (bf16, e4m3 or float32): positive and nonzero.
+template <int PREC>
 __device__ __forceinline__ bool mask_on(const void* m, int64_t i) {
-  if constexpr (F8) {
+  if constexpr (PREC == P_F32) {
+    const uint32_t u = __float_as_uint(reinterpret_cast<const float*>(m)[i]);
+    return u != 0 && !(u & 0x80000000u);
+  } else if constexpr (PREC == P_FP8) {
     const uint8_t b = reinterpret_cast<const uint8_t*>(m)[i];
     return b != 0 && !(b & 0x80u);
   } else {
@@ -582,22 +638,39 @@ __device__ __forceinline__ bool mask_on(const void* m, int64_t i) {
   }
 }
 
-// Writes an activation: bf16, or e4m3 at scale qs (F8 with an e4m3 output).
-__device__ __forceinline__ void put_act(void* out, int64_t i, float v, bool q8, float qs) {
-  if (q8) reinterpret_cast<uint8_t*>(out)[i] = f2q(v * qs);
+// Writes an activation: float32 (f32), e4m3 at scale qs (q8), or bf16.
+__device__ __forceinline__ void put_act(void* out, int64_t i, float v, bool q8, float qs,
+                                        bool f32) {
+  if (f32) reinterpret_cast<float*>(out)[i] = v;
+  else if (q8) reinterpret_cast<uint8_t*>(out)[i] = f2q(v * qs);
   else reinterpret_cast<uint16_t*>(out)[i] = f2bf(v);
+}
+
+// Bytes per element of an activation tensor written with (q8, f32).
+__device__ __forceinline__ int act_bytes(bool q8, bool f32) { return f32 ? 4 : q8 ? 1 : 2; }
+
+// h2 (bf16, or float32 in fp32 mode) and its ReLU mask.
+__device__ __forceinline__ float h2_at(const Step& s, int64_t i) {
+  if (s.fp32) return reinterpret_cast<const float*>(s.h2)[i];
+  return bf2f(reinterpret_cast<const uint16_t*>(s.h2)[i]);
+}
+__device__ __forceinline__ bool h2_on(const Step& s, int64_t i) {
+  if (s.fp32) return mask_on<P_F32>(s.h2, i);
+  return mask_on<P_BF16>(s.h2, i);
 }
 
 // Reads an activation of tensor t (e4m3 tensors in fp8 mode, bf16 otherwise).
 __device__ __forceinline__ float get_act(const Step& s, int c, int t, const void* p, int64_t i) {
+  if (s.fp32) return reinterpret_cast<const float*>(p)[i];
   if (s.fp8) return q2f(reinterpret_cast<const uint8_t*>(p)[i]) / s.q[(int64_t)c * kQWords + t];
   return bf2f(reinterpret_cast<const uint16_t*>(p)[i]);
 }
 
-template <int BK, bool F8, bool A_K, bool B_K, int EPI>
+template <int BK, int PREC, bool A_K, bool B_K, int EPI>
 __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
   static_assert(BN == 2 * BM, "tile shapes");
-  constexpr int EB = F8 ? 1 : 2;
+  constexpr bool F8 = PREC == P_FP8;
+  constexpr int EB = elem_bytes<PREC>();
   __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
   __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
   __shared__ float red[GT / kWave];
@@ -632,7 +705,7 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
       tb.load(B, g.ldb, k0 + BK, growB);
     }
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 16) mfma_step<BK, F8>(As, Bs, kk, lane, w, acc0, acc1);
+    for (int kk = 0; kk < BK; kk += k_step<PREC>()) mfma_step<BK, PREC>(As, Bs, kk, lane, w, acc0, acc1);
   }
 
   // F8: products carry the operand scales; ab undoes them
@@ -653,8 +726,9 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     }
   } else if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
     const bool q8 = F8 && g.tout >= 0;
+    const bool f32 = PREC == P_F32;
     const float qs = q8 ? qc[g.tout] : 1.0f;
-    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * (q8 ? 1 : 2);
+    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * act_bytes(q8, f32);
     const float bias = (EPI == EPI_BIAS_RELU) ? g.bias[(int64_t)c * g.sBias + n] : 0.f;
     float amax = 0.f;
 #pragma unroll
@@ -668,10 +742,10 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
         if (EPI == EPI_BIAS_RELU) {
           vv = fmaxf(a + bias, 0.0f);
         } else {
-          vv = mask_on<F8>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n) ? a : 0.0f;
+          vv = mask_on<PREC>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n) ? a : 0.0f;
         }
         amax = fmaxf(amax, fabsf(vv));
-        put_act(out, (int64_t)m * g.ldo + n, vv, q8, qs);
+        put_act(out, (int64_t)m * g.ldo + n, vv, q8, qs, f32);
       }
     }
     if (q8) q_note_amax(s, c, g.tout, amax);
@@ -772,8 +846,10 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
 
 // Split-K finish: out = epilogue(ab * sum over splits of the partial products), the
 // splits added in order; four consecutive columns per thread.
-template <int EPI, bool F8>
+template <int EPI, int PREC>
 __global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
+  constexpr bool F8 = PREC == P_FP8;
+  constexpr bool f32 = PREC == P_F32;
   __shared__ float red[GT / kWave];
   const int c = blockIdx.y;
   const int64_t e = ((int64_t)blockIdx.x * GT + threadIdx.x) * 4;
@@ -794,17 +870,17 @@ __global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
       a.w += v.w;
     }
     const float r[4] = {ab * a.x, ab * a.y, ab * a.z, ab * a.w};
-    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * (q8 ? 1 : 2);
+    void* out = reinterpret_cast<uint8_t*>(g.out) + (int64_t)c * g.sO * act_bytes(q8, f32);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       float v;
       if (EPI == EPI_BIAS_RELU) {
         v = fmaxf(r[t] + g.bias[(int64_t)c * g.sBias + n + t], 0.0f);
       } else {
-        v = mask_on<F8>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n + t) ? r[t] : 0.0f;
+        v = mask_on<PREC>(g.mask, (int64_t)c * g.sM + (int64_t)m * g.ldm + n + t) ? r[t] : 0.0f;
       }
       amax = fmaxf(amax, fabsf(v));
-      put_act(out, (int64_t)m * g.ldo + n + t, v, q8, qs);
+      put_act(out, (int64_t)m * g.ldo + n + t, v, q8, qs, f32);
     }
   }
   if (q8) q_note_amax_block(s, c, g.tout, amax, red);
@@ -821,12 +897,12 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
   const float* th = s.theta + (int64_t)c * s.P;
   const float* W3 = th + s.off[2];
   const float* b3 = th + s.off[5];
-  const uint16_t* h2 = s.h2 + ((int64_t)c * s.n + j) * s.H2;
+  const int64_t h2row = ((int64_t)c * s.n + j) * s.H2;
   float z[NCLS];
 #pragma unroll
   for (int k = 0; k < NCLS; ++k) z[k] = 0.f;
   for (int a = lane; a < s.H2; a += kWave) {
-    const float h = bf2f(h2[a]);
+    const float h = h2_at(s, h2row + a);
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) z[k] += h * W3[(int64_t)a * NCLS + k];
   }
@@ -871,10 +947,9 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
     float d = 0.f;
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) d += dl[k] * W3[(int64_t)a * NCLS + k];
-    const uint16_t hb = h2[a];
-    const float v = (hb != 0 && !(hb & 0x8000u)) ? d : 0.0f;
+    const float v = h2_on(s, h2row + a) ? d : 0.0f;
     amax = fmaxf(amax, fabsf(v));
-    put_act(s.da2, row * s.H2 + a, v, s.fp8 != 0, qs);
+    put_act(s.da2, row * s.H2 + a, v, s.fp8 != 0, qs, s.fp32 != 0);
   }
   if (s.fp8) q_note_amax(s, c, Q_DA2, amax);
 }
@@ -900,7 +975,7 @@ __global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
     float b2 = 0.f;
 #pragma unroll 4
     for (int r = 0; r < kStrip; ++r) {
-      const float h = bf2f(s.h2[(r0 + r) * s.H2 + a]);
+      const float h = h2_at(s, (r0 + r) * s.H2 + a);
       b2 += get_act(s, c, Q_DA2, s.da2, (r0 + r) * s.H2 + a);
       const float* dl = s.dL + (r0 + r) * NCLS;
 #pragma unroll
@@ -1196,7 +1271,7 @@ __global__ void __launch_bounds__(GT) mh_kernel(Step s, MH m) {
   float* mo = s.mom + (int64_t)c * s.P;
   float* tho = m.theta_old + (int64_t)c * s.P;
   float* moo = m.mom_old + (int64_t)c * s.P;
-  uint16_t* sh = s.shadow + (int64_t)c * s.P;
+  uint16_t* sh = s.fp32 ? nullptr : s.shadow + (int64_t)c * s.P;
   float* smp = save ? s.samples + ((int64_t)c * s.n_slots + slot) * s.P : nullptr;
   for (int64_t i = (int64_t)blockIdx.x * GT + threadIdx.x; i < s.P;
        i += (int64_t)gridDim.x * GT) {
@@ -1212,7 +1287,7 @@ __global__ void __launch_bounds__(GT) mh_kernel(Step s, MH m) {
       if (s.fp8 && i < s.off[2]) {
         const int l = i < s.off[1] ? 0 : 1;
         s.shadow8[(int64_t)c * s.P + i] = f2q(w * s.q[(int64_t)c * kQWords + Q_W1 + l]);
-      } else {
+      } else if (!s.fp32) {
         sh[i] = f2bf(w);
       }
     }
@@ -1262,12 +1337,13 @@ int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 Layout layout_of(int C, int n, int D_in, int H1, int H2) {
   Layout L;
   int64_t o = 0;
-  L.xb = o; o = align256(o + (int64_t)C * n * D_in * 2);
+  // activations sized for float32 elements (the widest precision)
+  L.xb = o; o = align256(o + (int64_t)C * n * D_in * 4);
   L.lab = o; o = align256(o + (int64_t)C * n * 4);
-  L.h1 = o; o = align256(o + (int64_t)C * n * H1 * 2);
-  L.h2 = o; o = align256(o + (int64_t)C * n * H2 * 2);
-  L.da2 = o; o = align256(o + (int64_t)C * n * H2 * 2);
-  L.da1 = o; o = align256(o + (int64_t)C * n * H1 * 2);
+  L.h1 = o; o = align256(o + (int64_t)C * n * H1 * 4);
+  L.h2 = o; o = align256(o + (int64_t)C * n * H2 * 4);
+  L.da2 = o; o = align256(o + (int64_t)C * n * H2 * 4);
+  L.da1 = o; o = align256(o + (int64_t)C * n * H1 * 4);
   L.dl = o; o = align256(o + (int64_t)C * n * NCLS * 4);
   L.lik = o; o = align256(o + (int64_t)C * n * 4);
   L.g4_blocks = (H1 / BM) * (H2 / BN);
@@ -1314,7 +1390,7 @@ namespace {
 // whole tile of MFMA work to land behind; 32 otherwise (K is a multiple of 32).
 constexpr int kMaxSplits = 2;
 
-template <int BK, bool F8, bool A_K, bool B_K, int EPI>
+template <int BK, int PREC, bool A_K, bool B_K, int EPI>
 void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part) {
   g.C = (int)grid.z;
   g.splits = 1;
@@ -1328,17 +1404,18 @@ void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* par
   }
   if (g.splits > 1) {
     g.part = part;
-    hipLaunchKernelGGL((gemm_kernel<BK, F8, A_K, B_K, EPI_PARTIAL>),
+    hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI_PARTIAL>),
                        dim3(grid.x, grid.y, grid.z * g.splits), dim3(GT), 0, st, g, s);
     const int64_t quads = (int64_t)g.M * g.N / 4;
-    hipLaunchKernelGGL((splitk_finish_kernel<EPI, F8>),
+    hipLaunchKernelGGL((splitk_finish_kernel<EPI, PREC>),
                        dim3((unsigned)((quads + GT - 1) / GT), g.C), dim3(GT), 0, st, g, s);
   } else {
-    hipLaunchKernelGGL((gemm_kernel<BK, F8, A_K, B_K, EPI>), grid, dim3(GT), 0, st, g, s);
+    hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI>), grid, dim3(GT), 0, st, g, s);
   }
 }
 
-// bf16: k-tiles of 128 (or 32); e4m3: 128 (or 64). K is a multiple of 64. The bf16
+// bf16: k-tiles of 128 (or 32); e4m3: 128 (or 64); float32: 32 (the transposed tiles of
+// 4-byte elements fit one 4x4 block per thread). K is a multiple of 64. The bf16
 // parameter-update GEMMs take 32-deep k-tiles: their short K (the minibatch) gains
 // little from deep tiles, and the smaller staging registers give three waves per SIMD
 // instead of two to the update epilogue (C3 MLP: +3.4%). The k order of the matrix
@@ -1346,12 +1423,14 @@ void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* par
 template <bool A_K, bool B_K, int EPI>
 void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st) {
   const bool deep = g.K % 128 == 0 && (s.fp8 || EPI != EPI_UPDATE);
-  if (s.fp8) {
-    if (deep) launch_gemm_bk<128, true, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
-    else launch_gemm_bk<64, true, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  if (s.fp32) {
+    launch_gemm_bk<32, P_F32, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+  } else if (s.fp8) {
+    if (deep) launch_gemm_bk<128, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    else launch_gemm_bk<64, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
   } else {
-    if (deep) launch_gemm_bk<128, false, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
-    else launch_gemm_bk<32, false, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    if (deep) launch_gemm_bk<128, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    else launch_gemm_bk<32, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
   }
 }
 
@@ -1367,7 +1446,10 @@ Gemm with_q(Gemm g, int ta, int tb, int tout) {
 int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   SGMC_REQUIRE(st, SGMC_E_ARG, "sgmc_mlp: null state");
   const sgmc_mlp_state& q = *st;
-  SGMC_REQUIRE(q.X && q.labels && q.theta && q.shadow && q.key && q.dkey && q.work && q.U &&
+  SGMC_REQUIRE(q.precision >= 0 && q.precision <= 2, SGMC_E_ARG,
+               "sgmc_mlp: precision %d is not 0 (bf16), 1 (fp8) or 2 (fp32)", q.precision);
+  SGMC_REQUIRE(q.X && q.labels && q.theta && (q.shadow || q.precision == 2) && q.key &&
+                   q.dkey && q.work && q.U &&
                    q.eps_table && q.iter && q.counters && q.done,
                SGMC_E_ARG, "sgmc_mlp: null pointer in state");
   SGMC_REQUIRE(update || q.grad, SGMC_E_ARG, "sgmc_mlp_step: gradient mode needs grad");
@@ -1403,12 +1485,12 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
     o += sizes[l];
   }
   s.key = q.key; s.dkey = q.dkey;
-  s.Xb = reinterpret_cast<uint16_t*>(w + L.xb);
+  s.Xb = w + L.xb;
   s.lab = reinterpret_cast<int32_t*>(w + L.lab);
-  s.h1 = reinterpret_cast<uint16_t*>(w + L.h1);
-  s.h2 = reinterpret_cast<uint16_t*>(w + L.h2);
-  s.da2 = reinterpret_cast<uint16_t*>(w + L.da2);
-  s.da1 = reinterpret_cast<uint16_t*>(w + L.da1);
+  s.h1 = w + L.h1;
+  s.h2 = w + L.h2;
+  s.da2 = w + L.da2;
+  s.da1 = w + L.da1;
   s.dL = reinterpret_cast<float*>(w + L.dl);
   s.lik = reinterpret_cast<float*>(w + L.lik);
   s.prior_part = reinterpret_cast<float*>(w + L.prior);
@@ -1435,6 +1517,7 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   s.full = 0;
   s.row0 = 0;
   s.fp8 = q.precision == 1 ? 1 : 0;
+  s.fp32 = q.precision == 2 ? 1 : 0;
   s.q = q.q;
   s.shadow8 = q.shadow8;
   if (s.fp8)
@@ -1442,13 +1525,20 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   return SGMC_OK;
 }
 
+// GEMM operand of weight leaf l (W1, W2): the e4m3 or bf16 shadow, or theta itself.
+const void* gemm_weight(const Step& s, const sgmc_mlp_state& q, int l) {
+  if (s.fp32) return q.theta + s.off[l];
+  if (s.fp8) return q.shadow8 + s.off[l];
+  return q.shadow + s.off[l];
+}
+
 // gather, forward GEMMs and the head (likelihood, and its backward unless s.full).
 void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   const int C = q.C, n = q.n;
   const dim3 rows_grid((n + 3) / 4, C);
   hipLaunchKernelGGL(gather_kernel, rows_grid, dim3(GT), 0, st_, s);
-  const void* W1 = s.fp8 ? (const void*)(q.shadow8 + s.off[0]) : (const void*)(q.shadow + s.off[0]);
-  const void* W2 = s.fp8 ? (const void*)(q.shadow8 + s.off[1]) : (const void*)(q.shadow + s.off[1]);
+  const void* W1 = gemm_weight(s, q, 0);
+  const void* W2 = gemm_weight(s, q, 1);
   Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, W1, q.H1, q.P, n, q.H1, q.D_in,
           s.h1, q.H1, (int64_t)n * q.H1, q.theta + s.off[3], q.P, nullptr, 0, 0, 0, 0};
   launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H1 / BN, C), with_q(g1, Q_X, Q_W1, Q_H1),
@@ -1473,7 +1563,7 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   hipStream_t st_ = as_stream(stream);
   const int C = q.C, n = q.n;
   launch_forward(s, q, st_);
-  const void* W2 = s.fp8 ? (const void*)(q.shadow8 + s.off[1]) : (const void*)(q.shadow + s.off[1]);
+  const void* W2 = gemm_weight(s, q, 1);
   // dA1 = (dA2 W2^T) * (H1 > 0): B[k=b][n=a] = W2[a][b], k-contiguous rows of W2.
   Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, W2, q.H2, q.P, n, q.H1, q.H2,
           s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};

@@ -16,6 +16,7 @@ its keys and its reduction order is fixed by its own block count). TableGroup ru
 several tables side by side on their own streams; ReplicaExchange, AmagoldMH and
 SggmcMH attach a swap or Metropolis-Hastings step to a table.
 """
+import ctypes
 import math
 import time
 

@@ -5,8 +5,9 @@ cross-entropy likelihood, Gaussian prior on every parameter. Parameters are a di
 {"W1","W2","W3","b1","b2","b3"} whose tree_flatten (sorted-key) order is W1, W2, W3,
 b1, b2, b3; one flat float32 vector per chain in that order.
 
-``FusedMLP`` advances C chains per call of ``sgmc_mlp_step`` (index draw, gather, bf16
-MFMA forward and hand-written backward, update fused into the weight-gradient GEMMs);
+``FusedMLP`` advances C chains per call of ``sgmc_mlp_step`` (index draw, gather, MFMA
+forward and hand-written backward in bf16, e4m3 or float32, update fused into the
+weight-gradient GEMMs);
 ``MLPPotential`` exposes the same kernels as a torch.autograd.Function (forward: U,
 backward: the kernel's gradient).
 """
@@ -20,6 +21,8 @@ from . import _lib
 from . import tree as T
 
 NCLS = 10
+# sgmc_mlp_state.precision codes
+PRECISIONS = {"bf16": 0, "fp8": 1, "fp32": 2}
 
 
 MlpState = _lib.MlpState
@@ -68,8 +71,10 @@ class FusedMLP:
   SGGMC (``integrator="sggmc"``: OBABO proposals with the full-data FP64 MH test).
 
   For SGGMC the keys are the solver's (split into the integrator key and the MH key
-  as solver.py:478-487 does). precision "fp8" runs the GEMMs on e4m3 operands with
-  per-tensor delayed scaling (state and updates stay float32)."""
+  as solver.py:478-487 does). precision "fp32" runs the GEMMs on float32 operands
+  (the fp32 matrix instruction, activations in float32, weights read from theta);
+  "bf16" on bf16 operands and a bf16 shadow of the parameters; "fp8" on e4m3 operands
+  with per-tensor delayed scaling. State, noise and updates are float32 in all three."""
 
   def __init__(self, X, labels, dims, n, theta0, eps_table, keys=None, dkeys=None, rms=False,
                alpha=0.9, lmbd=1e-5, prior_scale=1.0, has_prior=True, temperature=1.0,
@@ -93,10 +98,11 @@ class FusedMLP:
     assert th.shape[1] == self.P
     self.theta = torch.as_tensor(th, device=dev).contiguous()
     self.v = torch.ones_like(self.theta) if rms else None
-    self.shadow = torch.empty(self.theta.shape, dtype=torch.int16, device=dev)
-    if precision not in ("bf16", "fp8"):
-      raise ValueError("precision must be 'bf16' or 'fp8'")
+    if precision not in PRECISIONS:
+      raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
     self.precision = precision
+    self.shadow = (None if precision == "fp32" else
+                   torch.empty(self.theta.shape, dtype=torch.int16, device=dev))
     self.shadow8 = self.q = None
     if precision == "fp8":
       self.shadow8 = torch.zeros(self.theta.shape, dtype=torch.uint8, device=dev)
@@ -168,7 +174,10 @@ class FusedMLP:
 
   def refresh_shadow(self):
     """Rewrites the GEMM copy of the parameters after theta changed outside the
-    engine (bf16 shadow, or the e4m3 shadows and their scales)."""
+    engine (bf16 shadow, or the e4m3 shadows and their scales; float32 GEMMs read
+    theta itself)."""
+    if self.precision == "fp32":
+      return
     _lib.call("sgmc_mlp_shadow", _lib.ptr(self.theta), _lib.ptr(self.shadow),
               self.theta.numel(), _lib.stream())
     if self.precision == "fp8":
@@ -193,7 +202,7 @@ class FusedMLP:
     st.D_in, st.H1, st.H2, st.n, st.C = D_in, H1, H2, self.n, self.C
     st.theta = self.theta.data_ptr()
     st.v = self.v.data_ptr() if self.v is not None else None
-    st.shadow = self.shadow.data_ptr()
+    st.shadow = self.shadow.data_ptr() if self.shadow is not None else None
     st.P = self.P
     st.key, st.dkey = self.keys.data_ptr(), self.dkeys.data_ptr()
     st.work, st.work_bytes = self.work.data_ptr(), self.work.numel()
@@ -216,7 +225,7 @@ class FusedMLP:
     st.theta_old, st.mom_old, st.U_old = opt(self.theta_old), opt(self.mom_old), opt(self.U_old)
     st.mh_key, st.mh_stats, st.accepted = opt(self.mh_key), opt(self.mh_stats), opt(self.accepted)
     st.mode = 0
-    st.precision = 1 if self.precision == "fp8" else 0
+    st.precision = PRECISIONS[self.precision]
     st.shadow8, st.q = opt(self.shadow8), opt(self.q)
     return st
 

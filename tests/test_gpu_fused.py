@@ -87,6 +87,39 @@ def test_fused_sgld_linear_regression():
             pscale=np.sqrt(10.0))
 
 
+def test_fused_sgld_linear_regression_full_size():
+  """The SGLD linear-regression workload at its stated size: N = 10,000, D = 32,
+  n = 128, one chain, the polynomial schedule from 1e-4 to 1e-5 over 10,000 iterations
+  (its first 100 entries), data from jax-compatible draws of PRNGKey(0). The first
+  step sizes exceed 2 / lambda_max (eps lambda_max = 4.5), so the chain first grows
+  geometrically, as the reference's would; every iteration stays within 1e-5 of the
+  float64 oracle relative to the iterate's scale."""
+  print("fused sgld linear N=10000 D=32 n=128", flush=True)
+  from oracle import jaxrandom as jr
+  N, D, n, iters = 10_000, 32, 128, 100
+  kx, kw, ke = jr.split(jr.PRNGKey(0), 3)
+  X = jr.normal(kx, (N * D,)).reshape(N, D)
+  w = jr.normal(kw, (D,))
+  y = (X @ w + np.float32(0.5) * jr.normal(ke, (N,))).astype(np.float32)
+  eps = ost.polynomial_step_size_first_last(1e-4, 1e-5, 0.33, 10_000)[:iters]
+  ch = [dict(X=_dev(X), y=torch.as_tensor(y, device=DEV), D=D, n=n,
+             theta0=np.zeros(D, np.float32), key=(0, 0), dkey=(0, 0))]
+  eng = fused.FusedGLM(ch, "linear_gaussian", "sgld", eps, sigma=0.5,
+                       prior_scale=float(np.sqrt(10.0)), save_slots=np.arange(iters))
+  assert eng.loop_capable()
+  eng.run(iters)
+  torch.cuda.synchronize()
+  fam = om.Family("linear_gaussian", X, y, sigma=0.5)
+  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, np.sqrt(10.0))
+  r = ost.sgld(pot, np.zeros(D), iters, eps, 1.0, N, n, [D], key=np.zeros(2, np.uint32),
+               dkey=np.zeros(2, np.uint32))
+  tr = eng.sample_trace(0).cpu().numpy()
+  for t in range(iters):
+    _check(tr[t], r["theta"][t], f"iteration {t}")
+  np.testing.assert_array_equal(R.key_words(eng.keys[0]), r["key"])
+  np.testing.assert_array_equal(R.key_words(eng.dkeys[0]), r["dkey"])
+
+
 def test_fused_sgld_rms_logistic():
   print("fused sgld rms logistic", flush=True)
   eps = np.full(20, 1e-3, np.float32)
