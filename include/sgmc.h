@@ -275,7 +275,7 @@ int sgmc_glm_dclass(int D);
  * at most 4096 rows per block, that block_chain maps each range to its chain (or,
  * without a map, that chain c owns blocks [c nblocks / nchains, ...)), and that the
  * ranges add up to nblocks. loop != 0 checks a table for sgmc_glm_fused_loop (one
- * block per chain, block c for chain c). sgmc_glm_fused_step and sgmc_glm_fused_loop
+ * block per chain; the loop kernel runs chain c on block c and ignores block_begin). sgmc_glm_fused_step and sgmc_glm_fused_loop
  * assume a table that passed: their kernels read the map without bounds checks.
  * Returns SGMC_E_ARG with the first violation in sgmc_last_error(). */
 int sgmc_glm_check_table(const sgmc_glm_chain* chains, int nchains, const int32_t* block_chain,

@@ -1089,7 +1089,7 @@ int sgmc_glm_check_table(const sgmc_glm_chain* chains, int nchains, const int32_
                  "sgmc_glm_check_table: without a block map nblocks must be a multiple of "
                  "nchains");
   }
-  const int per = (!loop && !block_chain) ? nblocks / nchains : 1;
+  const int per = !block_chain ? nblocks / nchains : 1;
   int64_t covered = 0;
   for (int c = 0; c < nchains; ++c) {
     const sgmc_glm_chain& ch = chains[c];
@@ -1111,7 +1111,12 @@ int sgmc_glm_check_table(const sgmc_glm_chain* chains, int nchains, const int32_
     SGMC_REQUIRE((ch.n + ch.nblocks - 1) / ch.nblocks <= kLdsRows, SGMC_E_ARG,
                  "sgmc_glm_check_table: chain %d: %d rows over %d blocks exceed %d rows per "
                  "block", c, ch.n, ch.nblocks, kLdsRows);
-    if (loop || !block_chain) {
+    if (loop) {
+      // the loop launch runs chain c on block c (block_begin is not read)
+      SGMC_REQUIRE(ch.nblocks == 1, SGMC_E_ARG,
+                   "sgmc_glm_check_table: chain %d has %d blocks; a loop launch needs one", c,
+                   ch.nblocks);
+    } else if (!block_chain) {
       SGMC_REQUIRE(ch.nblocks == per && ch.block_begin == c * per, SGMC_E_ARG,
                    "sgmc_glm_check_table: chain %d must own blocks [%d, %d)", c, c * per,
                    (c + 1) * per);

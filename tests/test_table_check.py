@@ -50,6 +50,8 @@ def test_well_formed_tables_pass():
   arr = table([1, 3], begins=[3, 0])
   assert check(arr, 2, bmap([1, 1, 1, 0]), 4) == 0
   assert check(table([1, 1, 1, 1]), 4, None, 4, loop=1) == 0
+  # a loop launch ignores block_begin (FusedGLM orders blocks by bytes per block)
+  assert check(table([1, 1, 1], begins=[2, 0, 1]), 3, None, 3, loop=1) == 0
 
 
 @pytest.mark.parametrize("case", ["wrong_owner", "outside_grid", "gap", "uniform_begin",
