@@ -162,6 +162,34 @@ class StochasticPotential:
     return (U, aux), g
 
 
+  def differentiable(self, w, reference_data):
+    """U(w) of a recognized GLM family on one minibatch as a differentiable torch value:
+    the forward launch of sgmc_glm_potential also writes the gradient, and the backward
+    returns grad_output times that gradient (the hand-written backward in glm.hip), so
+    ``U.backward()`` launches nothing more. ``w`` is the weight vector (one leaf)."""
+    if not self._fast_ok(w, reference_data, None, None):
+      raise ValueError("differentiable() needs a GLM family potential (GaussianPrior or "
+                       "ZeroPrior), a float32 device weight vector and a device batch with "
+                       "16-byte aligned rows, D <= 1024")
+    return GLMPotentialFn.apply(w, self, reference_data)
+
+
+class GLMPotentialFn(torch.autograd.Function):
+  """Minibatch potential of a GLM family (potential.py:159-214) whose backward is the
+  kernel's gradient (jax.value_and_grad at integrator.py:166, 593, 792)."""
+
+  @staticmethod
+  def forward(ctx, w, potential, reference_data):
+    U, g, _, _ = potential._fast(w.detach(), reference_data, False)
+    ctx.save_for_backward(g)
+    return U
+
+  @staticmethod
+  def backward(ctx, grad_U):
+    (g,) = ctx.saved_tensors
+    return grad_U * g, None, None
+
+
 def minibatch_potential(prior, likelihood, strategy="map", has_state=False, is_batched=False,
                         temperature=1.0):
   """potential.py:94-216."""
