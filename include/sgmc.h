@@ -101,6 +101,21 @@ int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
                        float* out_U, float* out_grad, float* out_lik, float* out_var,
                        float* workspace, int64_t workspace_floats, int nblocks, void* stream);
 
+/* sgmc_glm_potential without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). It has no counterpart in the reference, whose potentials are jax
+ * functions; the arguments and results are sgmc_glm_potential's. The block records
+ * and the arrival counter live in a static 1 MiB device array per GPU, so nblocks *
+ * (4 ceil(D/4) + 2) must not exceed 262,080 floats (D = 1024: 255 blocks; D = 512: 509);
+ * larger calls return SGMC_E_ARG. No allocation happens (the first call on a device
+ * looks the array up and zeroes its counter on `stream`). Calls on one GPU share the
+ * array: calls that overlap in time on one GPU (other streams, other threads) race. */
+int sgmc_glm_potential_noalloc(int family, const float* X, int64_t ldx, const float* y,
+                               const int32_t* idx, int n, int64_t N, int D, const float* theta,
+                               float sigma, int has_prior, float prior_scale, float temperature,
+                               float* out_U, float* out_grad, float* out_lik, float* out_var,
+                               int nblocks, void* stream);
+
 /* full_potential (potential.py:219-293) over all N rows with FP64 accumulation of the
  * per-row likelihoods and the prior; out_U is one double. workspace:
  * sgmc_glm_full_workspace_doubles(N, rows_per_block) doubles. */

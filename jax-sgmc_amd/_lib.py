@@ -92,6 +92,10 @@ _SIGNATURES = {
                                    c_int, c_void_p, c_float, c_int, c_float, c_float, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                    c_void_p]),
+    "sgmc_glm_potential_noalloc": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p, c_int,
+                                           c_int64, c_int, c_void_p, c_float, c_int, c_float,
+                                           c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                           c_void_p]),
     "sgmc_glm_full_workspace_doubles": (c_int64, [c_int64, c_int]),
     "sgmc_glm_full_potential": (c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int,
                                         c_void_p, c_float, c_int, c_float, c_double, c_void_p,

@@ -20,6 +20,8 @@
 // block to arrive (deterministic order).
 #include <math.h>
 
+#include <mutex>
+
 #include "glm_common.h"
 #include "sgmc_common.h"
 #include "sgmc_device.h"
@@ -957,6 +959,107 @@ static int dclass_of(int D) {
 
 using namespace sgmc;
 
+namespace {
+
+// Validates the arguments of a minibatch potential and launches its kernel with the
+// block records at `partial` ([nblocks][4 ceil(D/4) + 2] floats) and the arrival counter
+// at `counter` (zero before the launch; the last block resets it).
+int launch_potential(const char* who, int family, const float* X, int64_t ldx, const float* y,
+                     const int32_t* idx, int n, int64_t N, int D, const float* theta,
+                     float sigma, int has_prior, float prior_scale, float temperature,
+                     float* out_U, float* out_grad, float* out_lik, float* out_var,
+                     float* partial, unsigned* counter, int nblocks, void* stream) {
+  SGMC_REQUIRE(X && y && theta && out_U && out_grad, SGMC_E_ARG, "%s: null pointer", who);
+  SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
+               "%s: unknown family %d", who, family);
+  SGMC_REQUIRE(D >= 1 && D <= kMaxD, SGMC_E_ARG, "%s: D=%d outside [1, %d]", who, D, kMaxD);
+  SGMC_REQUIRE(ldx % 4 == 0 && ldx >= 4 * ((D + 3) / 4), SGMC_E_ARG,
+               "%s: ldx=%lld must be a multiple of 4 and >= D rounded up", who, (long long)ldx);
+  SGMC_REQUIRE(((uintptr_t)X & 15) == 0, SGMC_E_ARG, "%s: X not 16B aligned", who);
+  SGMC_REQUIRE(n >= 1 && N >= 1, SGMC_E_ARG, "%s: empty batch", who);
+  SGMC_REQUIRE(nblocks >= 1 && (int64_t)nblocks * kLdsRows >= n, SGMC_E_ARG,
+               "%s: nblocks must be >= 1 and >= n / %d", who, kLdsRows);
+  SGMC_REQUIRE(temperature != 0.0f, SGMC_E_ARG, "%s: temperature is zero", who);
+  PotArgs a;
+  a.X = X;
+  a.y = y;
+  a.idx = idx;
+  a.theta = theta;
+  a.out_U = out_U;
+  a.out_grad = out_grad;
+  a.out_lik = out_lik;
+  a.out_var = out_var;
+  a.D = D;
+  a.D4 = (D + 3) / 4;
+  a.partial = partial;
+  a.counter = counter;
+  a.ldx = ldx;
+  a.N = N;
+  a.n = n;
+  a.L = make_lik(family, sigma);
+  a.pr = make_prior(has_prior, prior_scale);
+  a.temperature = temperature;
+  hipStream_t s = as_stream(stream);
+  dim3 grid(nblocks), block(kNT);
+  switch (dclass_of(D)) {
+    case 0: hipLaunchKernelGGL((glm_potential_kernel<4, 1>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((glm_potential_kernel<16, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((glm_potential_kernel<64, 1>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((glm_potential_kernel<64, 2>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((glm_potential_kernel<64, 4>), grid, block, 0, s, a); break;
+  }
+  return check_launch(who);
+}
+
+// ---- scratch of the no-allocation entry point (sgmc_glm_potential_noalloc) ----
+// One 1 MiB array per GPU, held until the process ends: word 0 is the arrival counter
+// (padded to 256 bytes, so it stays at one address whatever the call's size), the block
+// records follow. Every call on a GPU shares that GPU's array; calls that overlap on one
+// GPU race. The counter is zeroed when the array is first used on a device, and every
+// call leaves it at zero (the potential's last block resets it); the records are
+// written before they are read in every call.
+constexpr int64_t kScratchFloats = 1 << 18;
+constexpr int64_t kScratchCounterFloats = 64;
+constexpr int64_t kScratchRecordFloats = kScratchFloats - kScratchCounterFloats;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+__global__ void __launch_bounds__(kNT) zero_floats_kernel(float* p, int n) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+std::mutex g_scratch_mu;
+float* g_scratch[kScratchDevices] = {};
+
+// Device address of the current device's scratch (the host-side name `scratch` is only
+// a stand-in for it): looked up once per device with hipGetSymbolAddress, its counter
+// zeroed on `stream` at that first use.
+int scratch_of_current_device(void* stream, float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  SGMC_REQUIRE(e == hipSuccess, SGMC_E_HIP, "sgmc_glm_potential_noalloc: hipGetDevice: %s",
+               hipGetErrorString(e));
+  SGMC_REQUIRE(dev >= 0 && dev < kScratchDevices, SGMC_E_ARG,
+               "sgmc_glm_potential_noalloc: device %d beyond %d", dev, kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    float* p = nullptr;
+    e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    SGMC_REQUIRE(e == hipSuccess && p, SGMC_E_HIP,
+                 "sgmc_glm_potential_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(zero_floats_kernel, dim3(1), dim3(kNT), 0, as_stream(stream), p,
+                       (int)kScratchCounterFloats);
+    const int rc = check_launch("sgmc_glm_potential_noalloc: scratch set-up");
+    if (rc != SGMC_OK) return rc;
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return SGMC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t sgmc_glm_workspace_floats(int n, int D, int nblocks) {
@@ -977,51 +1080,38 @@ int sgmc_glm_potential(int family, const float* X, int64_t ldx, const float* y,
                        float sigma, int has_prior, float prior_scale, float temperature,
                        float* out_U, float* out_grad, float* out_lik, float* out_var,
                        float* workspace, int64_t workspace_floats, int nblocks, void* stream) {
-  SGMC_REQUIRE(X && y && theta && out_U && out_grad && workspace, SGMC_E_ARG,
-               "sgmc_glm_potential: null pointer");
-  SGMC_REQUIRE(family == SGMC_LIK_LINEAR_GAUSSIAN || family == SGMC_LIK_LOGISTIC, SGMC_E_ARG,
-               "sgmc_glm_potential: unknown family %d", family);
-  SGMC_REQUIRE(D >= 1 && D <= kMaxD, SGMC_E_ARG, "sgmc_glm_potential: D=%d outside [1, %d]", D,
-               kMaxD);
-  SGMC_REQUIRE(ldx % 4 == 0 && ldx >= 4 * ((D + 3) / 4), SGMC_E_ARG,
-               "sgmc_glm_potential: ldx=%lld must be a multiple of 4 and >= D rounded up",
-               (long long)ldx);
-  SGMC_REQUIRE(((uintptr_t)X & 15) == 0, SGMC_E_ARG, "sgmc_glm_potential: X not 16B aligned");
-  SGMC_REQUIRE(n >= 1 && N >= 1, SGMC_E_ARG, "sgmc_glm_potential: empty batch");
-  SGMC_REQUIRE(nblocks >= 1 && (int64_t)nblocks * kLdsRows >= n, SGMC_E_ARG,
-               "sgmc_glm_potential: nblocks must be >= 1 and >= n / %d", kLdsRows);
+  SGMC_REQUIRE(workspace, SGMC_E_ARG, "sgmc_glm_potential: null workspace");
+  SGMC_REQUIRE(D >= 1 && D <= kMaxD && nblocks >= 1, SGMC_E_ARG,
+               "sgmc_glm_potential: D=%d outside [1, %d] or nblocks=%d", D, kMaxD, nblocks);
   SGMC_REQUIRE(workspace_floats >= sgmc_glm_workspace_floats(n, D, nblocks), SGMC_E_ARG,
                "sgmc_glm_potential: workspace too small");
-  SGMC_REQUIRE(temperature != 0.0f, SGMC_E_ARG, "sgmc_glm_potential: temperature is zero");
-  PotArgs a;
-  a.X = X;
-  a.y = y;
-  a.idx = idx;
-  a.theta = theta;
-  a.out_U = out_U;
-  a.out_grad = out_grad;
-  a.out_lik = out_lik;
-  a.out_var = out_var;
-  a.D = D;
-  a.D4 = (D + 3) / 4;
-  a.partial = workspace;
-  a.counter = reinterpret_cast<unsigned*>(workspace + (int64_t)nblocks * (4 * a.D4 + 2));
-  a.ldx = ldx;
-  a.N = N;
-  a.n = n;
-  a.L = make_lik(family, sigma);
-  a.pr = make_prior(has_prior, prior_scale);
-  a.temperature = temperature;
-  hipStream_t s = as_stream(stream);
-  dim3 grid(nblocks), block(kNT);
-  switch (dclass_of(D)) {
-    case 0: hipLaunchKernelGGL((glm_potential_kernel<4, 1>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((glm_potential_kernel<16, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((glm_potential_kernel<64, 1>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((glm_potential_kernel<64, 2>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((glm_potential_kernel<64, 4>), grid, block, 0, s, a); break;
-  }
-  return check_launch("sgmc_glm_potential");
+  const int64_t records = (int64_t)nblocks * (4 * ((D + 3) / 4) + 2);
+  return launch_potential("sgmc_glm_potential", family, X, ldx, y, idx, n, N, D, theta, sigma,
+                          has_prior, prior_scale, temperature, out_U, out_grad, out_lik,
+                          out_var, workspace, reinterpret_cast<unsigned*>(workspace + records),
+                          nblocks, stream);
+}
+
+int sgmc_glm_potential_noalloc(int family, const float* X, int64_t ldx, const float* y,
+                               const int32_t* idx, int n, int64_t N, int D, const float* theta,
+                               float sigma, int has_prior, float prior_scale, float temperature,
+                               float* out_U, float* out_grad, float* out_lik, float* out_var,
+                               int nblocks, void* stream) {
+  SGMC_REQUIRE(D >= 1 && D <= kMaxD && nblocks >= 1, SGMC_E_ARG,
+               "sgmc_glm_potential_noalloc: D=%d outside [1, %d] or nblocks=%d", D, kMaxD,
+               nblocks);
+  const int64_t records = (int64_t)nblocks * (4 * ((D + 3) / 4) + 2);
+  SGMC_REQUIRE(records <= kScratchRecordFloats, SGMC_E_ARG,
+               "sgmc_glm_potential_noalloc: %d blocks of D=%d need %lld record floats; the "
+               "static scratch holds %lld (use sgmc_glm_potential with a workspace)",
+               nblocks, D, (long long)records, (long long)kScratchRecordFloats);
+  float* base = nullptr;
+  const int rc = scratch_of_current_device(stream, &base);
+  if (rc != SGMC_OK) return rc;
+  return launch_potential("sgmc_glm_potential_noalloc", family, X, ldx, y, idx, n, N, D, theta,
+                          sigma, has_prior, prior_scale, temperature, out_U, out_grad, out_lik,
+                          out_var, base + kScratchCounterFloats,
+                          reinterpret_cast<unsigned*>(base), nblocks, stream);
 }
 
 int64_t sgmc_glm_full_workspace_doubles(int64_t N, int rows_per_block) {
