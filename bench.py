@@ -17,13 +17,20 @@ port timed on the host cores as the CPU baseline.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
        python bench.py --copy-calibration   (rocprofv3 counter scaling run)
 """
-import argparse
-import json
 import os
-import sys
-import time
 
-import numpy as np
+# The CPU baselines alternate numpy's BLAS thread pool with OpenMP loops of the C port;
+# OpenMP threads that spin between parallel regions halve the BLAS speed. The policy is
+# read when the first OpenMP runtime loads (torch's, whose symbols the port then binds),
+# so it is set before any import.
+os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -65,6 +72,63 @@ def chain_keys(R, C):
   return keys, dkeys
 
 
+# ---------------------------------------------------------------- CPU baseline ----
+# Protocol (bounded so that a default run stays within minutes):
+# one calibration call, a warm-up of up to 100 iterations (about warm_s seconds), then
+# the median of 5 timed runs of about run_s seconds each, on the host's CPU share
+# (OMP_NUM_THREADS, else os.cpu_count()). The C port used is the baseline build
+# (vectorized float32 sums with FMA contraction, AVX-512 where the host has it).
+
+def host_info():
+  model = "unknown"
+  try:
+    for line in open("/proc/cpuinfo"):
+      if line.startswith("model name"):
+        model = line.split(":", 1)[1].strip()
+        break
+  except OSError:
+    pass
+  from oracle import cport
+  return {"cpu": model, "os_cpu_count": os.cpu_count(), "threads": cpu_threads(),
+          "isa": "avx512" if cport.host_has_avx512() else "avx2"}
+
+
+def cpu_threads():
+  n = os.cpu_count() or 1
+  env = os.environ.get("OMP_NUM_THREADS", "")
+  return max(1, min(n, int(env))) if env.isdigit() else n
+
+
+def cpu_protocol(step, warm_s=3.0, run_s=3.0, runs=5, max_warm=100, min_iters=1,
+                 max_iters=1_000_000):
+  """step(k) runs k more iterations of the workload. Returns the iterations per timed
+  run, the warm-up iterations and the median and all run times in seconds."""
+  t0 = time.perf_counter()
+  step(1)
+  t1 = max(time.perf_counter() - t0, 1e-6)
+  warm = int(min(max_warm, max(0, warm_s / t1)))
+  if warm:
+    step(warm)
+  per = int(min(max_iters, max(min_iters, run_s / t1)))
+  times = []
+  for _ in range(runs):
+    t0 = time.perf_counter()
+    step(per)
+    times.append(time.perf_counter() - t0)
+  return {"warmup_iterations": 1 + warm, "iterations_per_run": per, "runs": runs,
+          "median_s": float(np.median(times)), "run_s": [round(t, 4) for t in times]}
+
+
+def baseline_line(chains_per_iter, prot, what, threads):
+  """cpu_baseline object: chain-steps/s of the median run."""
+  value = chains_per_iter * prot["iterations_per_run"] / prot["median_s"]
+  return {"value": value, "unit": "chain-steps/s", "cores": threads, "kind": "port",
+          "sample": f"{what}; warm-up {prot['warmup_iterations']} iterations, then the median of "
+                    f"{prot['runs']} timed runs of {prot['iterations_per_run']} iterations "
+                    f"({prot['median_s']:.2f} s) on {threads} threads",
+          "protocol": prot, "host": host_info()}
+
+
 def run_fused(torch, fused, R, X, y, K, W, graph_iters=10, target_bytes=0):
   """The 64 chains as TABLES chain tables on separate streams (fused.TableGroup): each
   table's launch prologue and finalizer tail overlap the other table's row gather.
@@ -104,13 +168,14 @@ def run_fused(torch, fused, R, X, y, K, W, graph_iters=10, target_bytes=0):
                    nblocks=grp.nblocks, tuned=tuned)
 
 
-def run_plain(torch, L, R, T, X, y, iters, key, dkey):
+def run_plain(torch, L, R, T, X, y, iters, key, dkey, theta0=None, eps=None):
   """One chain through the modular kernels: per gradient evaluation an index draw,
   a row gather, the potential/gradient kernel and the elementwise update kernels."""
   dev = X.device
   lay = T.Layout(torch.zeros(D))
   table, nb = lay.leaf_table(dev, L.PAIRS_PER_BLOCK)
-  theta = torch.zeros(D, device=dev)
+  theta = (torch.zeros(D, device=dev) if theta0 is None else
+           torch.as_tensor(np.asarray(theta0, np.float32), device=dev).clone())
   p = torch.zeros(D, device=dev)
   g = torch.empty(D, device=dev)
   U = torch.empty(1, device=dev)
@@ -121,7 +186,7 @@ def run_plain(torch, L, R, T, X, y, iters, key, dkey):
   yb = torch.empty(BATCH, device=dev)
   blocks = L.query("sgmc_glm_blocks", BATCH, D)
   ws = torch.zeros(int(L.query("sgmc_glm_workspace_floats", BATCH, D, blocks)), device=dev)
-  eps = step_sizes(iters)
+  eps = step_sizes(iters) if eps is None else eps
   s = L.stream()
 
   def one(t):
@@ -144,27 +209,90 @@ def run_plain(torch, L, R, T, X, y, iters, key, dkey):
   return one, theta
 
 
-def cpu_baseline(X_host, y_host, keys, dkeys, target_s=12.0):
+def laplace_logistic(torch, X, y, newton=10):
+  """MAP and covariance of the Laplace approximation of the logistic posterior with prior
+  N(0, I): Newton's method on the log posterior, its Hessian X^T diag(p (1 - p)) X + I
+  formed on the GPU in float32 and solved in float64. At N = 1e6 rows and D = 512 the
+  posterior is close to this Gaussian (Bernstein-von Mises)."""
+  dev = X.device
+  w = torch.zeros(D, dtype=torch.float64, device=dev)
+  eye = torch.eye(D, dtype=torch.float64, device=dev)
+  for _ in range(newton + 1):
+    p = torch.sigmoid(X @ w.float())
+    g = (X.T @ (y - p)).double() - w
+    H = (X.T @ (X * (p * (1 - p))[:, None])).double() + eye
+    step = torch.linalg.solve(H, g)
+    w = w + step
+  return w, torch.linalg.inv(H), float(step.abs().max().item())
+
+
+def c2_posterior(torch, fused, L, R, T, X, y, keys, dkeys, its=400, burn=200):
+  """Posterior moments of the fused 64-chain run and of the plain single-chain path (the
+  same chain 0, same keys), each against the Laplace approximation, in posterior standard
+  deviations. Chains start at the MAP; the step-size schedule runs over the check's
+  iterations; samples after the burn-in are kept."""
+  mu, cov, last_step = laplace_logistic(torch, X, y)
+  sd = torch.sqrt(torch.diagonal(cov))
+  th0 = mu.float().cpu().numpy()
+  eps = step_sizes(its)
+  slots = np.where(np.arange(its) >= burn, np.arange(its) - burn, -1).astype(np.int32)
+  chains = [dict(X=X, y=y, D=D, n=BATCH, theta0=th0, key=tuple(keys[c]), dkey=tuple(dkeys[c]))
+            for c in range(CHAINS)]
+  eng = fused.FusedGLM(chains, "logistic", "sghmc", eps, steps=LEAPFROG, friction=FRICTION,
+                       prior_scale=1.0, save_slots=slots)
+  eng.run(its)
+  S = torch.stack([eng.sample_trace(c) for c in range(CHAINS)]).double()   # [C, its-burn, D]
+  one, th_plain = run_plain(torch, L, R, T, X, y, its, tuple(keys[0]), tuple(dkeys[0]),
+                            theta0=th0, eps=eps)
+  plain = []
+  for t in range(its):
+    one(t)
+    if t >= burn:
+      plain.append(th_plain.clone())
+  P = torch.stack(plain).double()
+  torch.cuda.synchronize()
+  cm = S.mean(1)                                  # chain means [C, D]
+  pooled = cm.mean(0)
+  se = cm.std(0) / np.sqrt(CHAINS)
+  within = ((S - cm[:, None]) ** 2).mean((0, 1))
+  rms = lambda v: float(torch.sqrt(torch.mean(v * v)).item())
+  z_f = (pooled - mu) / sd
+  z_p = (P.mean(0) - mu) / sd
+  return {"reference": "Laplace approximation of the logistic posterior (Newton MAP, inverse "
+                       "Hessian; last Newton step %.1e)" % last_step,
+          "iterations": its, "burn_in": burn, "start": "MAP",
+          "step_size": f"polynomial {EPS_FIRST} -> {EPS_LAST} over the {its} iterations",
+          "fused": {"chains": CHAINS, "mean_error_rms_sd": rms(z_f),
+                    "mean_error_max_sd": float(z_f.abs().max().item()),
+                    "mc_standard_error_rms_sd": rms(se / sd),
+                    "variance_over_laplace_median": float((within / sd ** 2).median().item())},
+          "plain": {"chains": 1, "mean_error_rms_sd": rms(z_p),
+                    "mc_standard_error_rms_sd": rms(cm.std(0) / sd)},
+          "chain0_fused_vs_plain_mean_max_diff_sd": float(((cm[0] - P.mean(0)) / sd).abs()
+                                                          .max().item())}
+
+
+def cpu_baseline(X_host, y_host, keys, dkeys):
+  """The C port of the same SGHMC loop for all 64 chains (OpenMP over chains, each chain's
+  row dot products and gradient sums vectorized)."""
   from oracle import cport
-  threads = max(1, min(16, os.cpu_count() or 1))
-  C = min(CHAINS, threads)
-  eps = step_sizes(1000)
+  threads = cpu_threads()
+  C = CHAINS
+  eps = step_sizes(100_000)
+  th = np.zeros((C, D), np.float32)
+  kk = np.array(keys[:C], np.uint32).copy()
+  dd = np.array(dkeys[:C], np.uint32).copy()
+  pos = [0]
 
-  def go(iters):
-    th = np.zeros((C, D), np.float32)
-    kk = np.array(keys[:C], np.uint32).copy()
-    dd = np.array(dkeys[:C], np.uint32).copy()
-    t0 = time.perf_counter()
-    cport.sghmc(1, X_host, y_host, BATCH, th, kk, dd, eps[:iters], LEAPFROG, FRICTION,
-                threads=threads)
-    return time.perf_counter() - t0
+  def step(k):
+    e = eps[pos[0]:pos[0] + k]
+    pos[0] += k
+    cport.sghmc(1, X_host, y_host, BATCH, th, kk, dd, e, LEAPFROG, FRICTION, threads=threads,
+                fast=True)
 
-  t1 = go(1)
-  iters = int(max(1, min(5000, target_s / max(t1, 1e-3))))
-  dt = go(iters)
-  return {"value": C * iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
-          "sample": f"{C} chains x {iters} iterations of the same workload "
-                    f"(oracle/cport.c SGHMC logistic, OpenMP over chains, {dt:.1f} s)"}
+  prot = cpu_protocol(step, warm_s=3.0, run_s=3.0, max_warm=100)
+  return baseline_line(C, prot, f"all {C} chains of the same workload (oracle/cport.c SGHMC "
+                       "logistic, baseline build, OpenMP over chains)", threads)
 
 
 # ---------------------------------------------------------------------------------
@@ -270,52 +398,124 @@ def run_ladder(args):
   print(json.dumps(out), flush=True)
 
 
-def ladder_cpu_baseline(probs, temps, keys, dkeys, eps, target_s=10.0):
-  """The oracle's C port of SGLD for every chain of the ladders, one chain per task on a
-  thread pool (ctypes releases the interpreter lock during the call). The swap tests,
-  one per pair every 50 iterations, are a negligible share of the work and are left
-  out."""
+def ladder_cpu_baseline(probs, temps, keys, dkeys, eps):
+  """The C port of SGLD for every chain of the ladders (baseline build), one chain per
+  task on a thread pool of the host's CPU share (ctypes releases the interpreter lock
+  during the call). The swap tests, one per pair every 50 iterations, are a negligible
+  share of the work and are left out."""
   from concurrent.futures import ThreadPoolExecutor
   from oracle import cport
-  threads = max(1, min(16, os.cpu_count() or 1))
+  threads = cpu_threads()
   Xs = [np.ascontiguousarray(X.cpu().numpy(), np.float32) for X, _ in probs]
   ys = [np.ascontiguousarray(y.cpu().numpy(), np.float32) for _, y in probs]
   chains = [(k, j) for k in range(len(probs)) for j in range(len(temps))]
+  th = [np.zeros((1, Xs[k].shape[1]), np.float32) for k, _ in chains]
+  kk = [np.array([keys[c]], np.uint32) for c in range(len(chains))]
+  dd = [np.array([dkeys[c]], np.uint32) for c in range(len(chains))]
+  pool = ThreadPoolExecutor(threads)
 
   def one(c, iters):
     k, j = chains[c]
-    th = np.zeros((1, Xs[k].shape[1]), np.float32)
-    kk = np.array([keys[c]], np.uint32)
-    dd = np.array([dkeys[c]], np.uint32)
-    cport.sgld(1, Xs[k], ys[k], LADDER_BATCH, th, kk, dd, np.resize(eps, iters),
-               temperature=temps[j], threads=1)
+    cport.sgld(1, Xs[k], ys[k], LADDER_BATCH, th[c], kk[c], dd[c], np.resize(eps, iters),
+               temperature=temps[j], threads=1, fast=True)
 
-  def go(iters):
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
-      list(ex.map(lambda c: one(c, iters), range(len(chains))))
-    return time.perf_counter() - t0
+  def step(iters):
+    list(pool.map(lambda c: one(c, iters), range(len(chains))))
 
-  t1 = go(50)  # long enough that the pool's start-up does not dominate
-  iters = int(max(50, min(20000, target_s * 50 / max(t1, 1e-6))))
-  dt = go(iters)
-  return {"value": len(chains) * iters / dt, "unit": "chain-steps/s", "cores": threads,
-          "kind": "port",
-          "sample": f"all {len(chains)} chains x {iters} iterations (oracle/cport.c SGLD, one "
-                    f"chain per task on {threads} threads, {dt:.1f} s; the swap tests left out)"}
+  # calibration and warm-up on blocks of 10 iterations: the pool's start-up per call
+  # stays a small share
+  prot = cpu_protocol(lambda k: step(10 * k), warm_s=2.0, run_s=3.0, max_warm=10)
+  pool.shutdown()
+  prot = dict(prot, warmup_iterations=10 * prot["warmup_iterations"],
+              iterations_per_run=10 * prot["iterations_per_run"])
+  return baseline_line(len(chains), prot, f"all {len(chains)} chains (oracle/cport.c SGLD, "
+                       "baseline build, one chain per task; the swap tests left out)", threads)
 
 
-def mlp_data(torch, R, M):
-  """Inputs U[0,1]^3072 [50000]; labels = argmax of a fixed random teacher MLP."""
-  D_in, H1, H2 = MLP_DIMS
-  kx, kt = R.split(R.PRNGKey(5), 2)
-  X = R.uniform(kx, (MLP_N * D_in,)).reshape(MLP_N, D_in)
-  t = torch.as_tensor(M.init_params(99, D_in, H1, H2), device=X.device)
-  W1, W2, W3, b1, b2, b3 = torch.split(t, [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10])
+def mlp_teacher(torch, M, dims, device):
+  """The fixed random teacher MLP whose argmax labels the inputs."""
+  D_in, H1, H2 = dims
+  t = torch.as_tensor(M.init_params(99, D_in, H1, H2), device=device)
+  return torch.split(t, [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10])
+
+
+def mlp_forward(torch, params, dims, X):
+  """Class probabilities of the MLP (float32 torch, for the posterior checks)."""
+  D_in, H1, H2 = dims
+  W1, W2, W3, b1, b2, b3 = params
   h = torch.relu(X @ W1.view(D_in, H1) + b1)
   h = torch.relu(h @ W2.view(H1, H2) + b2)
-  labels = torch.argmax(h @ W3.view(H2, 10) + b3, dim=1).to(torch.int32)
-  return X, labels
+  return torch.softmax(h @ W3.view(H2, 10) + b3, dim=1)
+
+
+def mlp_data(torch, R, M, key=5, rows=MLP_N):
+  """Inputs U[0,1]^3072 [rows]; labels = argmax of a fixed random teacher MLP
+  3072-512-512-10 (both MLP workloads use these data)."""
+  kx, _ = R.split(R.PRNGKey(key), 2)
+  X = R.uniform(kx, (rows * MLP_DIMS[0],)).reshape(rows, MLP_DIMS[0])
+  p = mlp_forward(torch, mlp_teacher(torch, M, MLP_DIMS, X.device), MLP_DIMS, X)
+  return X, torch.argmax(p, dim=1).to(torch.int32)
+
+
+def split_flat(torch, flat, dims):
+  D_in, H1, H2 = dims
+  return torch.split(flat, [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10])
+
+
+def mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, dims, n, prec, integrator, its, burn,
+                  thin, steps=1, eps_range=(1e-4, 1e-5)):
+  """Posterior moments of the fused engine in `prec` against the same chains (same
+  starts, keys and minibatches) in float32, after the burn-in, every `thin` iterations:
+  the posterior predictive of 1024 held-out inputs (teacher labels), the mean potential
+  and the per-chain parameter means. Monte Carlo standard errors come from the spread
+  over chains."""
+  from jax_sgmc_amd import scheduler
+  C = th0.shape[0]
+  if eps_range[0] == eps_range[1]:
+    eps = np.full(its, eps_range[0], np.float32)
+  else:
+    eps = scheduler.polynomial_table(eps_range[0], eps_range[1], its)
+  keep = (np.arange(its) >= burn) & ((np.arange(its) - burn) % thin == 0)
+  slots = np.full(its, -1, np.int32)
+  slots[keep] = np.arange(int(keep.sum()), dtype=np.int32)
+  Xp, yp = mlp_data(torch, R, M, key=6, rows=1024)
+  res = {}
+  for p in (prec, "fp32"):
+    kw = dict(rms=True) if integrator == "sgld" else dict(integrator=integrator, friction=1.0,
+                                                          steps=steps)
+    eng = M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys, prior_scale=1.0,
+                     save_slots=slots, precision=p, **kw)
+    eng.run(its)
+    torch.cuda.synchronize()
+    S = eng.samples
+    assert bool(torch.isfinite(S).all().item())
+    probs = torch.stack([torch.stack([mlp_forward(torch, split_flat(torch, S[c, k], dims), dims,
+                                                  Xp) for k in range(S.shape[1])]).mean(0)
+                         for c in range(C)])             # [C, 1024, 10]
+    res[p] = dict(probs=probs, U=eng.samples_U.double(), mean=S.mean(1),
+                  accepted=eng.accepted.tolist() if eng.accepted is not None else None)
+    del eng, S
+  f, q = res[prec], res["fp32"]
+  th0_t = torch.as_tensor(th0, device=X.device)
+  pf, pq = f["probs"].mean(0), q["probs"].mean(0)
+  se = (q["probs"].std(0) / np.sqrt(C)).mean().item() if C > 1 else None
+  acc = lambda pr: float((pr.argmax(1) == yp.long()).float().mean().item())
+  Uf, Uq = f["U"].mean(1), q["U"].mean(1)
+  drift = [float((torch.norm(f["mean"][c] - q["mean"][c]) /
+                  torch.norm(q["mean"][c] - th0_t[c])).item()) for c in range(C)]
+  out = {"reference": "the same chains with float32 GEMMs (precision fp32)",
+         "iterations": its, "burn_in": burn, "thinning": thin, "samples_per_chain":
+         int(keep.sum()), "chains": C,
+         "predictive_mean_abs_diff": float((pf - pq).abs().mean().item()),
+         "predictive_mc_standard_error": se,
+         "probe_accuracy": {prec: acc(pf), "fp32": acc(pq)},
+         "potential_mean": {prec: float(Uf.mean().item()), "fp32": float(Uq.mean().item()),
+                            "mc_standard_error": (float(Uq.std().item() / np.sqrt(C))
+                                                  if C > 1 else None)},
+         "parameter_mean_diff_over_displacement": float(np.mean(drift))}
+  if f["accepted"] is not None:
+    out["accepted"] = {prec: f["accepted"], "fp32": q["accepted"]}
+  return out
 
 
 def run_mlp(args):
@@ -334,93 +534,88 @@ def run_mlp(args):
   G = 10
   eps = scheduler.polynomial_table(1e-4, 1e-5, K + W + 2 * G)
   prec = args.precision or "bf16"
+
+  def timed(eng, iters):
+    g = eng.capture(G)
+    for _ in range(max(1, W // G)):
+      g.replay()
+    torch.cuda.synchronize()
+    reps = max(1, iters // G)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+      g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(eng.theta).all().item())
+    return e0.elapsed_time(e1), reps * G
+
   eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0, eps, keys=keys, dkeys=dkeys, rms=True,
                    prior_scale=1.0, precision=prec)
-  g = eng.capture(G)
-  for _ in range(max(1, W // G)):
-    g.replay()
-  torch.cuda.synchronize()
-  reps = max(1, K // G)
-  e0 = torch.cuda.Event(enable_timing=True)
-  e1 = torch.cuda.Event(enable_timing=True)
-  e0.record()
-  for _ in range(reps):
-    g.replay()
-  e1.record()
-  torch.cuda.synchronize()
-  ms = e0.elapsed_time(e1)
-  iters = reps * G
+  ms, iters = timed(eng, K)
+  del eng
   value = C * iters / (ms / 1e3)
   macs = D_in * H1 + H1 * H2 + H2 * 10
   flops = 2 * MLP_BATCH * macs * 2 + 2 * MLP_BATCH * (H1 * H2 + H2 * 10)
   achieved = C * flops / (ms / 1e3 / iters) / 1e12
-  out = {"metric": "chain-steps/sec, pSGLD Bayesian MLP (fused bf16 MFMA, 1 MI355X)",
+  peak = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3}[prec]
+  out = {"metric": "chain-steps/sec, pSGLD Bayesian MLP (fused MFMA, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
          "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
          "vs_baseline": None, "dtype": prec,
          "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
-         "config": {"workload": "pSGLD (rms_prop 0.9, 1e-5) MLP 3072-512-512-10, N=50000, "
-                                "batch 256, 16 chains", "chains": C, "P": eng.P,
+         "config": {"workload": f"pSGLD (rms_prop 0.9, 1e-5) MLP 3072-512-512-10, N=50000, "
+                                f"batch 256, {C} chains", "chains": C, "P": int(th0.shape[1]),
                     "batch": MLP_BATCH, "N": MLP_N},
-         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0,
-                      "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                       "kernel": "all kernels of sgmc_mlp_step (GEMM flops per iteration)",
                       "flops_per_chain_step": flops}}
   if not args.no_plain:
-    log("plain fp32 torch path (generic potential, one chain)")
-    from jax_sgmc_amd import alias, data as DT, likelihoods as LK, potential as PT
-    loader = DT.DeviceNumpyDataLoader(x=X, y=labels)
-    pot = PT.minibatch_potential(LK.GaussianPrior(1.0), M.MLPSoftmax(), strategy="vmap")
-    run = alias.sgld(pot, loader, cache_size=1, batch_size=MLP_BATCH, rms_prop=True,
-                     first_step_size=1e-4, last_step_size=1e-5, accepted_samples=1,
-                     save_to_numpy=False)
-    p0 = {k: torch.as_tensor(v, device=X.device) for k, v in
-          zip(("W1", "W2", "W3", "b1", "b2", "b3"),
-              np.split(th0[0], np.cumsum([D_in * H1, H1 * H2, H2 * 10, H1, H2])))}
-    p0 = {k: v.reshape(M.leaf_shapes(D_in, H1, H2)[k]) for k, v in p0.items()}
-    run(p0, iterations=3)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    its = 20
-    run(p0, iterations=its)
-    torch.cuda.synchronize()
-    plain = its / (time.perf_counter() - t0)
+    log("plain path: one chain, float32 GEMMs")
+    eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0[:1], eps, keys=keys[:1],
+                     dkeys=dkeys[:1], rms=True, prior_scale=1.0, precision="fp32")
+    pms, pits = timed(eng, min(K, 300))
+    del eng
+    plain = pits / (pms / 1e3)
     out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
-                    "path": "generic torch fp32 potential (vmap + autograd) with the "
-                            "alias.sgld rms_prop loop"}
+                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain"}
     out["fused_over_plain"] = value / plain
+    log(f"posterior moments: {prec} against fp32 ({C} chains)")
+    out["posterior"] = mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, MLP_DIMS,
+                                     MLP_BATCH, prec, "sgld", its=300, burn=200, thin=10)
   if not args.no_cpu:
-    log("CPU baseline (numpy oracle, one chain)")
-    out["cpu_baseline"] = mlp_cpu_baseline(X.cpu().numpy(), labels.cpu().numpy(), th0[0], eps)
+    log("CPU baseline (float32 port: BLAS GEMMs, C port noise and updates)")
+    out["cpu_baseline"] = mlp_cpu_baseline(X.cpu().numpy(), labels.cpu().numpy(), th0[0],
+                                           tuple(keys[0]), tuple(dkeys[0]), eps)
     out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
   else:
     out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
 
 
-def mlp_cpu_baseline(X_host, labels_host, theta0, eps, target_s=10.0):
-  """The numpy oracle's pSGLD (float64 MLP forward / backward, jax-compatible noise) on
-  one chain of the same workload; numpy's BLAS threads do the matrix products."""
-  from oracle import model as om
-  from oracle import steps as ost
-  D_in, H1, H2 = MLP_DIMS
-  fam = om.Family("mlp", X_host, labels_host, dims=MLP_DIMS + (10,))
-  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
-  leaves = [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10]
+def mlp_cpu_baseline(X_host, labels_host, theta0, key, dkey, eps):
+  """The float32 port of the same pSGLD iteration (oracle/mlp_port.py: numpy's float32
+  BLAS for the matrix products on all threads, the C port's OpenMP loops for the noise
+  and the update), one chain at a time: a chain-step uses every core already."""
+  from oracle import mlp_port as mp
+  threads = cpu_threads()
+  port = mp.MLPPort(X_host, labels_host, MLP_DIMS, MLP_BATCH, threads=threads)
+  theta = np.array(theta0, np.float32)
+  v = np.ones_like(theta)
+  k = np.array(key, np.uint32)
+  dk = np.array(dkey, np.uint32)
+  pos = [0]
 
-  def go(iters):
-    t0 = time.perf_counter()
-    ost.sgld(pot, np.asarray(theta0, np.float64), iters, eps[:iters], 1.0, MLP_N, MLP_BATCH,
-             leaves, rms=(0.9, 1e-5))
-    return time.perf_counter() - t0
+  def step(m):
+    e = np.resize(eps, pos[0] + m)[pos[0]:]
+    pos[0] += m
+    port.psgld(theta, v, k, dk, e)
 
-  t1 = go(1)
-  iters = int(max(1, min(len(eps), target_s / max(t1, 1e-6))))
-  dt = go(iters)
-  threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-  return {"value": iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
-          "sample": f"1 chain x {iters} iterations (oracle/model.py + oracle/steps.py pSGLD "
-                    f"in float64 numpy, {dt:.1f} s)"}
+  prot = cpu_protocol(step, warm_s=2.0, run_s=3.0, max_warm=100)
+  return baseline_line(1, prot, "1 chain of the same workload (oracle/mlp_port.py pSGLD in "
+                       "float32: numpy BLAS GEMMs, C port noise and rms_prop update)", threads)
 
 
 def run_sggmc_mlp(args):
@@ -430,7 +625,7 @@ def run_sggmc_mlp(args):
   import torch
   import __graft_entry__
   __graft_entry__.load()
-  from jax_sgmc_amd import _lib as L, mlp as M, random as R, scheduler
+  from jax_sgmc_amd import _lib as L, mlp as M, random as R
   L.require_gpu()
   dims = (3072, 1024, 1024)
   X, labels = mlp_data(torch, R, M)
@@ -438,25 +633,33 @@ def run_sggmc_mlp(args):
   th0 = np.stack([M.init_params(10 + c, *dims) for c in range(C)])
   keys = R.key_words(R.split(R.PRNGKey(0), C))
   dkeys = R.key_words(R.split(R.PRNGKey(2), C))
-  K, W = max(1, args.steps), args.warmup
+  # one iteration of 4 chains takes about 70 ms: the timed work stays within seconds
+  K, W = max(1, min(args.steps, 30)), max(1, min(args.warmup, 3))
   # a constant step size (a polynomial table with equal first and last step sizes has
   # NaN coefficients, as the reference's schedule does)
   eps = np.full(K + W + 2, 1e-4, np.float32)
   prec = args.precision or "fp8"
+
+  def timed(eng, iters):
+    g = eng.capture(1)
+    for _ in range(W):
+      g.replay()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+      g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    # the state after the timed iterations is finite (the line checks its result)
+    assert bool(torch.isfinite(eng.theta).all().item()) and \
+        bool(torch.isfinite(eng.U_full).all().item())
+    return e0.elapsed_time(e1)
+
   eng = M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys, prior_scale=1.0,
                    integrator="sggmc", friction=1.0, steps=steps, precision=prec)
-  g = eng.capture(1)
-  for _ in range(max(1, W)):
-    g.replay()
-  torch.cuda.synchronize()
-  e0 = torch.cuda.Event(enable_timing=True)
-  e1 = torch.cuda.Event(enable_timing=True)
-  e0.record()
-  for _ in range(K):
-    g.replay()
-  e1.record()
-  torch.cuda.synchronize()
-  ms = e0.elapsed_time(e1)
+  ms = timed(eng, K)
   value = C * K / (ms / 1e3)
   D_in, H1, H2 = dims
   macs = D_in * H1 + H1 * H2 + H2 * 10
@@ -464,7 +667,7 @@ def run_sggmc_mlp(args):
   full_flops = 2 * MLP_N * macs
   flops = 2 * steps * grad_flops + full_flops
   achieved = C * flops / (ms / 1e3 / K) / 1e12
-  peak = 5000.0 if prec == "fp8" else 2500.0
+  peak = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3}[prec]
   out = {"metric": "chain-steps/sec, SGGMC Bayesian MLP (fused MFMA, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
          "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
@@ -478,8 +681,26 @@ def run_sggmc_mlp(args):
                       "kernel": "all kernels of one SGGMC iteration (GEMM flops)",
                       "flops_per_chain_step": flops},
          "accepted": eng.accepted.tolist()}
+  log("float64 oracle errors of chain 0")
+  out["fp64_oracle_error"] = sggmc_mlp_oracle_errors(torch, M, X, labels, dims, n, eng)
+  del eng
+  if not args.no_plain:
+    log("plain path: one chain, float32 GEMMs")
+    eng = M.FusedMLP(X, labels, dims, n, th0[:1], eps, keys=keys[:1], dkeys=dkeys[:1],
+                     prior_scale=1.0, integrator="sggmc", friction=1.0, steps=steps,
+                     precision="fp32")
+    pms = timed(eng, min(K, 5))
+    del eng
+    plain = min(K, 5) / (pms / 1e3)
+    out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
+                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain"}
+    out["fused_over_plain"] = value / plain
+    log(f"posterior moments: {prec} against fp32 ({C} chains)")
+    out["posterior"] = mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, dims, n, prec,
+                                     "sggmc", its=12, burn=2, thin=1, steps=steps,
+                                     eps_range=(1e-4, 1e-4))
   if not args.no_cpu:
-    log("CPU baseline (numpy oracle, sampled)")
+    log("CPU baseline (float32 port, sampled)")
     out["cpu_baseline"] = sggmc_mlp_cpu_baseline(X.cpu().numpy(), labels.cpu().numpy(), th0[0],
                                                  dims, n, steps)
     out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
@@ -488,51 +709,82 @@ def run_sggmc_mlp(args):
   print(json.dumps(out), flush=True)
 
 
-def sggmc_mlp_cpu_baseline(X_host, labels_host, theta0, dims, n, steps, sample_steps=2):
-  """The numpy oracle's SGGMC iteration on one chain, timed on a bounded sample: two
-  OBABO steps (four minibatch gradients and their noise) and one full-data potential,
-  scaled to an iteration of ``steps`` OBABO steps plus the full-data pass."""
+def sggmc_mlp_oracle_errors(torch, M, X, labels, dims, n, eng):
+  """Chain 0 of the engine against the float64 oracle: the full-data potential the MH test
+  kept (float64 sum of the engine's likelihoods) against the oracle's at the same
+  parameters, and the potential and gradient of the minibatch the chain's data key draws
+  next."""
   from oracle import model as om
   from oracle import steps as ost
-  D_in, H1, H2 = dims
-  fam = om.Family("mlp", X_host, labels_host, dims=tuple(dims) + (10,))
-  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
-  leaves = [D_in * H1, H1 * H2, H2 * 10, H1, H2, 10]
-  th = np.asarray(theta0, np.float64)
-  ones = np.ones(th.size)
-  t0 = time.perf_counter()
-  ost.obabo_integrate(pot, th, np.zeros(th.size), np.array([0, 1], np.uint32),
-                      np.array([0, 2], np.uint32), np.float32(1e-4), 1.0, 1.0, sample_steps,
-                      MLP_N, n, leaves, ones, ones)
-  t_steps = time.perf_counter() - t0
-  t0 = time.perf_counter()
-  om.full_potential(fam, th, 1.0, 5000)
-  t_full = time.perf_counter() - t0
-  per_iter = t_steps * steps / sample_steps + t_full
-  threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+  Xh, yh = X.cpu().numpy(), labels.cpu().numpy()
+  th = eng.theta[0].double().cpu().numpy()
+  fam = om.Family("mlp", Xh, yh, dims=tuple(dims) + (10,))
+  U_ref = om.full_potential(fam, th, 1.0, mb=5000)
+  U_kept = float(eng.U_full[0].item())
+  U_now = float(eng.full_potential()[0].item())
+  dk = eng.dkeys[0].cpu().numpy().view(np.uint32).copy()
+  _, idx = ost.draw_indices(dk, Xh.shape[0], n)
+  Ur, gr, _ = om.stochastic_potential(fam, th, idx, 1.0)
+  U_mb, g = eng.potential_and_grad()
+  g0 = g[0].double().cpu().numpy()
+  return {"full_potential_rel": abs(U_now - U_ref) / abs(U_ref),
+          "kept_full_potential_rel": abs(U_kept - U_ref) / abs(U_ref),
+          "minibatch_potential_rel": abs(float(U_mb[0].item()) - Ur) / abs(Ur),
+          "gradient_rel_norm": float(np.linalg.norm(g0 - gr) / np.linalg.norm(gr)),
+          "full_potential_oracle": U_ref}
+
+
+def sggmc_mlp_cpu_baseline(X_host, labels_host, theta0, dims, n, steps):
+  """The float32 port's SGGMC iteration on one chain, timed on a bounded sample: OBABO
+  steps (two minibatch gradients and their noise each) by the protocol, the full-data
+  potential (float64 sum of float32 batch likelihoods) three times; an iteration is
+  `steps` OBABO steps plus one full-data pass."""
+  from oracle import mlp_port as mp
+  threads = cpu_threads()
+  port = mp.MLPPort(X_host, labels_host, dims, n, threads=threads)
+  theta = np.array(theta0, np.float32)
+  p = np.zeros_like(theta)
+  key = np.array([0, 1], np.uint32)
+  dkey = np.array([0, 2], np.uint32)
+  prot = cpu_protocol(lambda k: port.obabo_steps(theta, p, key, dkey, 1e-4, 1.0, k),
+                      warm_s=1.0, run_s=1.0, max_warm=2)
+  t_step = prot["median_s"] / prot["iterations_per_run"]
+  tf = []
+  for _ in range(3):
+    t0 = time.perf_counter()
+    port.full_potential(theta, 5000)
+    tf.append(time.perf_counter() - t0)
+  t_full = float(np.median(tf))
+  per_iter = t_step * steps + t_full
   return {"value": 1.0 / per_iter, "unit": "chain-steps/s", "cores": threads, "kind": "port",
-          "sample": f"1 chain: {sample_steps} OBABO steps ({t_steps:.1f} s) and one full-data "
-                    f"potential ({t_full:.1f} s) of the numpy oracle in float64, scaled to an "
-                    f"iteration of {steps} steps ({per_iter:.0f} s)"}
+          "sample": f"1 chain of the float32 port (oracle/mlp_port.py): OBABO steps by the "
+                    f"protocol ({t_step * 1e3:.0f} ms each: median of "
+                    f"{prot['runs']} runs of {prot['iterations_per_run']}), the full-data "
+                    f"potential ({t_full:.2f} s, median of 3); an iteration of {steps} steps "
+                    f"and one full-data pass takes {per_iter:.1f} s",
+          "protocol": prot, "host": host_info()}
 
 
 def run_sgld_linreg(args):
   """SGLD on Bayesian linear regression, N=10,000, D=32, batch 128, one chain, through
-  the fused engine's loop launch (the chain's iterations back to back in one block)."""
+  the fused engine's loop launch (the chain's iterations back to back in one block);
+  the plain path runs the modular kernels (index draw, gathers, potential, SGLD step)
+  with one launch each per iteration."""
   import torch
   import __graft_entry__
   __graft_entry__.load()
-  from jax_sgmc_amd import _lib as L, fused, random as R, scheduler
+  from jax_sgmc_amd import _lib as L, fused, random as R, scheduler, tree as T
   L.require_gpu()
   N, Dl, n = 10_000, 32, 128
   kx, kw, ke = R.split(R.PRNGKey(0), 3)
   X = R.normal(kx, (N * Dl,)).reshape(N, Dl)
   w = R.normal(kw, (Dl,))
-  y = X @ w + 0.5 * R.normal(ke, (N,))
+  y = (X @ w + 0.5 * R.normal(ke, (N,))).contiguous()
   K, W, G = args.steps, args.warmup, 100
+  sigma, tau = 0.5, float(np.sqrt(10.0))
   eps = scheduler.polynomial_table(1e-4, 1e-5, K + W + 2 * G)
-  eng = fused.FusedGLM([dict(X=X, y=y.contiguous(), D=Dl, n=n, theta0=np.zeros(Dl, np.float32))],
-                       "linear_gaussian", "sgld", eps, sigma=0.5, prior_scale=float(np.sqrt(10.0)))
+  eng = fused.FusedGLM([dict(X=X, y=y, D=Dl, n=n, theta0=np.zeros(Dl, np.float32))],
+                       "linear_gaussian", "sgld", eps, sigma=sigma, prior_scale=tau)
   # one chain of one block: the loop launch runs all iterations back to back
   assert eng.loop_capable()
   eng.run(max(1, W), loop=True)
@@ -547,6 +799,7 @@ def run_sgld_linreg(args):
   ms = e0.elapsed_time(e1)
   iters = reps * G
   value = iters / (ms / 1e3)
+  assert bool(torch.isfinite(eng.theta_flat).all().item())
   bytes_step = n * (4 * Dl + 8) + 8 * Dl
   out = {"metric": "chain-steps/sec, SGLD linear regression (one chain, 1 MI355X)",
          "value": value, "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
@@ -561,37 +814,134 @@ def run_sgld_linreg(args):
                       "frac": bytes_step / (ms / 1e3 / iters) / 1e9 / HBM_PEAK_GBS,
                       "traffic": None, "kernel": "glm_loop_kernel (latency bound: one block)",
                       "bytes_per_chain_step": bytes_step}}
+  if not args.no_plain:
+    log("plain path (modular kernels) and posterior moments against the analytic posterior")
+    one, _ = linreg_plain(torch, L, R, T, X, y, eps, sigma, tau)
+    for t in range(20):
+      one(t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = 400
+    for t in range(its):
+      one(t)
+    torch.cuda.synchronize()
+    plain = its / (time.perf_counter() - t0)
+    out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
+                    "path": "per iteration: draw_batch, 2x gather_rows, glm_potential, "
+                            "sgld_step (eager launches)"}
+    out["fused_over_plain"] = value / plain
+    out["posterior"] = c1_posterior(torch, fused, L, R, T, X, y, sigma, tau)
   if not args.no_cpu:
     log("CPU baseline (oracle C port, one chain)")
-    out["cpu_baseline"] = linreg_cpu_baseline(X.cpu().numpy(), y.cpu().numpy(), eps)
+    out["cpu_baseline"] = linreg_cpu_baseline(X.cpu().numpy(), y.cpu().numpy(), sigma, tau)
     out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
   else:
     out["cpu_baseline"] = None
   print(json.dumps(out), flush=True)
 
 
-def linreg_cpu_baseline(X_host, y_host, eps_table, target_s=10.0):
-  """The oracle's C port of the same single-chain SGLD loop on one host core (a single
-  chain has nothing to spread over more)."""
+def linreg_plain(torch, L, R, T, X, y, eps, sigma, tau):
+  """One chain of langevin_diffusion (integrator.py:859-922) through the modular kernels,
+  one launch each: index draw, row and label gathers, potential / gradient, SGLD step."""
+  dev = X.device
+  N, Dl = X.shape
+  n = 128
+  lay = T.Layout(torch.zeros(Dl))
+  table, nb = lay.leaf_table(dev, L.PAIRS_PER_BLOCK)
+  theta = torch.zeros(Dl, device=dev)
+  g = torch.empty(Dl, device=dev)
+  U = torch.empty(1, device=dev)
+  k = [R.key_from_words((0, 0)), torch.empty(2, dtype=torch.int32, device=dev)]
+  dk = [R.key_from_words((0, 0)), torch.empty(2, dtype=torch.int32, device=dev)]
+  idx = torch.empty(n, dtype=torch.int32, device=dev)
+  Xb = torch.empty(n, Dl, device=dev)
+  yb = torch.empty(n, device=dev)
+  blocks = L.query("sgmc_glm_blocks", n, Dl)
+  ws = torch.zeros(int(L.query("sgmc_glm_workspace_floats", n, Dl, blocks)), device=dev)
+  s = L.stream()
+
+  def one(t):
+    L.call("sgmc_draw_batch", L.ptr(dk[0]), L.ptr(dk[1]), L.ptr(idx), n, N, s)
+    dk.reverse()
+    L.call("sgmc_gather_rows", L.ptr(X), N, L.ptr(idx), L.ptr(Xb), n, 4 * Dl, 4 * Dl, s)
+    L.call("sgmc_gather_rows", L.ptr(y), N, L.ptr(idx), L.ptr(yb), n, 4, 4, s)
+    L.call("sgmc_glm_potential", L.LIK_LINEAR_GAUSSIAN, L.ptr(Xb), Dl, L.ptr(yb), None, n, N,
+           Dl, L.ptr(theta), sigma, 1, tau, 1.0, L.ptr(U), L.ptr(g), None, None, L.ptr(ws),
+           ws.numel(), blocks, s)
+    L.call("sgmc_sgld_step", L.ptr(theta), L.ptr(g), None, Dl, L.ptr(table), nb, L.ptr(k[0]),
+           L.ptr(k[1]), float(eps[t]), 1.0, 0.9, 1e-5, s)
+    k.reverse()
+
+  return one, theta
+
+
+def c1_posterior(torch, fused, L, R, T, X, y, sigma, tau, its=10_000, burn=2_000, thin=10):
+  """The full 10,000-iteration run of the configuration (polynomial 1e-4 -> 1e-5) through
+  the fused loop launch and through the plain modular kernels, samples after the burn-in
+  (every 10th) against the analytic Gaussian posterior: Lambda = X^T X / s^2 + I / t^2,
+  mu = Lambda^-1 X^T y / s^2; errors in posterior standard deviations. The first step
+  sizes exceed 2 / lambda_max (the configuration's 1e-4 start), so the chain first grows
+  and then contracts once the schedule falls below it; the burn-in covers that."""
+  from jax_sgmc_amd import scheduler
+  Xh = X.double().cpu().numpy()
+  yh = y.double().cpu().numpy()
+  Lam = Xh.T @ Xh / sigma ** 2 + np.eye(Xh.shape[1]) / tau ** 2
+  cov = np.linalg.inv(Lam)
+  mu = cov @ (Xh.T @ yh) / sigma ** 2
+  sd = np.sqrt(np.diag(cov))
+  eps = scheduler.polynomial_table(1e-4, 1e-5, its)
+  keep = (np.arange(its) >= burn) & ((np.arange(its) - burn) % thin == 0)
+  slots = np.full(its, -1, np.int32)
+  slots[keep] = np.arange(int(keep.sum()), dtype=np.int32)
+  eng = fused.FusedGLM([dict(X=X, y=y, D=X.shape[1], n=128, theta0=np.zeros(X.shape[1],
+                                                                          np.float32))],
+                       "linear_gaussian", "sgld", eps, sigma=sigma, prior_scale=tau,
+                       save_slots=slots)
+  eng.run(its, loop=True)
+  Sf = eng.sample_trace(0).double().cpu().numpy()
+  one, theta = linreg_plain(torch, L, R, T, X, y, eps, sigma, tau)
+  plain = []
+  for t in range(its):
+    one(t)
+    if keep[t]:
+      plain.append(theta.clone())
+  Sp = torch.stack(plain).double().cpu().numpy()
+
+  def stats(S):
+    z = (S.mean(0) - mu) / sd
+    # batch-means standard error of the mean (10 batches of consecutive samples)
+    bm = S[: len(S) // 10 * 10].reshape(10, -1, S.shape[1]).mean(1)
+    se = bm.std(0, ddof=1) / np.sqrt(10) / sd
+    return {"mean_error_rms_sd": float(np.sqrt(np.mean(z * z))),
+            "mean_error_max_sd": float(np.abs(z).max()),
+            "mc_standard_error_rms_sd": float(np.sqrt(np.mean(se * se))),
+            "variance_over_posterior_median": float(np.median(S.var(0) / sd ** 2))}
+
+  return {"reference": "analytic Gaussian posterior of the linear regression",
+          "iterations": its, "burn_in": burn, "thinning": thin, "samples": int(keep.sum()),
+          "fused": stats(Sf), "plain": stats(Sp),
+          "fused_vs_plain_mean_max_diff_sd": float(np.abs((Sf.mean(0) - Sp.mean(0)) / sd).max())}
+
+
+def linreg_cpu_baseline(X_host, y_host, sigma, tau):
+  """The C port of the same single-chain SGLD loop on one host core (one chain of 128
+  rows x 32 features per step has too little work to split over threads): the full
+  10,000-iteration run per timed run."""
   from oracle import cport
-  N, Dl = X_host.shape
+  from jax_sgmc_amd import scheduler
+  Dl = X_host.shape[1]
+  eps_run = scheduler.polynomial_table(1e-4, 1e-5, 10_000)
+  theta = np.zeros((1, Dl), np.float32)
+  keys = np.zeros((1, 2), np.uint32)
+  dkeys = np.zeros((1, 2), np.uint32)
 
-  def go(iters):
-    theta = np.zeros((1, Dl), np.float32)
-    keys = np.zeros((1, 2), np.uint32)
-    dkeys = np.zeros((1, 2), np.uint32)
-    eps = np.resize(np.asarray(eps_table, np.float32), iters)
-    t0 = time.perf_counter()
-    cport.sgld(0, X_host, y_host, 128, theta, keys, dkeys, eps, sigma=0.5,
-               prior_scale=float(np.sqrt(10.0)), threads=1)
-    return time.perf_counter() - t0
+  def step(k):
+    cport.sgld(0, X_host, y_host, 128, theta, keys, dkeys, np.resize(eps_run, k), sigma=sigma,
+               prior_scale=tau, threads=1, fast=True)
 
-  t1 = go(2000)
-  iters = int(max(2000, min(2_000_000, target_s * 2000 / max(t1, 1e-6))))
-  dt = go(iters)
-  return {"value": iters / dt, "unit": "chain-steps/s", "cores": 1, "kind": "port",
-          "sample": f"1 chain x {iters} iterations of the same workload (oracle/cport.c "
-                    f"SGLD linear regression, {dt:.1f} s)"}
+  prot = cpu_protocol(step, warm_s=1.0, run_s=1.0, min_iters=10_000, max_iters=10_000)
+  return baseline_line(1, prot, "1 chain of the same workload (oracle/cport.c SGLD linear "
+                       "regression, baseline build)", 1)
 
 
 # Full-data MH workloads on the headline's data: leapfrog steps per test, friction and
@@ -599,42 +949,40 @@ def linreg_cpu_baseline(X_host, y_host, eps_table, target_s=10.0):
 MH_SETTINGS = {"amagold": (5, 0.25, (1e-4, 5e-5)), "sggmc": (5, 1.0, (1e-4, 5e-5))}
 
 
-def mh_cpu_baseline(kind, X_host, y_host, theta0, keys, dkeys, eps, target_s=12.0):
-  """The oracle's C port of the same MH loop (reversible leapfrog or OBABO, FP64 full
-  potential over all rows, MH), OpenMP over chains."""
+def mh_cpu_baseline(kind, X_host, y_host, theta0, keys, dkeys, eps):
+  """The C port of the same MH loop (reversible leapfrog or OBABO, FP64 full potential
+  over all rows, MH) for all 64 chains, baseline build, OpenMP over chains."""
   from oracle import cport
   from oracle import jaxrandom as jr
   steps, fric, _ = MH_SETTINGS[kind]
-  threads = max(1, min(16, os.cpu_count() or 1))
-  C = min(CHAINS, threads)
+  threads = cpu_threads()
+  C = CHAINS
   Xh = np.ascontiguousarray(X_host, np.float32)
   yh = np.ascontiguousarray(y_host, np.float32)
+  th = np.tile(np.asarray(theta0, np.float32), (C, 1))
+  mom = np.zeros((C, D), np.float32)
+  kk = np.zeros((C, 2), np.uint32)
+  mk = np.zeros((C, 2), np.uint32)
+  for c in range(C):
+    if kind == "amagold":
+      mom[c], kk[c], mk[c] = cport.amagold_init(th[c], np.asarray(keys[c], np.uint32))
+    else:
+      kk[c], mk[c] = jr.split(np.asarray(keys[c], np.uint32))
+  dd = np.array(dkeys[:C], np.uint32).copy()
+  U = np.full(C, cport.full_potential(1, Xh, yh, th[0]))
+  run = cport.amagold if kind == "amagold" else cport.sggmc
+  pos = [0]
 
-  def go(iters):
-    th = np.tile(np.asarray(theta0, np.float32), (C, 1))
-    mom = np.zeros((C, D), np.float32)
-    kk = np.zeros((C, 2), np.uint32)
-    mk = np.zeros((C, 2), np.uint32)
-    for c in range(C):
-      if kind == "amagold":
-        mom[c], kk[c], mk[c] = cport.amagold_init(th[c], np.asarray(keys[c], np.uint32))
-      else:
-        kk[c], mk[c] = jr.split(np.asarray(keys[c], np.uint32))
-    dd = np.array(dkeys[:C], np.uint32).copy()
-    U = np.full(C, cport.full_potential(1, Xh, yh, th[0]))
-    run = cport.amagold if kind == "amagold" else cport.sggmc
-    t0 = time.perf_counter()
-    run(1, Xh, yh, BATCH, th, mom, kk, mk, dd, U, eps[:iters], steps, fric, threads=threads)
-    return time.perf_counter() - t0
+  def step(k):
+    e = np.resize(eps, pos[0] + k)[pos[0]:]
+    pos[0] += k
+    run(1, Xh, yh, BATCH, th, mom, kk, mk, dd, U, e, steps, fric, threads=threads, fast=True)
 
-  t1 = go(1)
-  iters = int(max(1, min(500, target_s / max(t1, 1e-3))))
-  dt = go(iters)
+  prot = cpu_protocol(step, warm_s=1.0, run_s=1.0, max_warm=1)
   name = "AMAGOLD" if kind == "amagold" else "SGGMC"
-  return {"value": C * iters / dt, "unit": "chain-steps/s", "cores": threads, "kind": "port",
-          "sample": f"{C} chains x {iters} iterations of the same workload (oracle/cport.c "
-                    f"{name} logistic with its FP64 full-data potential, OpenMP over chains, "
-                    f"{dt:.1f} s)"}
+  return baseline_line(C, prot, f"all {C} chains of the same workload (oracle/cport.c {name} "
+                       "logistic with its FP64 full-data potential, baseline build, OpenMP "
+                       "over chains)", threads)
 
 
 def run_mh(args, kind):
@@ -865,6 +1213,8 @@ def main():
                             "sghmc_position, sghmc_momentum (eager launches)",
                     "same_chain_max_rel_diff_after_8_iterations": rel}
     out["fused_over_plain"] = value / plain_rate
+    log("posterior moments (fused 64 chains and plain chain 0 against the Laplace posterior)")
+    out["posterior"] = c2_posterior(torch, fused, L, R, T, X, y, keys, dkeys)
 
   if not args.no_cpu:
     log("CPU baseline (oracle C port)")

@@ -1,7 +1,10 @@
-"""ctypes handle on the oracle's C port (oracle/_build/liboracle_port.so).
+"""ctypes handle on the oracle's C port (oracle/_build/liboracle_port*.so).
 
 Test infrastructure: the timed CPU baseline of bench.py and a compiled cross-check of
-the numpy oracle. Build with ``make -C oracle``.
+the numpy oracle. Build with ``make -C oracle``. ``fast=False`` (the default) loads the
+oracle build (written operation order, float64 accumulators: what tests/ compare);
+``fast=True`` the baseline build (vectorized float32 sums with FMA contraction, AVX-512
+when the host has it, AVX2 otherwise), which bench.py times.
 """
 import ctypes
 import os
@@ -10,16 +13,39 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "_build", "liboracle_port.so")
-_lib = None
+_libs = {}
 
 
-def load():
-  global _lib
-  if _lib is None:
-    if not os.path.exists(LIB):
-      raise ImportError(f"{LIB} missing: run make -C oracle")
-    _lib = ctypes.CDLL(LIB)
-  return _lib
+def host_has_avx512():
+  try:
+    flags = open("/proc/cpuinfo").read()
+  except OSError:
+    return False
+  return all(f in flags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl"))
+
+
+def lib_path(fast=False):
+  if not fast:
+    return LIB
+  return os.path.join(_HERE, "_build",
+                      f"liboracle_port_fast_{'v4' if host_has_avx512() else 'v3'}.so")
+
+
+def load(fast=False):
+  path = lib_path(fast)
+  if path not in _libs:
+    # The MLP port alternates BLAS calls (numpy's own thread pool) with OpenMP loops:
+    # threads that spin while the other pool works halve its speed, so OpenMP threads
+    # sleep between parallel regions (read by libgomp when it first loads).
+    os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+    if not os.path.exists(path):
+      raise ImportError(f"{path} missing: run make -C oracle")
+    lib = ctypes.CDLL(path)
+    lib.cport_obabo_first.restype = ctypes.c_double
+    lib.cport_obabo_second.restype = ctypes.c_double
+    lib.cport_full_potential.restype = ctypes.c_double
+    _libs[path] = lib
+  return _libs[path]
 
 
 def _p(a):
@@ -27,9 +53,9 @@ def _p(a):
 
 
 def sghmc(family, X, y, n, theta, keys, dkeys, eps, steps, friction, sigma=1.0,
-          prior_scale=1.0, threads=1):
+          prior_scale=1.0, threads=1, fast=False):
   """Runs len(eps) SGHMC iterations for theta.shape[0] chains in place."""
-  lib = load()
+  lib = load(fast)
   X = np.ascontiguousarray(X, np.float32)
   y = np.ascontiguousarray(y, np.float32)
   eps = np.ascontiguousarray(eps, np.float32)
@@ -44,8 +70,8 @@ def sghmc(family, X, y, n, theta, keys, dkeys, eps, steps, friction, sigma=1.0,
 
 
 def sgld(family, X, y, n, theta, keys, dkeys, eps, temperature=1.0, sigma=1.0,
-         prior_scale=1.0, threads=1):
-  lib = load()
+         prior_scale=1.0, threads=1, fast=False):
+  lib = load(fast)
   X = np.ascontiguousarray(X, np.float32)
   y = np.ascontiguousarray(y, np.float32)
   eps = np.ascontiguousarray(eps, np.float32)
@@ -70,10 +96,10 @@ def amagold_init(theta0, key):
 
 
 def amagold(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, steps, friction,
-            sigma=1.0, prior_scale=1.0, threads=1):
+            sigma=1.0, prior_scale=1.0, threads=1, fast=False):
   """Runs len(eps) AMAGOLD iterations for theta.shape[0] chains in place; returns the
   accept decisions [C, iterations]."""
-  lib = load()
+  lib = load(fast)
   X = np.ascontiguousarray(X, np.float32)
   y = np.ascontiguousarray(y, np.float32)
   eps = np.ascontiguousarray(eps, np.float32)
@@ -89,10 +115,10 @@ def amagold(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, step
 
 
 def sggmc(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, steps, friction,
-          temperature=1.0, sigma=1.0, prior_scale=1.0, threads=1):
+          temperature=1.0, sigma=1.0, prior_scale=1.0, threads=1, fast=False):
   """Runs len(eps) SGGMC iterations (OBABO proposals, full-data MH test) for
   theta.shape[0] chains in place; returns the accept decisions [C, iterations]."""
-  lib = load()
+  lib = load(fast)
   X = np.ascontiguousarray(X, np.float32)
   y = np.ascontiguousarray(y, np.float32)
   eps = np.ascontiguousarray(eps, np.float32)
@@ -110,7 +136,6 @@ def sggmc(family, X, y, n, theta, mom, keys, mh_keys, dkeys, U_full, eps, steps,
 def full_potential(family, X, y, theta, sigma=1.0, prior_scale=1.0):
   """FP64 full-data potential of one position (the C port's MH reference value)."""
   lib = load()
-  lib.cport_full_potential.restype = ctypes.c_double
   X = np.ascontiguousarray(X, np.float32)
   y = np.ascontiguousarray(y, np.float32)
   w = np.ascontiguousarray(theta, np.float32)
@@ -125,3 +150,43 @@ def normal(key, S):
   k = np.ascontiguousarray(key, np.uint32)
   lib.cport_normal(_p(k), ctypes.c_uint32(S), _p(out))
   return out
+
+
+# ---- helpers of the float32 MLP port (oracle/mlp_port.py) ----
+
+def tree_normals(key, sizes, threads=1, fast=False):
+  """random_tree(key) over a flat vector with leaves of the given sizes."""
+  lib = load(fast)
+  sz = np.ascontiguousarray(sizes, np.int64)
+  out = np.empty(int(sz.sum()), np.float32)
+  k = np.ascontiguousarray(key, np.uint32)
+  lib.cport_tree_normals(_p(k), _p(sz), ctypes.c_int(sz.size), _p(out), ctypes.c_int(threads))
+  return out
+
+
+def draw_indices(dkey, n, N, fast=False):
+  """Advances dkey (uint32[2], in place) and returns the n minibatch indices."""
+  lib = load(fast)
+  idx = np.empty(n, np.int32)
+  lib.cport_draw_indices(_p(dkey), ctypes.c_uint32(n), ctypes.c_uint32(N), _p(idx))
+  return idx
+
+
+def sgld_update(theta, v, g, xi, eps, noise_scale, alpha, lmbd, rms, threads=1, fast=False):
+  load(fast).cport_sgld_update(
+      _p(theta), _p(v) if v is not None else None, _p(g), _p(xi), ctypes.c_int64(theta.size),
+      ctypes.c_float(eps), ctypes.c_float(noise_scale), ctypes.c_float(alpha),
+      ctypes.c_float(lmbd), ctypes.c_int(1 if rms else 0), ctypes.c_int(threads))
+
+
+def obabo_first(theta, p, g, xi, eps, decay, noise, threads=1, fast=False):
+  return load(fast).cport_obabo_first(_p(theta), _p(p), _p(g), _p(xi),
+                                      ctypes.c_int64(theta.size), ctypes.c_float(eps),
+                                      ctypes.c_float(decay), ctypes.c_float(noise),
+                                      ctypes.c_int(threads))
+
+
+def obabo_second(p, g, xi, eps, decay, noise, threads=1, fast=False):
+  return load(fast).cport_obabo_second(_p(p), _p(g), _p(xi), ctypes.c_int64(p.size),
+                                       ctypes.c_float(eps), ctypes.c_float(decay),
+                                       ctypes.c_float(noise), ctypes.c_int(threads))
