@@ -117,6 +117,19 @@ def initial_burn_in(n=0):
                             lambda s, i, **kw: F32(1.0) if s <= i else F32(0.0))
 
 
+def gumbel_topk(uniforms, probs, k):
+  """The draw of jax.random.choice(key, arange(n), (k,), replace=False, p=probs) given
+  u = uniform(key, (n,), minval=tiny): the Gumbel top-k construction, the k smallest
+  entries of -gumbel - log p with gumbel = -log(-log u), in float32 (zero-probability
+  entries rank last). Returned in increasing order (the scheduler only tests
+  membership)."""
+  u = np.asarray(uniforms, np.float32)
+  with np.errstate(divide="ignore"):
+    gumbel = -np.log(-np.log(u))
+    g = -gumbel - np.log(np.asarray(probs, np.float32))
+  return np.sort(np.argsort(g, kind="stable")[:k])
+
+
 def random_thinning(step_size_schedule, burn_in_schedule, selections, key=None):
   """scheduler.py:599-666: ``selections`` iterations drawn without replacement with
   probability proportional to step_size * burn_in, by the Gumbel top-k construction
@@ -134,11 +147,7 @@ def random_thinning(step_size_schedule, burn_in_schedule, selections, key=None):
     assert np.count_nonzero(probs) >= selections, "Cannot select enough values"
     k = R.PRNGKey(0) if key is None else key
     u = R.uniform(k, (iterations,), np.finfo(np.float32).tiny, 1.0).cpu().numpy()
-    with np.errstate(divide="ignore"):
-      gumbel = -np.log(-np.log(u.astype(np.float32)))
-      g = -gumbel - np.log(probs)
-    accepted = np.argsort(g, kind="stable")[:selections]
-    return np.sort(accepted), selections
+    return gumbel_topk(u, probs, selections), selections
 
   def get_fn(state, iteration, **kw):
     return bool(np.any(state == iteration))

@@ -382,3 +382,23 @@ def test_obabo_kernels():
          P, eps, L.stream())
   p1 = p.cpu().numpy()  # continue from the device state
   np.testing.assert_allclose(th.cpu().numpy(), theta + f(eps) * p1, rtol=1e-6, atol=1e-6)
+
+
+def test_random_thinning_on_device_uniforms():
+  """scheduler.random_thinning (scheduler.py:599-666) draws its Gumbel keys from the
+  device generator: the selection equals the Gumbel top-k of the oracle's uniforms of
+  the same key (bit-identical words), burn-in iterations excluded."""
+  print("random_thinning", flush=True)
+  from jax_sgmc_amd import scheduler as SC
+  steps = SC.polynomial_step_size_first_last(first=1e-3, last=1e-4)
+  burn = SC.initial_burn_in(20)
+  th = SC.random_thinning(steps, burn, 15, key=R.PRNGKey(3, device=DEV))
+  state, count = th.init(100)
+  eps = steps.init(100)
+  bst, _ = burn.init(100)
+  probs = np.array([eps[i] * burn.get(bst, i) for i in range(100)], np.float32)
+  u = jr.uniform(jr.PRNGKey(3), (100,), np.finfo(np.float32).tiny, 1.0)
+  assert count == 15
+  np.testing.assert_array_equal(state, SC.gumbel_topk(u, probs, 15))
+  assert state.min() >= 20
+  assert th.get(state, int(state[0])) and not th.get(state, 0)

@@ -86,3 +86,44 @@ def test_mass_matrix_matches_welford_closed_form():
     m = mm.get(st)
     np.testing.assert_allclose(m.inv.tensor.numpy(), ref[t][0], rtol=1e-10)
     np.testing.assert_allclose(m.sqrt.tensor.numpy(), ref[t][1], rtol=1e-10)
+
+
+def _signs(v, M):
+  """Coefficients of v in the eigenbasis of the symmetric matrix M, each divided by the
+  square root of its eigenvalue: +-1 each when v = U sqrt(s) for an SVD U s V^T of M."""
+  lam, E = np.linalg.eigh(M)
+  return (E.T @ v) / np.sqrt(lam)
+
+
+def test_full_noise_model_up_to_singular_vector_signs():
+  """fisher_information(diagonal=False) (adaption.py:441-452) returns U sqrt(s) of the SVDs
+  of diag(friction) - b and of b: a vector whose value depends on the sign of each
+  singular vector, which LAPACK (and the GPU solvers) leave unspecified. What is fixed
+  is that every coefficient of that vector in the eigenbasis of the (symmetric,
+  positive semi-definite) matrix is +-sqrt(s_j): checked for the oracle and for the
+  library (torch, here on the CPU). A square root left out, or U transposed, fails."""
+  X, y, idx, mb = _batch(D=5, n=16)
+  pot = PT.minibatch_potential(LK.GaussianPrior(1.0), LK.Logistic(), strategy="vmap")
+  w = np.random.default_rng(4).standard_normal(5).astype(np.float32) * 0.3
+  fam = om.Family("logistic", X, y)
+  _, g, _ = om.stochastic_potential(fam, w.astype(np.float64), idx, 1.0)
+  _, G = om.logistic(w, X[idx], y[idx])
+  eps, fric = 0.05, 0.8
+  diff = G - g / 200
+  b = 0.5 * np.float32(eps) * (diff.T @ diff / 15)
+  Mcb = np.diag(np.full(5, fric)) - b
+  assert np.linalg.eigvalsh(b).min() > 1e-9 and np.linalg.eigvalsh(Mcb).min() > 1e-3
+  cb, bs = ost.fisher_noise(G, g, 200, 16, fric, eps, diagonal=False)
+  np.testing.assert_allclose(np.abs(_signs(cb, Mcb)), 1.0, rtol=1e-8)
+  np.testing.assert_allclose(np.abs(_signs(bs, b)), 1.0, rtol=1e-8)
+  nm = AD.fisher_information(minibatch_potential=pot, diagonal=False)
+  out = nm.get(nm.init(torch.zeros(5)), torch.as_tensor(w), torch.as_tensor(g, dtype=torch.float32),
+               torch.full((5,), fric), mini_batch=mb, step_size=eps)
+  np.testing.assert_allclose(np.abs(_signs(out.cb_diff_sqrt.tensor.double().numpy(), Mcb)), 1.0,
+                             rtol=2e-3)
+  np.testing.assert_allclose(np.abs(_signs(out.b_sqrt.tensor.double().numpy(), b)), 1.0,
+                             rtol=2e-3)
+  # planted mistakes: singular values without the square root, U transposed
+  lam, E = np.linalg.svd(b)[1], np.linalg.svd(b)[0]
+  assert not np.allclose(np.abs(_signs(E @ lam, b)), 1.0, rtol=1e-2)
+  assert not np.allclose(np.abs(_signs(E.T @ np.sqrt(lam), b)), 1.0, rtol=1e-2)
