@@ -61,6 +61,19 @@ constexpr int NCLS = 10;
 constexpr int NLEAF = 6;     // W1, W2, W3, b1, b2, b3
 constexpr int kStrip = 32;   // rows per partial sum of the small leaves' gradients
 constexpr int kFinalBlocks = 64;
+// Update epilogue tuning: parameters pairs per prefetch chunk (1, 2, 4 or 8 of a lane's
+// 16 pairs) and the waves per SIMD the update GEMMs are compiled for (register budget).
+#ifndef SGMC_UPD_CHUNK
+#define SGMC_UPD_CHUNK 2
+#endif
+#ifndef SGMC_UPD_NT
+#define SGMC_UPD_NT 0
+#endif
+#ifndef SGMC_UPD_WAVES
+#define SGMC_UPD_WAVES 1
+#endif
+constexpr int kUpdChunk = SGMC_UPD_CHUNK;
+static_assert(16 % kUpdChunk == 0, "chunks of the 16 pairs");
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -179,50 +192,10 @@ __device__ __forceinline__ void q_note_amax_block(const Step& s, int c, int t, f
   }
 }
 
-// Shadow copy of an updated weight: bf16, or e4m3 at the next weight scale (W1, W2).
-__device__ __forceinline__ void write_shadow(const Step& s, int c, int leaf, int64_t base,
-                                             int64_t i, float wn, float& wmax) {
-  if (s.fp32) return;  // the float32 GEMMs read theta itself
-  if (s.fp8 && leaf < 2) {
-    s.shadow8[base + i] = f2q(wn * s.q[(int64_t)c * kQWords + 6 + leaf]);
-    wmax = fmaxf(wmax, fabsf(wn));
-  } else {
-    s.shadow[base + i] = f2bf(wn);
-  }
-}
-
 enum MlpMode { MLP_SGLD = 0, MLP_OBABO_BA = 1, MLP_OBABO_BO = 2 };
 
 __device__ __forceinline__ int cur_iter(const Step& s) {
   return max(0, min(s.iter[0], s.table_len - 1));
-}
-
-// SGLD / rms_prop update of one parameter (integrator.py:882-912, adaption.py:254-291).
-__device__ __forceinline__ void sgld_param(const Step& s, int c, int leaf, int64_t base,
-                                           int64_t i, float gsum, float xi, float eps,
-                                           float noise_scale, float& lp, float& wmax) {
-  float* th = s.theta + base;
-  float* v = s.v ? s.v + base : nullptr;
-  const float w = th[i];
-  float g = gsum;
-  if (s.has_prior) {
-    g = g + w * s.prior_inv_var;
-    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
-  }
-  g = g / s.temperature;
-  const float sg = (-eps) * g;
-  const float sn = noise_scale * xi;
-  float wn;
-  if (s.rms) {
-    const float vn = s.alpha * v[i] + (1.0f - s.alpha) * (g * g);
-    v[i] = vn;
-    const float G = 1.0f / (s.lmbd + sqrtf(vn));
-    wn = w + ((0.0f + G * sg) + sqrtf(G) * sn);
-  } else {
-    wn = w + (sg + sn);
-  }
-  th[i] = wn;
-  write_shadow(s, c, leaf, base, i, wn, wmax);
 }
 
 // Gradient-only variant: grad = (gsum + prior) / T, no update.
@@ -236,6 +209,31 @@ __device__ __forceinline__ void grad_param(const Step& s, const float* th, float
   }
   gout[i] = g / s.temperature;
 }
+
+
+// Parameter update of one leaf of chain c, applied to values so that a thread can
+// issue the loads of all its parameters before the arithmetic. The leaf's arrays are
+// addressed from uniform base pointers with 32-bit offsets (a leaf holds < 2^32
+// entries). Update rules (prior gradient and temperature folded into g = (gsum + w /
+// s^2) / T, written with fused multiply-adds: within an ulp or two of the reference's
+// separate roundings, far inside the bf16 / e4m3 GEMM error):
+//   SGLD (integrator.py:882-912): w += (-eps) g + sqrt(2 T eps) xi; with rms_prop
+//     (adaption.py:254-291): v = alpha v + (1 - alpha) g^2, G = 1 / (lmbd + sqrt(v)),
+//     w += G (-eps) g + sqrt(G) sqrt(2 T eps) xi (hardware reciprocal and square roots);
+//   OBABO with the identity mass (integrator.py:181-273), in two halves:
+//     BA: p1 = sqrt(pi) p + sqrt((1 - pi) T) xi(s1); KE_start += p1^2 / 2;
+//         p2 = (-eps/2) g + p1; w += eps p2
+//     BO: p3 = (-eps/2) g(w') + p2; KE_end += p3^2 / 2; p = sqrt(pi) p3 + ... xi(s2).
+struct Upd {
+  float* th;          // theta of the leaf
+  float* aux;         // rms_prop v or OBABO momentum of the leaf, or null (plain SGLD)
+  uint16_t* sh;       // bf16 shadow of the leaf, or null
+  uint8_t* sh8;       // e4m3 shadow (W1, W2 in fp8 mode), or null
+  float q8;           // scale the e4m3 shadow is written with
+  int mode, rms, has_prior;
+  float inv_var, log_norm, inv_T, eps, noise_scale, decay, noise, half, alpha, one_m_alpha,
+      lmbd;
+};
 
 // Per-kernel step coefficients of the current iteration.
 struct Coef {
@@ -252,115 +250,99 @@ __device__ __forceinline__ Coef coef_of(const Step& s) {
   return k;
 }
 
-// OBABO halves for one parameter with the identity mass (integrator.py:181-273):
-//   BA: p1 = sqrt(pi) p + sqrt((1 - pi) T) xi(s1); KE_start += p1 p1 / 2;
-//       p2 = (-eps/2) g(theta) + p1; theta' = theta + eps p2
-//   BO: p3 = (-eps/2) g(theta') + p2; KE_end += p3 p3 / 2; p' = sqrt(pi) p3 + ... xi(s2)
-// with g the potential's gradient at the evaluation point (prior and temperature in).
-__device__ __forceinline__ void obabo_param(const Step& s, int c, int leaf, int64_t base,
-                                            int64_t i, float gsum, float xi, const Coef& k,
-                                            float& lp, float& ke, float& wmax) {
-  float* th = s.theta + base;
-  float* mom = s.mom + base;
-  const float w = th[i];
+__device__ __forceinline__ Upd make_upd(const Step& s, int c, int leaf, int64_t base,
+                                        const Coef& k) {
+  Upd u;
+  u.th = s.theta + base;
+  u.aux = s.mode == MLP_SGLD ? (s.rms ? s.v + base : nullptr) : s.mom + base;
+  u.sh = nullptr;
+  u.sh8 = nullptr;
+  u.q8 = 1.0f;
+  if (s.fp8 && leaf < 2) {
+    u.sh8 = s.shadow8 + base;
+    u.q8 = s.q[(int64_t)c * kQWords + 6 + leaf];
+  } else if (!s.fp32) {
+    u.sh = s.shadow + base;
+  }
+  u.mode = s.mode;
+  u.rms = s.rms;
+  u.has_prior = s.has_prior;
+  u.inv_var = s.prior_inv_var;
+  u.log_norm = s.prior_log_norm;
+  u.inv_T = 1.0f / s.temperature;
+  u.eps = k.eps;
+  u.noise_scale = k.noise_scale;
+  u.decay = k.decay;
+  u.noise = k.noise;
+  u.half = 0.5f * k.eps;
+  u.alpha = s.alpha;
+  u.one_m_alpha = 1.0f - s.alpha;
+  u.lmbd = s.lmbd;
+  return u;
+}
+
+// Parameter state is streamed (read once, written once per launch): with SGMC_UPD_NT the
+// accesses carry the non-temporal hint.
+template <class T>
+__device__ __forceinline__ T upd_ld(const T* p) {
+  if (SGMC_UPD_NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <class T>
+__device__ __forceinline__ void upd_st(T v, T* p) {
+  if (SGMC_UPD_NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+__device__ __forceinline__ void upd_load(const Upd& u, uint32_t i, float& w, float& aux) {
+  w = upd_ld(u.th + i);
+  aux = u.aux ? upd_ld(u.aux + i) : 0.0f;
+}
+
+__device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, float gsum,
+                                         float xi, float& lp, float& ke) {
   float g = gsum;
-  if (s.has_prior) {
-    g = g + w * s.prior_inv_var;
-    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
+  if (u.has_prior) {
+    g = __builtin_fmaf(w, u.inv_var, g);
+    lp = __builtin_fmaf(__builtin_fmaf(w * w, u.inv_var, u.log_norm), -0.5f, lp);
   }
-  g = g / s.temperature;
-  const float half = 0.5f * k.eps;
-  if (s.mode == MLP_OBABO_BA) {
-    const float p1 = k.decay * mom[i] + k.noise * xi;
-    ke += p1 * p1;
-    const float p2 = (-half) * g + p1;
-    mom[i] = p2;
-    const float wn = w + k.eps * p2;
-    th[i] = wn;
-    write_shadow(s, c, leaf, base, i, wn, wmax);
-  } else {
-    const float p3 = (-half) * g + mom[i];
-    ke += p3 * p3;
-    mom[i] = k.decay * p3 + k.noise * xi;
-  }
-}
-
-__device__ __forceinline__ void update_param(const Step& s, int c, int leaf, int64_t base,
-                                             int64_t i, float gsum, float xi, const Coef& k,
-                                             float& lp, float& ke, float& wmax) {
-  if (s.mode == MLP_SGLD)
-    sgld_param(s, c, leaf, base, i, gsum, xi, k.eps, k.noise_scale, lp, wmax);
-  else
-    obabo_param(s, c, leaf, base, i, gsum, xi, k, lp, ke, wmax);
-}
-
-// The same updates on values, so that a thread can issue the loads of all its
-// parameters before the arithmetic: w and aux (the rms_prop second moment, or the
-// OBABO momentum) are loaded, advanced, then stored.
-struct PState {
-  float w, aux;
-};
-
-__device__ __forceinline__ PState load_pstate(const Step& s, int64_t gi) {
-  PState p;
-  p.w = s.theta[gi];
-  p.aux = s.mode == MLP_SGLD ? (s.rms ? s.v[gi] : 0.0f) : s.mom[gi];
-  return p;
-}
-
-__device__ __forceinline__ void step_pstate(const Step& s, PState& p, float gsum, float xi,
-                                            const Coef& k, float& lp, float& ke) {
-  const float w = p.w;
-  float g = gsum;
-  if (s.has_prior) {
-    g = g + w * s.prior_inv_var;
-    lp += (s.prior_log_norm + (w * w) * s.prior_inv_var) / -2.0f;
-  }
-  g = g / s.temperature;
-  if (s.mode == MLP_SGLD) {
-    const float sg = (-k.eps) * g;
-    const float sn = k.noise_scale * xi;
-    if (s.rms) {
-      // G = 1 / (lambda + sqrt(v)) and sqrt(G) with the hardware reciprocal and square
-      // root (about 1 ulp each) instead of the correctly rounded sequences: this
-      // epilogue is bound by its per-parameter arithmetic, and the bf16 / fp8 GEMM path
-      // it serves is compared at a far looser tolerance.
-      const float vn = s.alpha * p.aux + (1.0f - s.alpha) * (g * g);
-      p.aux = vn;
-      const float G = __builtin_amdgcn_rcpf(s.lmbd + __builtin_amdgcn_sqrtf(vn));
-      p.w = w + ((0.0f + G * sg) + __builtin_amdgcn_sqrtf(G) * sn);
+  g = g * u.inv_T;
+  if (u.mode == MLP_SGLD) {
+    const float sg = (-u.eps) * g;
+    const float sn = u.noise_scale * xi;
+    if (u.rms) {
+      const float vn = __builtin_fmaf(u.alpha, aux, u.one_m_alpha * (g * g));
+      aux = vn;
+      const float G = __builtin_amdgcn_rcpf(u.lmbd + __builtin_amdgcn_sqrtf(vn));
+      w = w + __builtin_fmaf(__builtin_amdgcn_sqrtf(G), sn, G * sg);
     } else {
-      p.w = w + (sg + sn);
+      w = w + (sg + sn);
     }
-  } else if (s.mode == MLP_OBABO_BA) {
-    const float half = 0.5f * k.eps;
-    const float p1 = k.decay * p.aux + k.noise * xi;
-    ke += p1 * p1;
-    const float p2 = (-half) * g + p1;
-    p.aux = p2;
-    p.w = w + k.eps * p2;
+  } else if (u.mode == MLP_OBABO_BA) {
+    const float p1 = __builtin_fmaf(u.decay, aux, u.noise * xi);
+    ke = __builtin_fmaf(p1, p1, ke);
+    const float p2 = __builtin_fmaf(-u.half, g, p1);
+    aux = p2;
+    w = __builtin_fmaf(u.eps, p2, w);
   } else {
-    const float half = 0.5f * k.eps;
-    const float p3 = (-half) * g + p.aux;
-    ke += p3 * p3;
-    p.aux = k.decay * p3 + k.noise * xi;
+    const float p3 = __builtin_fmaf(-u.half, g, aux);
+    ke = __builtin_fmaf(p3, p3, ke);
+    aux = __builtin_fmaf(u.decay, p3, u.noise * xi);
   }
 }
 
-__device__ __forceinline__ void store_pstate(const Step& s, int c, int leaf, int64_t base,
-                                             int64_t i, const PState& p, float& wmax) {
-  const int64_t gi = base + i;
-  if (s.mode == MLP_SGLD) {
-    s.theta[gi] = p.w;
-    if (s.rms) s.v[gi] = p.aux;
-    write_shadow(s, c, leaf, base, i, p.w, wmax);
-  } else if (s.mode == MLP_OBABO_BA) {
-    s.mom[gi] = p.aux;
-    s.theta[gi] = p.w;
-    write_shadow(s, c, leaf, base, i, p.w, wmax);
-  } else {
-    s.mom[gi] = p.aux;
+__device__ __forceinline__ void upd_store(const Upd& u, uint32_t i, float w, float aux,
+                                          float& wmax) {
+  if (u.mode != MLP_OBABO_BO) {
+    upd_st(w, u.th + i);
+    if (u.sh8) {
+      upd_st(f2q(w * u.q8), u.sh8 + i);
+      wmax = fmaxf(wmax, fabsf(w));
+    } else if (u.sh) {
+      upd_st(f2bf(w), u.sh + i);
+    }
   }
+  if (u.aux) upd_st(aux, u.aux + i);
 }
 
 // Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
@@ -667,7 +649,8 @@ __device__ __forceinline__ float get_act(const Step& s, int c, int t, const void
 }
 
 template <int BK, int PREC, bool A_K, bool B_K, int EPI>
-__global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
+__global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
+    gemm_kernel(Gemm g, Step s) {
   static_assert(BN == 2 * BM, "tile shapes");
   constexpr bool F8 = PREC == P_FP8;
   constexpr int EB = elem_bytes<PREC>();
@@ -759,48 +742,52 @@ __global__ void __launch_bounds__(GT) gemm_kernel(Gemm g, Step s) {
     float lp = 0.0f, ke = 0.0f, wmax = 0.0f;
     if (EPI == EPI_UPDATE) {
       const Key lk = leaf_key(s, c, g.leaf);
-      // Four chunks of eight parameters, software-pipelined: the next chunk's loads go
-      // out before this chunk's noise arithmetic and stores (each parameter is read and
+      const Upd up = make_upd(s, c, g.leaf, base, k);
+      // 32-bit offsets of this lane's parameters: element j of band 0 at
+      // o0 + j_row * N, its band-1 pair half * N further
+      const uint32_t o0 = (uint32_t)(bm * 32 + 4 * (lane >> 5)) * (uint32_t)g.N + (uint32_t)n;
+      const uint32_t oh = (uint32_t)half * (uint32_t)g.N;
+      auto off = [&](int j) -> uint32_t {
+        return o0 + (uint32_t)(8 * (j >> 2) + (j & 3)) * (uint32_t)g.N;
+      };
+      // Chunks of kUpdChunk pairs, software-pipelined: the next chunk's loads go out
+      // before this chunk's noise arithmetic and stores (each parameter is read and
       // written by this thread only), so its memory time overlaps the arithmetic.
-      PState p0[4], p1[4];
+      constexpr int CJ = kUpdChunk;
+      float w0[CJ], a0[CJ], w1[CJ], a1[CJ];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int r = 8 * (u >> 2) + 4 * (lane >> 5) + (u & 3);
-        const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-        p0[u] = load_pstate(s, base + i0);
-        p1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+      for (int u = 0; u < CJ; ++u) {
+        upd_load(up, off(u), w0[u], a0[u]);
+        upd_load(up, off(u) + oh, w1[u], a1[u]);
       }
 #pragma unroll
-      for (int j0 = 0; j0 < 16; j0 += 4) {
-        PState q0[4], q1[4];
-        if (j0 + 4 < 16) {
+      for (int j0 = 0; j0 < 16; j0 += CJ) {
+        float nw0[CJ], na0[CJ], nw1[CJ], na1[CJ];
+        if (j0 + CJ < 16) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int j = j0 + 4 + u;
-            const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-            const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-            q0[u] = load_pstate(s, base + i0);
-            q1[u] = load_pstate(s, base + i0 + (int64_t)half * g.N);
+          for (int u = 0; u < CJ; ++u) {
+            upd_load(up, off(j0 + CJ + u), nw0[u], na0[u]);
+            upd_load(up, off(j0 + CJ + u) + oh, nw1[u], na1[u]);
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < CJ; ++u) {
           const int j = j0 + u;
-          const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-          const int64_t i0 = (int64_t)(bm * 32 + r) * g.N + n;
-          const int64_t i1 = i0 + (int64_t)half * g.N;
-          uint32_t w0, w1;
-          bits_pair(lk, S, (uint32_t)i0, w0, w1);
-          step_pstate(s, p0[u], ab * acc0[j], bits_to_normal(w0), k, lp, ke);
-          step_pstate(s, p1[u], ab * acc1[j], bits_to_normal(w1), k, lp, ke);
-          store_pstate(s, c, g.leaf, base, i0, p0[u], wmax);
-          store_pstate(s, c, g.leaf, base, i1, p1[u], wmax);
+          const uint32_t i0 = off(j);
+          uint32_t b0, b1;
+          bits_pair(lk, S, i0, b0, b1);
+          upd_step(up, w0[u], a0[u], ab * acc0[j], bits_to_normal_fast(b0), lp, ke);
+          upd_step(up, w1[u], a1[u], ab * acc1[j], bits_to_normal_fast(b1), lp, ke);
+          upd_store(up, i0, w0[u], a0[u], wmax);
+          upd_store(up, i0 + oh, w1[u], a1[u], wmax);
         }
-        if (j0 + 4 < 16) {
+        if (j0 + CJ < 16) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            p0[u] = q0[u];
-            p1[u] = q1[u];
+          for (int u = 0; u < CJ; ++u) {
+            w0[u] = nw0[u];
+            a0[u] = na0[u];
+            w1[u] = nw1[u];
+            a1[u] = na1[u];
           }
         }
       }
@@ -1045,9 +1032,12 @@ __global__ void __launch_bounds__(GT) small_kernel(Step s, int part0) {
     const int64_t base = (int64_t)c * s.P + s.off[leaf];
     if (s.update) {
       const Key lk = leaf_key(s, c, leaf);
-      const float xi = bits_to_normal(bits_word(lk, (uint32_t)s.size[leaf], (uint32_t)i));
-      float wmax = 0.f;
-      update_param(s, c, leaf, base, i, gsum, xi, cf, lp, ke, wmax);
+      const float xi = bits_to_normal_fast(bits_word(lk, (uint32_t)s.size[leaf], (uint32_t)i));
+      float wmax = 0.f, w, aux;
+      const Upd up = make_upd(s, c, leaf, base, cf);
+      upd_load(up, (uint32_t)i, w, aux);
+      upd_step(up, w, aux, gsum, xi, lp, ke);
+      upd_store(up, (uint32_t)i, w, aux, wmax);
     } else {
       grad_param(s, s.theta + base, s.grad + base, i, gsum, lp);
     }

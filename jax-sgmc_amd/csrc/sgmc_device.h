@@ -229,6 +229,46 @@ __device__ __forceinline__ float bits_to_normal(uint32_t bits) {
   return 1.41421354f * erfinv_giles(u);
 }
 
+// Standard normal for the MLP update epilogues, from the same word as bits_to_normal:
+// the same Giles polynomial, with w = -log1p(-x^2) taken as -ln2 log2(1 - x^2) on the
+// hardware log (x^2 rounded first, as log1p(-x * x) rounds it; 1 - x^2 is then exact for
+// x^2 >= 1/2 and within 3e-8 below) and the polynomial in fused multiply-adds. |x| < 1
+// always (the uniform lies in [-0.99999994, 0.99999982]). Within 3 ulp of
+// bits_to_normal over 8.4M words, about a fifth of its instructions (no division, no
+// divergent log branches).
+__device__ __forceinline__ float bits_to_normal_fast(uint32_t bits) {
+  const float lo = -0.99999994f;
+  const float f = unit_float(bits);
+  const float x = fmaxf(lo, f * 2.0f + lo);
+  const float x2 = x * x;
+  float w = -0.693147182f * __builtin_amdgcn_logf(1.0f - x2);
+  float p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = __builtin_fmaf(p, w, 3.43273939e-07f);
+    p = __builtin_fmaf(p, w, -3.5233877e-06f);
+    p = __builtin_fmaf(p, w, -4.39150654e-06f);
+    p = __builtin_fmaf(p, w, 0.00021858087f);
+    p = __builtin_fmaf(p, w, -0.00125372503f);
+    p = __builtin_fmaf(p, w, -0.00417768164f);
+    p = __builtin_fmaf(p, w, 0.246640727f);
+    p = __builtin_fmaf(p, w, 1.50140941f);
+  } else {
+    w = __builtin_amdgcn_sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = __builtin_fmaf(p, w, 0.000100950558f);
+    p = __builtin_fmaf(p, w, 0.00134934322f);
+    p = __builtin_fmaf(p, w, -0.00367342844f);
+    p = __builtin_fmaf(p, w, 0.00573950773f);
+    p = __builtin_fmaf(p, w, -0.0076224613f);
+    p = __builtin_fmaf(p, w, 0.00943887047f);
+    p = __builtin_fmaf(p, w, 1.00167406f);
+    p = __builtin_fmaf(p, w, 2.83297682f);
+  }
+  return 1.41421354f * (p * x);
+}
+
 // jax.random.uniform(key, (), lo, hi) from one word.
 __device__ __forceinline__ float bits_to_uniform(uint32_t bits, float lo, float hi) {
 #pragma clang fp contract(off)

@@ -244,8 +244,13 @@ def test_c5_obabo_half_step_follows_reference_rule(precision):
     p1 = decay * np.zeros_like(xi) + noise * xi
     p2 = (-half) * g[c] + p1
     ref = th[c] + f(eps) * p2
-    np.testing.assert_allclose(eng.mom[c].cpu().numpy(), p2, rtol=1e-6, atol=1e-9)
-    np.testing.assert_allclose(eng.theta[c].cpu().numpy(), ref, rtol=1e-6, atol=1e-9)
+    # the kernel's update is written with fused multiply-adds and its normals are within
+    # 3 ulp of the oracle's: agreement to 1e-6 relative, with an absolute floor of 1e-6
+    # of the noise term where p2 = p1 - (eps/2) g nearly cancels
+    floor = 1e-6 * float(np.abs(p1).max())
+    np.testing.assert_allclose(eng.mom[c].cpu().numpy(), p2, rtol=1e-6, atol=floor)
+    np.testing.assert_allclose(eng.theta[c].cpu().numpy(), ref, rtol=1e-6,
+                               atol=f(eps) * floor)
     ks = eng.ke[c, 0].item()
     ks_ref = 0.5 * np.dot(p1.astype(np.float64), p1.astype(np.float64))
     assert abs(ks - ks_ref) <= 1e-5 * ks_ref, (ks, ks_ref)
