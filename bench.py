@@ -518,6 +518,23 @@ def mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, dims, n, prec, integ
   return out
 
 
+def tune_tiles(eng, args):
+  """Times the GEMM tilings on the live problem (FusedMLP.autotune) unless disabled;
+  returns the plan and its provenance for the bench line's config."""
+  names = ("G1 Xb.W1", "G2 h1.W2", "G3 dA2.W2^T", "G4 h1^T.dA2", "G5 Xb^T.dA1")
+  if args.no_autotune:
+    plan, how = eng.gemm_plan(), {"source": "built-in"}
+  else:
+    log("autotune: GEMM k-tile depth and split-K on the live problem")
+    plan = eng.autotune()
+    b, t = eng.tune_gain
+    how = {"source": "autotune", "unit_ms_builtin": b, "unit_ms_tuned": t,
+           "unit_ms": {f"{names[i].split()[0]} k{c >> 4} s{c & 15}": round(ms, 5)
+                       for (i, c), ms in sorted(eng.tune_ms.items())}}
+  tiles = {nm: {"BM": 64, "BN": 128, "BK": k, "split_k": s} for nm, (k, s) in zip(names, plan)}
+  return dict(how, tiles=tiles)
+
+
 def run_mlp(args):
   import torch
   import __graft_entry__
@@ -536,6 +553,7 @@ def run_mlp(args):
   prec = args.precision or "bf16"
 
   def timed(eng, iters):
+    eng.gemm_tiles = tune_tiles(eng, args)
     g = eng.capture(G)
     for _ in range(max(1, W // G)):
       g.replay()
@@ -554,6 +572,7 @@ def run_mlp(args):
   eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0, eps, keys=keys, dkeys=dkeys, rms=True,
                    prior_scale=1.0, precision=prec)
   ms, iters = timed(eng, K)
+  gemm_tiles = eng.gemm_tiles
   del eng
   value = C * iters / (ms / 1e3)
   macs = D_in * H1 + H1 * H2 + H2 * 10
@@ -567,7 +586,7 @@ def run_mlp(args):
          "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
          "config": {"workload": f"pSGLD (rms_prop 0.9, 1e-5) MLP 3072-512-512-10, N=50000, "
                                 f"batch 256, {C} chains", "chains": C, "P": int(th0.shape[1]),
-                    "batch": MLP_BATCH, "N": MLP_N},
+                    "batch": MLP_BATCH, "N": MLP_N, "gemm_tiles": gemm_tiles},
          "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                       "kernel": "all kernels of sgmc_mlp_step (GEMM flops per iteration)",
@@ -577,10 +596,12 @@ def run_mlp(args):
     eng = M.FusedMLP(X, labels, MLP_DIMS, MLP_BATCH, th0[:1], eps, keys=keys[:1],
                      dkeys=dkeys[:1], rms=True, prior_scale=1.0, precision="fp32")
     pms, pits = timed(eng, min(K, 300))
+    plain_tiles = eng.gemm_tiles
     del eng
     plain = pits / (pms / 1e3)
     out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
-                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain"}
+                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain",
+                    "gemm_tiles": plain_tiles}
     out["fused_over_plain"] = value / plain
     log(f"posterior moments: {prec} against fp32 ({C} chains)")
     out["posterior"] = mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, MLP_DIMS,
@@ -641,6 +662,7 @@ def run_sggmc_mlp(args):
   prec = args.precision or "fp8"
 
   def timed(eng, iters):
+    eng.gemm_tiles = tune_tiles(eng, args)
     g = eng.capture(1)
     for _ in range(W):
       g.replay()
@@ -675,7 +697,8 @@ def run_sggmc_mlp(args):
          "data": "synthetic (U[0,1] inputs from PRNGKey(5), teacher-MLP labels)",
          "config": {"workload": "SGGMC: OBABO 100 steps per MH test, MLP 3072-1024-1024-10, "
                                 "batch 512, N=50000, float64 full-data potential",
-                    "chains": C, "P": eng.P, "batch": n, "N": MLP_N, "obabo_steps": steps},
+                    "chains": C, "P": eng.P, "batch": n, "N": MLP_N, "obabo_steps": steps,
+                    "gemm_tiles": eng.gemm_tiles},
          "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                       "kernel": "all kernels of one SGGMC iteration (GEMM flops)",
@@ -690,10 +713,12 @@ def run_sggmc_mlp(args):
                      prior_scale=1.0, integrator="sggmc", friction=1.0, steps=steps,
                      precision="fp32")
     pms = timed(eng, min(K, 5))
+    plain_tiles = eng.gemm_tiles
     del eng
     plain = min(K, 5) / (pms / 1e3)
     out["plain"] = {"value": plain, "unit": "chain-steps/s", "chains": 1,
-                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain"}
+                    "path": "the same engine with float32 GEMMs (precision fp32) on one chain",
+                    "gemm_tiles": plain_tiles}
     out["fused_over_plain"] = value / plain
     log(f"posterior moments: {prec} against fp32 ({C} chains)")
     out["posterior"] = mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, dims, n, prec,
@@ -1100,6 +1125,9 @@ def main():
   ap.add_argument("--warmup", type=int, default=50)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--no-autotune", action="store_true",
+                  help="MLP workloads: keep the built-in GEMM tiling instead of timing the "
+                       "candidates on the live problem before the timed region")
   ap.add_argument("--copy-calibration", action="store_true",
                   help="only copy a 512 MiB device buffer 5 times (the known-size kernel "
                        "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")

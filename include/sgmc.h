@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SGMC_ABI_VERSION 1
+#define SGMC_ABI_VERSION 2
 
 #define SGMC_OK 0
 #define SGMC_E_ARG 1 /* invalid argument: null pointer, shape, alignment, size */
@@ -459,6 +459,13 @@ typedef struct sgmc_mlp_state {
   int32_t precision;
   uint8_t* shadow8;        /* [C][P] e4m3 copy of W1, W2 (precision 1) */
   float* q;                /* [C][16] scale words (precision 1) */
+  /* GEMM tiling per call site -- G1 = Xb W1, G2 = h1 W2, G3 = dA2 W2^T, G4 = h1^T dA2
+   * (gW2), G5 = Xb^T dA1 (gW1): 0 keeps the built-in choice, else 16 k + s with k the
+   * k-tile depth (bf16 32 or 128, e4m3 64 or 128, float32 32) and s the split-K factor
+   * (1, 2 or 4; 1 for G4, G5), the GEMM's K a multiple of k s. The k order of the
+   * matrix instructions changes with the split only for the forward GEMMs, whose
+   * partial products are added in split order. Set by FusedMLP.autotune. */
+  int32_t tune[5];
 } sgmc_mlp_state;
 
 int64_t sgmc_mlp_param_count(int D_in, int H1, int H2);
@@ -468,6 +475,13 @@ int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* s
  * e4m3 shadows and sets the activation scales to 1 (an evaluation in gradient mode then
  * calibrates them). */
 int sgmc_mlp_init_fp8(const sgmc_mlp_state* st, void* stream);
+/* The tiling sgmc_mlp_step launches (update = its update flag) for these shapes and a
+ * tune[5] array (see sgmc_mlp_state; NULL = all zero): plan[i] = 16 k + s of GEMM i,
+ * the built-in choice where tune[i] is 0. Validates tune like sgmc_mlp_step (SGMC_E_ARG
+ * with the message in sgmc_last_error). Host only, no GPU work. No counterpart in the
+ * reference (XLA picks its own GEMM tiling under jit). */
+int sgmc_mlp_gemm_plan(int D_in, int H1, int H2, int n, int C, int precision, int update,
+                       const int32_t* tune, int32_t* plan);
 int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream);
 /* full_potential (potential.py:219-293) of every chain's current theta: the
  * likelihood of all N rows, n rows per pass, summed in float64 with the float64 log

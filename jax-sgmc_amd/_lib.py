@@ -18,6 +18,7 @@ c_int64 = ctypes.c_int64
 c_float = ctypes.c_float
 c_double = ctypes.c_double
 
+ABI_VERSION = 2  # SGMC_ABI_VERSION of the structures mirrored below
 OK = 0
 E_ARG = 1
 E_HIP = 2
@@ -71,6 +72,7 @@ class MlpState(ctypes.Structure):
       ("U_prop", c_void_p), ("theta_old", c_void_p), ("mom_old", c_void_p),
       ("U_old", c_void_p), ("mh_key", c_void_p), ("mh_stats", c_void_p),
       ("accepted", c_void_p), ("precision", c_int32), ("shadow8", c_void_p), ("q", c_void_p),
+      ("tune", c_int32 * 5),
   ]
 
 
@@ -154,6 +156,7 @@ _SIGNATURES = {
     "sgmc_mlp_workspace_bytes": (c_int64, [c_int] * 5),
     "sgmc_mlp_shadow": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "sgmc_mlp_step": (c_int, [ctypes.POINTER(MlpState), c_int, c_void_p]),
+    "sgmc_mlp_gemm_plan": (c_int, [c_int] * 7 + [c_void_p, c_void_p]),
 }
 
 # Entry points added after ABI version 1 register themselves here.
@@ -177,6 +180,10 @@ def load():
         "(python jax-sgmc_amd/build.py, or __graft_entry__.build()). There is no "
         "CPU fallback.")
   lib = ctypes.CDLL(LIB_PATH)
+  lib.sgmc_abi_version.restype = c_int
+  if lib.sgmc_abi_version() != ABI_VERSION:
+    raise ImportError(f"{LIB_PATH} has ABI version {lib.sgmc_abi_version()}, the bindings "
+                      f"expect {ABI_VERSION}: rebuild it (python jax-sgmc_amd/build.py)")
   for name, (res, args) in list(_SIGNATURES.items()) + list(EXTRA_SIGNATURES.items()):
     fn = getattr(lib, name)
     fn.restype = res

@@ -155,6 +155,7 @@ struct Step {
   // fp8 GEMM operands (Xb, h1, da2, da1 and the W1 / W2 shadows in e4m3)
   int fp8;
   int fp32;                // float32 activations and GEMM operands (weights from theta)
+  int plan[5];             // per GEMM (G1..G5): 16 k-tile + split-K factor
   float* q;                // [C][kQWords]
   uint8_t* shadow8;        // [C][P] (W1 and W2 leaves)
 };
@@ -1376,22 +1377,19 @@ int sgmc_mlp_shadow(const float* theta, uint16_t* shadow, int64_t count, void* s
 
 namespace {
 
-// Deep k-tiles (128) when K allows, so the register prefetch of the next tile has a
-// whole tile of MFMA work to land behind; 32 otherwise (K is a multiple of 32).
+// Split-K factor of the heuristic; a tuned GEMM may take 4 (the partial-product
+// workspace holds 4 C n max(H1, H2) floats).
 constexpr int kMaxSplits = 2;
 
+// GEMM call sites of one evaluation (the tune[] slots): G1 = Xb W1, G2 = H1 W2,
+// G3 = dA2 W2^T, G4 = H1^T dA2 (gW2), G5 = Xb^T dA1 (gW1).
+enum GemmSlot { SLOT_G1 = 0, SLOT_G2, SLOT_G3, SLOT_G4, SLOT_G5, kSlots };
+
 template <int BK, int PREC, bool A_K, bool B_K, int EPI>
-void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part) {
+void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part,
+                    int splits) {
   g.C = (int)grid.z;
-  g.splits = 1;
-  if (EPI == EPI_BIAS_RELU || EPI == EPI_MASK) {
-    // Split K until the grid holds about four blocks per CU: the forward GEMMs have
-    // few output tiles but long K.
-    const int64_t blocks = (int64_t)grid.x * grid.y * grid.z;
-    const int ktiles = g.K / BK;
-    while (g.splits < kMaxSplits && blocks * g.splits * 2 <= 1024 && ktiles % (g.splits * 2) == 0)
-      g.splits *= 2;
-  }
+  g.splits = splits;
   if (g.splits > 1) {
     g.part = part;
     hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI_PARTIAL>),
@@ -1404,23 +1402,77 @@ void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* par
   }
 }
 
-// bf16: k-tiles of 128 (or 32); e4m3: 128 (or 64); float32: 32 (the transposed tiles of
-// 4-byte elements fit one 4x4 block per thread). K is a multiple of 64. The bf16
-// parameter-update GEMMs take 32-deep k-tiles: their short K (the minibatch) gains
-// little from deep tiles, and the smaller staging registers give three waves per SIMD
-// instead of two to the update epilogue (C3 MLP: +3.4%). The k order of the matrix
-// instructions is the same either way, so the results are identical.
+// Launches GEMM `slot` with the k-tile depth and split-K factor of s.plan[slot].
 template <bool A_K, bool B_K, int EPI>
-void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st) {
-  const bool deep = g.K % 128 == 0 && (s.fp8 || EPI != EPI_UPDATE);
+void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st, int slot) {
+  const int tv = s.plan[slot];
+  const int splits = tv & 15;
+  const bool deep = (tv >> 4) == 128;
   if (s.fp32) {
-    launch_gemm_bk<32, P_F32, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    launch_gemm_bk<32, P_F32, A_K, B_K, EPI>(grid, g, s, st, s.split_part, splits);
   } else if (s.fp8) {
-    if (deep) launch_gemm_bk<128, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
-    else launch_gemm_bk<64, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    if (deep) launch_gemm_bk<128, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part, splits);
+    else launch_gemm_bk<64, P_FP8, A_K, B_K, EPI>(grid, g, s, st, s.split_part, splits);
   } else {
-    if (deep) launch_gemm_bk<128, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
-    else launch_gemm_bk<32, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part);
+    if (deep) launch_gemm_bk<128, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part, splits);
+    else launch_gemm_bk<32, P_BF16, A_K, B_K, EPI>(grid, g, s, st, s.split_part, splits);
+  }
+}
+
+// Checks tune[] against the shapes: each nonzero entry is 16 k + s with k a k-tile of
+// the precision (bf16 32 / 128, e4m3 64 / 128, float32 32), s in {1, 2, 4} (s > 1 for
+// the forward GEMMs G1-G3 only) and the GEMM's K a multiple of k s.
+int check_tune(int D_in, int H1, int H2, int n, int precision, const int32_t* tune) {
+  if (!tune) return SGMC_OK;
+  const int K[kSlots] = {D_in, H1, H2, n, n};
+  for (int i = 0; i < kSlots; ++i) {
+    const int tv = tune[i];
+    if (tv == 0) continue;
+    const int bk = tv >> 4, sp = tv & 15;
+    const bool bk_ok = precision == 2 ? bk == 32 : precision == 1 ? (bk == 64 || bk == 128)
+                                                                : (bk == 32 || bk == 128);
+    SGMC_REQUIRE(bk_ok && (sp == 1 || sp == 2 || sp == 4) && (sp == 1 || i < SLOT_G4) &&
+                     K[i] % (bk * sp) == 0,
+                 SGMC_E_ARG, "sgmc_mlp: tune[%d] = %d (k-tile %d, split %d) does not fit K = %d",
+                 i, tv, bk, sp, K[i]);
+  }
+  return SGMC_OK;
+}
+
+// The k-tile depth and split-K factor of each GEMM (16 k + s): tune[i] where set, else
+// the built-in choice --
+// - k-tiles of 128 when K allows (bf16, e4m3), so the register prefetch of the next tile
+//   has a whole tile of matrix work to land behind; 32 (bf16) or 64 (e4m3) otherwise;
+//   float32 always 32 (the transposed tiles of 4-byte elements fit one 4x4 block per
+//   thread). The bf16 parameter-update GEMMs (G4, G5 with update) take 32: their short
+//   K (the minibatch) gains little from deep tiles, and the smaller staging registers
+//   give three waves per SIMD instead of two to the update epilogue (+3.4% on a
+//   784-1024-1024 MLP). The k order of the matrix instructions is the same either way,
+//   so the results are identical.
+// - forward GEMMs G1-G3 split K in two while the grid holds at most about four blocks
+//   per CU: they have few output tiles but long K. The parameter GEMMs never split.
+void plan_gemms(int D_in, int H1, int H2, int n, int C, int precision, int update,
+                const int32_t* tune, int* plan) {
+  const int K[kSlots] = {D_in, H1, H2, n, n};
+  const int64_t tiles[kSlots] = {(int64_t)(n / BM) * (H1 / BN), (int64_t)(n / BM) * (H2 / BN),
+                                 (int64_t)(n / BM) * (H1 / BN), (int64_t)(H1 / BM) * (H2 / BN),
+                                 (int64_t)(D_in / BM) * (H1 / BN)};
+  for (int i = 0; i < kSlots; ++i) {
+    if (tune && tune[i]) {
+      plan[i] = tune[i];
+      continue;
+    }
+    int bk;
+    if (precision == 2) bk = 32;
+    else if (precision == 1) bk = K[i] % 128 == 0 ? 128 : 64;
+    else bk = K[i] % 128 == 0 && !(update && i >= SLOT_G4) ? 128 : 32;
+    int sp = 1;
+    if (i < SLOT_G4) {
+      const int64_t blocks = tiles[i] * C;
+      const int ktiles = K[i] / bk;
+      while (sp < kMaxSplits && blocks * sp * 2 <= 1024 && ktiles % (sp * 2) == 0) sp *= 2;
+    }
+    plan[i] = bk * 16 + sp;
   }
 }
 
@@ -1508,6 +1560,9 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   s.row0 = 0;
   s.fp8 = q.precision == 1 ? 1 : 0;
   s.fp32 = q.precision == 2 ? 1 : 0;
+  const int trc = check_tune(q.D_in, q.H1, q.H2, q.n, q.precision, q.tune);
+  if (trc != SGMC_OK) return trc;
+  plan_gemms(q.D_in, q.H1, q.H2, q.n, q.C, q.precision, update, q.tune, s.plan);
   s.q = q.q;
   s.shadow8 = q.shadow8;
   if (s.fp8)
@@ -1532,17 +1587,30 @@ void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   Gemm g1{s.Xb, q.D_in, (int64_t)n * q.D_in, W1, q.H1, q.P, n, q.H1, q.D_in,
           s.h1, q.H1, (int64_t)n * q.H1, q.theta + s.off[3], q.P, nullptr, 0, 0, 0, 0};
   launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H1 / BN, C), with_q(g1, Q_X, Q_W1, Q_H1),
-                                          s, st_);
+                                          s, st_, SLOT_G1);
   Gemm g2{s.h1, q.H1, (int64_t)n * q.H1, W2, q.H2, q.P, n, q.H2, q.H1,
           s.h2, q.H2, (int64_t)n * q.H2, q.theta + s.off[4], q.P, nullptr, 0, 0, 0, 0};
   launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H2 / BN, C), with_q(g2, Q_H1, Q_W2, -1),
-                                          s, st_);
+                                          s, st_, SLOT_G2);
   hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
 }
 
 }  // namespace
 
 extern "C" {
+
+int sgmc_mlp_gemm_plan(int D_in, int H1, int H2, int n, int C, int precision, int update,
+                       const int32_t* tune, int32_t* plan) {
+  SGMC_REQUIRE(plan && C >= 1 && n >= BM && n % BM == 0 && D_in % BM == 0 && H1 % BN == 0 &&
+                   H2 % BN == 0 && precision >= 0 && precision <= 2,
+               SGMC_E_ARG, "sgmc_mlp_gemm_plan: bad shape or precision");
+  const int rc = check_tune(D_in, H1, H2, n, precision, tune);
+  if (rc != SGMC_OK) return rc;
+  int p[kSlots];
+  plan_gemms(D_in, H1, H2, n, C, precision, update, tune, p);
+  for (int i = 0; i < kSlots; ++i) plan[i] = p[i];
+  return SGMC_OK;
+}
 
 int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   Layout L;
@@ -1558,7 +1626,7 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   Gemm g3{s.da2, q.H2, (int64_t)n * q.H2, W2, q.H2, q.P, n, q.H1, q.H2,
           s.da1, q.H1, (int64_t)n * q.H1, nullptr, 0, s.h1, q.H1, (int64_t)n * q.H1, 0, 0};
   launch_gemm<true, true, EPI_MASK>(dim3(n / BM, q.H1 / BN, C), with_q(g3, Q_DA2, Q_W2, Q_DA1), s,
-                                    st_);
+                                    st_, SLOT_G3);
   // gW2 = H1^T dA2: A[m=a][k=i] = H1[i][a] (m-contiguous), B[k=i][n=b] = dA2[i][b].
   Gemm g4{s.h1, q.H1, (int64_t)n * q.H1, s.da2, q.H2, (int64_t)n * q.H2, q.H1, q.H2, n,
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 1, 0};
@@ -1568,11 +1636,11 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
   g4 = with_q(g4, Q_H1, Q_DA2, -1);
   g5 = with_q(g5, Q_X, Q_DA1, -1);
   if (update) {
-    launch_gemm<false, false, EPI_UPDATE>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_);
-    launch_gemm<false, false, EPI_UPDATE>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_);
+    launch_gemm<false, false, EPI_UPDATE>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_, SLOT_G4);
+    launch_gemm<false, false, EPI_UPDATE>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_, SLOT_G5);
   } else {
-    launch_gemm<false, false, EPI_GRAD>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_);
-    launch_gemm<false, false, EPI_GRAD>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_);
+    launch_gemm<false, false, EPI_GRAD>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_, SLOT_G4);
+    launch_gemm<false, false, EPI_GRAD>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_, SLOT_G5);
   }
   hipLaunchKernelGGL(colsum_kernel,
                      dim3((q.H2 + GT - 1) / GT + (q.H1 + GT - 1) / GT + 1, n / kStrip, C),

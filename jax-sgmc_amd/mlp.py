@@ -74,12 +74,15 @@ class FusedMLP:
   as solver.py:478-487 does). precision "fp32" runs the GEMMs on float32 operands
   (the fp32 matrix instruction, activations in float32, weights read from theta);
   "bf16" on bf16 operands and a bf16 shadow of the parameters; "fp8" on e4m3 operands
-  with per-tensor delayed scaling. State, noise and updates are float32 in all three."""
+  with per-tensor delayed scaling. State, noise and updates are float32 in all three.
+
+  ``tune`` (five ints, see sgmc_mlp_state.tune) overrides the k-tile depth and split-K
+  factor of each GEMM; autotune() chooses them by timing the live problem."""
 
   def __init__(self, X, labels, dims, n, theta0, eps_table, keys=None, dkeys=None, rms=False,
                alpha=0.9, lmbd=1e-5, prior_scale=1.0, has_prior=True, temperature=1.0,
                noise_temperature=1.0, save_slots=None, device="cuda", integrator="sgld",
-               friction=1.0, steps=1, precision="bf16"):
+               friction=1.0, steps=1, precision="bf16", tune=None):
     _lib.require_gpu()
     _lib.load()
     dev = torch.device(device)
@@ -101,6 +104,8 @@ class FusedMLP:
     if precision not in PRECISIONS:
       raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
     self.precision = precision
+    self.tune = [0] * 5 if tune is None else [int(t) for t in tune]
+    self.gemm_plan()  # validates tune against the shapes
     self.shadow = (None if precision == "fp32" else
                    torch.empty(self.theta.shape, dtype=torch.int16, device=dev))
     self.shadow8 = self.q = None
@@ -227,7 +232,84 @@ class FusedMLP:
     st.mode = 0
     st.precision = PRECISIONS[self.precision]
     st.shadow8, st.q = opt(self.shadow8), opt(self.q)
+    st.tune[:] = self.tune
     return st
+
+  def gemm_plan(self, update=True):
+    """(k-tile depth, split-K factor) of the GEMMs G1 = Xb W1, G2 = h1 W2, G3 = dA2 W2^T,
+    G4 = h1^T dA2, G5 = Xb^T dA1 as an iteration (update) or a gradient evaluation
+    launches them."""
+    D_in, H1, H2 = self.dims
+    out = (ctypes.c_int32 * 5)()
+    _lib.call("sgmc_mlp_gemm_plan", D_in, H1, H2, self.n, self.C, PRECISIONS[self.precision],
+              1 if update else 0, (ctypes.c_int32 * 5)(*self.tune), out)
+    return [(v >> 4, v & 15) for v in out]
+
+  def tile_candidates(self, slot):
+    """Codes 16 k + s that sgmc_mlp_step accepts for GEMM `slot` at these shapes."""
+    D_in, H1, H2 = self.dims
+    K = (D_in, H1, H2, self.n, self.n)[slot]
+    depths = {"fp32": (32,), "fp8": (64, 128), "bf16": (32, 128)}[self.precision]
+    splits = (1, 2, 4) if slot < 3 else (1,)
+    return [k * 16 + s for k in depths for s in splits if K % (k * s) == 0]
+
+  def _mutable(self):
+    names = ("theta", "v", "shadow", "shadow8", "q", "keys", "dkeys", "U", "iter", "counters",
+             "done", "grad", "samples", "samples_U", "mom", "ke", "U_half", "U_full", "U_prop",
+             "theta_old", "mom_old", "U_old", "mh_key", "mh_stats", "accepted", "work")
+    return [t for t in (getattr(self, n) for n in names) if t is not None]
+
+  def autotune(self, reps=20, min_gain=0.01):
+    """Chooses the k-tile depth and split-K factor of each GEMM by timing the live
+    problem (HIP events on the launch stream around `reps` back-to-back units after one
+    warm-up unit): whole iterations for SGLD, one OBABO step (two evaluations, the unit
+    OBABO and SGGMC iterations repeat) otherwise. One GEMM at a time, starting from the
+    built-in choice; a candidate replaces the current one only if it is faster by more
+    than `min_gain`. Each timing starts from a snapshot of the engine's state (the
+    iteration counter runs at most reps + 1 past it; the schedule tables clamp it),
+    which is restored at the end, so the chains continue where they were. Sets
+    self.tune (and self.tune_ms: the time per unit of every plan tried, self.tune_gain:
+    (built-in ms, tuned ms)) and returns gemm_plan()."""
+    unit = self.iteration if self.integrator == "sgld" else self.obabo_step
+    tensors = self._mutable()
+    snap = [t.clone() for t in tensors]
+
+    def restore():
+      for t, s0 in zip(tensors, snap):
+        t.copy_(s0)
+
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed():
+      restore()
+      unit()  # warm-up
+      e0.record()
+      for _ in range(reps):
+        unit()
+      e1.record()
+      e1.synchronize()
+      return e0.elapsed_time(e1) / reps
+
+    self.tune = [k * 16 + s for k, s in self.gemm_plan()]
+    self.tune_ms = {}
+    timed()  # the first timing after other work runs slow by a few percent
+    best = builtin = timed()
+    for slot in range(5):
+      best_code = self.tune[slot]
+      self.tune_ms[(slot, best_code)] = best
+      for code in self.tile_candidates(slot):
+        if code == best_code:
+          continue
+        self.tune[slot] = code
+        t = timed()
+        self.tune_ms[(slot, code)] = t
+        if t < best * (1.0 - min_gain):
+          best, best_code = t, code
+      self.tune[slot] = best_code
+    self.tune_gain = (builtin, best)
+    restore()
+    torch.cuda.synchronize()
+    return self.gemm_plan()
 
   def iteration(self):
     if self.integrator == "sgld":

@@ -48,7 +48,7 @@ def test_binding_declares_every_header_function():
   missing = [n for n in header_functions() if n not in names]
   assert not missing, f"no ctypes signature: {missing}"
   lib = L.load()
-  assert lib.sgmc_abi_version() >= 1
+  assert lib.sgmc_abi_version() == L.ABI_VERSION
 
 
 STRUCTS = {
