@@ -144,7 +144,7 @@ def run_fused(torch, fused, R, X, y, K, W, graph_iters=10, target_bytes=0):
     # a fixed grid (counter collection serializes launches, which misleads the tuner)
     tuned = grp.autotune(candidates=[target_bytes], reps=1)
   else:
-    tuned = grp.autotune(reps=2)
+    tuned = grp.autotune()
   log(f"autotune: {tuned}")
   grp.capture(graph_iters)
   # warmup (graph replays) then the timed region, bracketed by synchronizations; the
@@ -321,6 +321,49 @@ def ladder_problems(torch, R):
   return out
 
 
+def ladder_tables(torch, fused, R, probs, eps, G):
+  """The 32 ladders as one chain table per D class on its own stream (fused.TableGroup),
+  each table with its own replica exchange. Per class the kernel is compiled for that
+  row width only (fewer registers, more waves per SIMD than the mixed-class kernel);
+  the D = 16 and 64 tables keep one block per chain and run their iterations in loop
+  launches cut at the exchange's test iterations, concurrent with the D = 256 and 1024
+  tables, which take one launch per iteration. Keys are those of the chains' positions
+  in a single table ordered by problem, so every chain and exchange gives what it gives
+  there."""
+  C = LADDER_PROBLEMS * LADDER_RUNGS
+  temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
+  keys = R.key_words(R.split(R.PRNGKey(0), C))
+  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
+  skeys = R.key_words(R.split(R.PRNGKey(4), LADDER_PROBLEMS * (LADDER_RUNGS - 1)))
+  engines, exchanges, dims = [], [], []
+  for D in LADDER_DIMS:
+    chains, pairs, ptemps, parity, sk = [], [], [], [], []
+    for k, (X, y) in enumerate(probs):
+      if X.shape[1] != D:
+        continue
+      for j in range(LADDER_RUNGS):
+        c = k * LADDER_RUNGS + j
+        chains.append(dict(X=X, y=y, D=D, n=LADDER_BATCH, theta0=np.zeros(D, np.float32),
+                           key=tuple(keys[c]), dkey=tuple(dkeys[c]), temperature=temps[j],
+                           save=False))
+        if j + 1 < LADDER_RUNGS:
+          pairs.append((len(chains) - 1, len(chains)))
+          ptemps.append((temps[j], temps[j + 1]))
+          parity.append(j % 2)
+          sk.append(tuple(skeys[k * (LADDER_RUNGS - 1) + j]))
+    small = D <= 64
+    eng = fused.FusedGLM(chains, "logistic", "sgld", eps, prior_scale=1.0,
+                         target_bytes=(1 << 30) if small else None)
+    exchanges.append(fused.ReplicaExchange(eng, pairs, ptemps, keys=sk, swap_every=G,
+                                           parity=parity))
+    engines.append(eng)
+    dims.append(D)
+  # three streams (the null stream takes the fourth hardware queue): the D = 16 and 64
+  # loop tables share one, one after the other, well inside an iteration block of the
+  # others
+  return fused.TableGroup(engines, stream_of=[0, 0, 1, 2]), exchanges, dims
+
+
 def run_ladder(args):
   import torch
   import __graft_entry__
@@ -330,41 +373,31 @@ def run_ladder(args):
   log("ladder data")
   probs = ladder_problems(torch, R)
   C = LADDER_PROBLEMS * LADDER_RUNGS
-  temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
-  keys = R.key_words(R.split(R.PRNGKey(0), C))
-  dkeys = R.key_words(R.split(R.PRNGKey(2), C))
-  npairs = LADDER_PROBLEMS * (LADDER_RUNGS - 1)
-  skeys = R.key_words(R.split(R.PRNGKey(4), npairs))
-  chains, pairs, ptemps, parity = [], [], [], []
-  for k, (X, y) in enumerate(probs):
-    for j in range(LADDER_RUNGS):
-      c = k * LADDER_RUNGS + j
-      chains.append(dict(X=X, y=y, D=X.shape[1], n=LADDER_BATCH,
-                         theta0=np.zeros(X.shape[1], np.float32), key=tuple(keys[c]),
-                         dkey=tuple(dkeys[c]), temperature=temps[j], save=False))
-      if j + 1 < LADDER_RUNGS:
-        pairs.append((c, c + 1))
-        ptemps.append((temps[j], temps[j + 1]))
-        parity.append(j % 2)
   K, W = args.steps, args.warmup
   G = LADDER_SWAP_EVERY
-  total = K + W + 2 * G
+  warm_cap = 5000  # time-bounded warm-up (below): at most this many extra iterations
+  total = K + W + warm_cap + 2 * G
   eps = scheduler.polynomial_table(1e-3, 1e-4, total)
-  eng = fused.FusedGLM(chains, "logistic", "sgld", eps, prior_scale=1.0)
-  tuned = eng.autotune(reps=3)
+  grp, exchanges, dims = ladder_tables(torch, fused, R, probs, eps, G)
+  big = [i for i, e in enumerate(grp.engines) if not e.loop_capable()]
+  tuned = grp.autotune(tables=big)
   log(f"autotune: {tuned}")
-  ex = fused.ReplicaExchange(eng, pairs, ptemps, keys=[tuple(k) for k in skeys], swap_every=G,
-                             parity=parity)
-  g = eng.capture(G)
-  for _ in range(max(1, W // G)):
-    g.replay()
+  assert all(e.loop_capable() == (d <= 64) for e, d in zip(grp.engines, dims))
+  grp.capture(G)
+  grp.run(max(1, W // G) * G, G)
   torch.cuda.synchronize()
+  # warm-up continued for at least half a second (the device's clocks settle under
+  # load), in whole iteration blocks, within the schedule
+  t0, warm = time.perf_counter(), 0
+  while time.perf_counter() - t0 < 0.5 and warm + G <= warm_cap:
+    grp.run(G, G)
+    torch.cuda.synchronize()
+    warm += G
   reps = max(1, K // G)
   e0 = torch.cuda.Event(enable_timing=True)
   e1 = torch.cuda.Event(enable_timing=True)
   e0.record()
-  for _ in range(reps):
-    g.replay()
+  grp.run(reps * G, G)
   e1.record()
   torch.cuda.synchronize()
   ms = e0.elapsed_time(e1)
@@ -373,24 +406,34 @@ def run_ladder(args):
   bytes_it = sum(LADDER_RUNGS * (LADDER_BATCH * (4 * X.shape[1] + 8) + 8 * X.shape[1])
                  for X, _ in probs)
   achieved = bytes_it / (ms / 1e3 / iters) / 1e9
+  tables = [{"D": d, "chains": e.C, "grid_blocks": e.nblocks,
+             "launch": "loop (50 iterations, then the exchange)" if e.loop_capable() else
+                       "one per iteration, exchange every 50",
+             "kernel": (f"glm_loop_kernel<{e.uniform_dclass}>" if e.loop_capable() else
+                        f"glm_fused_kernel<{e.uniform_dclass}>"),
+             "target_bytes_per_block": e.target_bytes} for e, d in zip(grp.engines, dims)]
   out = {"metric": "chain-steps/sec, reSGLD ladders (fused, 1 MI355X)", "value": value,
          "unit": "chain-steps/s", "n_gpus": 1, "steps": iters, "warmup": W,
          "ms_per_step": ms / iters, "higher_is_better": True, "scaling": "weak",
-         "vs_baseline": None, "dtype": "fp32",
+         "vs_baseline": None, "dtype": "fp32", "warmup_extra_iterations": warm,
          "data": "synthetic (jax-compatible Threefry draws from PRNGKey(3))",
          "config": {"workload": "reSGLD: 32 logistic problems x 8 rungs (T_j = 20^(j/7)), "
                                 "swap every 50 steps", "chains": C, "batch": LADDER_BATCH,
                     "N": [int(X.shape[0]) for X, _ in probs],
                     "D": [int(X.shape[1]) for X, _ in probs],
-                    "step_size": "polynomial 1e-3 -> 1e-4",
-                    "target_bytes_per_block": tuned["target_bytes"], "grid_blocks": eng.nblocks},
+                    "step_size": "polynomial 1e-3 -> 1e-4", "tables": tables},
          "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                      "kernel": "glm_fused_kernel<-1> (mixed D classes) + swap_kernel",
+                      "kernel": "all tables of one iteration (glm_fused_kernel<4>, <2>, "
+                                "glm_loop_kernel<1>, <0>, swap_kernel)",
                       "bytes_per_iteration": bytes_it},
-         "exchanges": int(ex.swaps.sum().item()), "tests": int(ex.step.sum().item())}
+         "exchanges": int(sum(int(ex.swaps.sum().item()) for ex in exchanges)),
+         "tests": int(sum(int(ex.step.sum().item()) for ex in exchanges))}
   if not args.no_cpu:
     log("CPU baseline (oracle C port per chain)")
+    temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
+    keys = R.key_words(R.split(R.PRNGKey(0), C))
+    dkeys = R.key_words(R.split(R.PRNGKey(2), C))
     out["cpu_baseline"] = ladder_cpu_baseline(probs, temps, keys, dkeys, eps)
     out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
   else:

@@ -48,6 +48,15 @@ extern "C" {
 const char* sgmc_last_error(void);
 int sgmc_abi_version(void);
 
+/* A stream of the current device with a hardware queue of its own: created with a CU
+ * mask of every CU (hipExtStreamCreateWithCUMask), since streams created otherwise share
+ * the process's few hardware queues (GPU_MAX_HW_QUEUES, 4 by default), and kernels of
+ * two streams on one queue run one after the other. Default priority. For chain tables
+ * run side by side (fused.TableGroup). No counterpart in the reference (XLA schedules
+ * its own streams). */
+int sgmc_stream_create(void** stream);
+int sgmc_stream_destroy(void* stream);
+
 /* ---------------------------------------------------------------- RNG ----
  * jax.random with the original (non-partitionable) Threefry-2x32 derivation, the
  * generator behind every random number of the reference's hot path:

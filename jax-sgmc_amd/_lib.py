@@ -3,7 +3,9 @@
 Every entry point of include/sgmc.h is declared here with its argument types.
 Loading fails loudly: there is no fallback implementation of any kernel.
 """
+import contextlib
 import ctypes
+import gc
 import os
 
 import torch
@@ -79,6 +81,8 @@ class MlpState(ctypes.Structure):
 _SIGNATURES = {
     "sgmc_last_error": (ctypes.c_char_p, []),
     "sgmc_abi_version": (c_int, []),
+    "sgmc_stream_create": (c_int, [c_void_p]),
+    "sgmc_stream_destroy": (c_int, [c_void_p]),
     "sgmc_split": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "sgmc_random_bits": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "sgmc_uniform": (c_int, [c_void_p, c_void_p, c_int64, c_float, c_float, c_void_p]),
@@ -219,6 +223,39 @@ def ptr(t):
   if not t.is_cuda:
     raise ValueError("libsgmc entry points take device tensors; got a CPU tensor")
   return c_void_p(t.data_ptr())
+
+
+@contextlib.contextmanager
+def capture_guard():
+  """Around a HIP graph capture: collects garbage first and keeps the cyclic collector
+  off until the capture ends. A graph kept alive only by a dead reference cycle would
+  otherwise be destroyed whenever the collector runs, and destroying a graph while a
+  stream is capturing is an error that aborts the process."""
+  gc.collect()
+  enabled = gc.isenabled()
+  gc.disable()
+  try:
+    yield
+  finally:
+    if enabled:
+      gc.enable()
+
+
+_DEDICATED = {}
+
+
+def dedicated_stream(device, index):
+  """Stream ``index`` of a per-device pool of streams with hardware queues of their own
+  (sgmc_stream_create), created on first use and kept until the process ends; wrapped
+  as a torch stream."""
+  dev = torch.device(device)
+  key = (dev.index if dev.index is not None else torch.cuda.current_device(), int(index))
+  if key not in _DEDICATED:
+    with torch.cuda.device(key[0]):
+      h = c_void_p()
+      call("sgmc_stream_create", ctypes.byref(h))
+    _DEDICATED[key] = torch.cuda.ExternalStream(h.value, device=torch.device("cuda", key[0]))
+  return _DEDICATED[key]
 
 
 def stream():

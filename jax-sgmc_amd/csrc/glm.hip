@@ -126,7 +126,14 @@ struct RowGroup {
   bool ok[U];
 };
 
+// Buffer resource word 3 for raw (unformatted) buffer loads on CDNA.
+constexpr int kRawBufferFlags = 0x00020000;
+
 // Issues the loads of the U row groups starting at group-local offset gl of a chunk.
+// Rows of a whole wavefront (LPR = 64) are read through a buffer resource whose base is
+// the row and whose size is the row's 4 D4 floats: the base lives in scalar registers,
+// the chunks are one lane offset plus immediate offsets, and chunks past the row (or
+// every chunk of a row outside the data, size 0) read as zero without a clamp.
 template <int LPR, int CPL>
 __device__ __forceinline__ void load_group(gcfloat* __restrict__ X, int64_t ldx,
                                            gcfloat* __restrict__ y, int D4, int gl, int rid,
@@ -145,12 +152,21 @@ __device__ __forceinline__ void load_group(gcfloat* __restrict__ X, int64_t ldx,
     g.ok[u] = r >= 0;
     const int64_t rc = r >= 0 ? r : 0;
     g.y[u] = y[rc];
-    SGMC_GLOBAL const f4* rp = reinterpret_cast<SGMC_GLOBAL const f4*>(X + rc * ldx);
-    // every lane loads (clamped to a valid chunk); invalid chunks are zeroed at use
+    if constexpr (RPW == 1) {
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(const float*)(X + rc * ldx), (short)0, r >= 0 ? 16 * D4 : 0, kRawBufferFlags);
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const int c4 = q + LPR * k;
-      g.x[u][k] = rp[c4 < D4 ? c4 : D4 - 1];
+      for (int k = 0; k < CPL; ++k)
+        g.x[u][k] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(
+                                               rs, 16 * (q + LPR * k), 0, 0));
+    } else {
+      SGMC_GLOBAL const f4* rp = reinterpret_cast<SGMC_GLOBAL const f4*>(X + rc * ldx);
+      // every lane loads (clamped to a valid chunk); invalid chunks are zeroed at use
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const int c4 = q + LPR * k;
+        g.x[u][k] = rp[c4 < D4 ? c4 : D4 - 1];
+      }
     }
 
   }
@@ -185,10 +201,11 @@ __device__ __forceinline__ void process_group(RowGroup<LPR, CPL>& g, int row0, i
   const int seg = lane / LPR;
   const int q = lane % LPR;
   // chunks past D (clamped loads) and rows outside the data contribute zero; the
-  // masking is skipped when the whole wavefront's group is valid
-  bool bad = D4 < LPR * CPL;
+  // masking is skipped when the whole wavefront's group is valid, and always for rows
+  // read through a buffer resource (those chunks read as zero)
+  bool bad = RPW > 1 && D4 < LPR * CPL;
 #pragma unroll
-  for (int u = 0; u < U; ++u) bad = bad || !g.ok[u];
+  for (int u = 0; u < U; ++u) bad = bad || (RPW > 1 && !g.ok[u]);
   if (__any(bad)) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -559,10 +576,13 @@ __device__ __forceinline__ float leaf_normal(Key leafkey, uint32_t S, uint32_t i
   return bits_to_normal(bits_word(leafkey, S, i));
 }
 
-template <int LPR, int CPL>
+// MODE: a SGMC_MODE_* fixed at compile time (its branches folded away, fewer registers),
+// or -1 for the mode in a.mode.
+template <int LPR, int CPL, int MODE>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_chain& ch,
                                            int bchain, float* lds_cols, float* lds_small,
                                            float* lds_lik, int* flag) {
+  const int mode = MODE >= 0 ? MODE : a.mode;
   const int it = max(0, min(a.iter[0] + a.iter_offset, a.table_len - 1));
   const float eps = a.eps_table[it];
   const int D = ch.D;
@@ -576,10 +596,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   gcfloat* g_inv_mass = to_global(a.inv_mass);
   gcfloat* g_sqrt_mass = to_global(a.sqrt_mass);
   const Key key0 = load_key(ch.key);
-  const bool hmc = a.mode == SGMC_MODE_SGHMC;
-  const bool amg = a.mode == SGMC_MODE_AMAGOLD;
-  const bool ob_mh = a.mode == SGMC_MODE_SGGMC;
-  const bool ob = a.mode == SGMC_MODE_OBABO || ob_mh;
+  const bool hmc = mode == SGMC_MODE_SGHMC;
+  const bool amg = mode == SGMC_MODE_AMAGOLD;
+  const bool ob_mh = mode == SGMC_MODE_SGGMC;
+  const bool ob = mode == SGMC_MODE_OBABO || ob_mh;
   const bool has_mom = hmc || amg || ob;
   const bool shift = hmc || amg;  // evaluation point theta + scale M^-1 p
   const bool resample = hmc && a.step == 0;
@@ -721,7 +741,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     ob_noise = sqrtf((1.0f - perm) * Tn);
     ob_decay = sqrtf(perm);
   }
-  const bool rms = a.mode == SGMC_MODE_SGLD_RMS;
+  const bool rms = mode == SGMC_MODE_SGLD_RMS;
   // reversible_leapfrog momentum coefficients (integrator.py:420-437), float32.
   const float ef = eps * a.friction;
   const float am_decay = 1.0f - ef;
@@ -885,7 +905,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
 // reading its parameters and keys back from memory after a barrier, so no launch and
 // no arrival counter separates two evaluations. The last chain to finish advances the
 // iteration counter by the number of iterations.
-template <int DCLASS>
+template <int DCLASS, int MODE>
 __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iterations) {
   __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
@@ -897,17 +917,18 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   FusedArgs b = a;
   b.advance = -1;
-  const int launches = a.mode == SGMC_MODE_OBABO ? 2 * a.steps : a.steps;
+  const int mode = MODE >= 0 ? MODE : a.mode;
+  const int launches = mode == SGMC_MODE_OBABO ? 2 * a.steps : a.steps;
   for (int t = 0; t < iterations; ++t) {
     b.iter_offset = t;
     for (int st = 0; st < launches; ++st) {
       b.step = st;
       switch (dc) {
-        case 0: fused_body<4, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
-        case 1: fused_body<16, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
-        case 2: fused_body<64, 1>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
-        case 3: fused_body<64, 2>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
-        default: fused_body<64, 4>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 0: fused_body<4, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 1: fused_body<16, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 2: fused_body<64, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 3: fused_body<64, 2, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        default: fused_body<64, 4, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
       }
       __syncthreads();
     }
@@ -921,7 +942,7 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   }
 }
 
-template <int DCLASS>
+template <int DCLASS, int MODE>
 __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
@@ -939,11 +960,41 @@ __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   const int bchain = blockIdx.x - ch.block_begin;
   const int dc = DCLASS >= 0 ? DCLASS : ch.dclass;
   switch (dc) {
-    case 0: fused_body<4, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
-    case 1: fused_body<16, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
-    case 2: fused_body<64, 1>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
-    case 3: fused_body<64, 2>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
-    default: fused_body<64, 4>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 0: fused_body<4, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 1: fused_body<16, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 2: fused_body<64, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 3: fused_body<64, 2, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    default: fused_body<64, 4, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+  }
+}
+
+// The fused kernels of a table: one D class (or -1, mixed, with the class switch per
+// block) and the SGLD mode compiled in, or any mode read at run time (MODE = -1).
+template <int MODE>
+void launch_fused(int dclass, int nblocks, hipStream_t s, const FusedArgs& a) {
+  const dim3 grid(nblocks), block(kNT);
+  switch (dclass) {
+    case 0: hipLaunchKernelGGL((glm_fused_kernel<0, MODE>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((glm_fused_kernel<1, MODE>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((glm_fused_kernel<2, MODE>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((glm_fused_kernel<3, MODE>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((glm_fused_kernel<4, MODE>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((glm_fused_kernel<-1, MODE>), grid, block, 0, s, a); break;
+  }
+}
+
+template <int MODE>
+void launch_loop(int dclass, int nchains, int iterations, hipStream_t s, const FusedArgs& a) {
+  const dim3 grid(nchains), block(kNT);
+  switch (dclass) {
+    case 0: hipLaunchKernelGGL((glm_loop_kernel<0, MODE>), grid, block, 0, s, a, iterations); break;
+    case 1: hipLaunchKernelGGL((glm_loop_kernel<1, MODE>), grid, block, 0, s, a, iterations); break;
+    case 2: hipLaunchKernelGGL((glm_loop_kernel<2, MODE>), grid, block, 0, s, a, iterations); break;
+    case 3: hipLaunchKernelGGL((glm_loop_kernel<3, MODE>), grid, block, 0, s, a, iterations); break;
+    case 4: hipLaunchKernelGGL((glm_loop_kernel<4, MODE>), grid, block, 0, s, a, iterations); break;
+    default:
+      hipLaunchKernelGGL((glm_loop_kernel<-1, MODE>), grid, block, 0, s, a, iterations);
+      break;
   }
 }
 
@@ -1275,15 +1326,10 @@ int sgmc_glm_fused_step(const sgmc_glm_chain* chains, int nchains, const int32_t
   a.advance = advance_iter ? 1 : 0;
   a.iter_offset = 0;
   hipStream_t s = as_stream(stream);
-  dim3 grid(nblocks), block(kNT);
-  switch (uniform_dclass) {
-    case 0: hipLaunchKernelGGL((glm_fused_kernel<0>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((glm_fused_kernel<1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((glm_fused_kernel<2>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((glm_fused_kernel<3>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((glm_fused_kernel<4>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((glm_fused_kernel<-1>), grid, block, 0, s, a); break;
-  }
+  if (mode == SGMC_MODE_SGLD)
+    launch_fused<SGMC_MODE_SGLD>(uniform_dclass, nblocks, s, a);
+  else
+    launch_fused<-1>(uniform_dclass, nblocks, s, a);
   return check_launch("sgmc_glm_fused_step");
 }
 
@@ -1331,15 +1377,10 @@ int sgmc_glm_fused_loop(const sgmc_glm_chain* chains, int nchains, int uniform_d
   a.advance = -1;
   a.iter_offset = 0;
   hipStream_t s = as_stream(stream);
-  dim3 grid(nchains), block(kNT);
-  switch (uniform_dclass) {
-    case 0: hipLaunchKernelGGL((glm_loop_kernel<0>), grid, block, 0, s, a, iterations); break;
-    case 1: hipLaunchKernelGGL((glm_loop_kernel<1>), grid, block, 0, s, a, iterations); break;
-    case 2: hipLaunchKernelGGL((glm_loop_kernel<2>), grid, block, 0, s, a, iterations); break;
-    case 3: hipLaunchKernelGGL((glm_loop_kernel<3>), grid, block, 0, s, a, iterations); break;
-    case 4: hipLaunchKernelGGL((glm_loop_kernel<4>), grid, block, 0, s, a, iterations); break;
-    default: hipLaunchKernelGGL((glm_loop_kernel<-1>), grid, block, 0, s, a, iterations); break;
-  }
+  if (mode == SGMC_MODE_SGLD)
+    launch_loop<SGMC_MODE_SGLD>(uniform_dclass, nchains, iterations, s, a);
+  else
+    launch_loop<-1>(uniform_dclass, nchains, iterations, s, a);
   return check_launch("sgmc_glm_fused_loop");
 }
 

@@ -100,6 +100,35 @@ const char* sgmc_last_error(void) { return g_err; }
 
 int sgmc_abi_version(void) { return SGMC_ABI_VERSION; }
 
+int sgmc_stream_create(void** stream) {
+  SGMC_REQUIRE(stream, SGMC_E_ARG, "sgmc_stream_create: null pointer");
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  SGMC_REQUIRE(e == hipSuccess && cus > 0 && cus <= 1024, SGMC_E_HIP,
+               "sgmc_stream_create: device query failed (%s)", hipGetErrorString(e));
+  // every CU: the mask gives the stream a hardware queue of its own without
+  // restricting where its kernels run
+  uint32_t mask[32];
+  const int words = (cus + 31) / 32;
+  for (int i = 0; i < words; ++i) {
+    const int bits = cus - 32 * i;
+    mask[i] = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1u);
+  }
+  hipStream_t s = nullptr;
+  e = hipExtStreamCreateWithCUMask(&s, (uint32_t)words, mask);
+  SGMC_REQUIRE(e == hipSuccess, SGMC_E_HIP, "sgmc_stream_create: %s", hipGetErrorString(e));
+  *stream = s;
+  return SGMC_OK;
+}
+
+int sgmc_stream_destroy(void* stream) {
+  SGMC_REQUIRE(stream, SGMC_E_ARG, "sgmc_stream_destroy: null stream");
+  const hipError_t e = hipStreamDestroy(static_cast<hipStream_t>(stream));
+  SGMC_REQUIRE(e == hipSuccess, SGMC_E_HIP, "sgmc_stream_destroy: %s", hipGetErrorString(e));
+  return SGMC_OK;
+}
+
 int sgmc_split(const uint32_t* key, uint32_t* out, int num, void* stream) {
   SGMC_REQUIRE(key && out, SGMC_E_ARG, "sgmc_split: null pointer");
   SGMC_REQUIRE(num >= 1, SGMC_E_ARG, "sgmc_split: num must be >= 1, got %d", num);

@@ -19,6 +19,7 @@ SggmcMH attach a swap or Metropolis-Hastings step to a table.
 import ctypes
 import math
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -125,6 +126,15 @@ class FusedGLM:
     self.keep_lik = bool(keep_lik)
     self.exchange = None
     self.mh = None
+    # iterations launched through this object: the device counter's value as the
+    # launches leave it, known on the host without a synchronization (a sparse
+    # exchange and the loop launches are planned from it)
+    self.host_iter = 0
+    self._graph = None
+    self._capture_stream = None
+    # run / capture / replay default for loop launches: None = whenever the table
+    # allows them
+    self.prefer_loop = None
     self._allocate()
     self._build_table()
 
@@ -245,11 +255,13 @@ class FusedGLM:
       self._launch(s)
 
   def loop_capable(self):
-    """Chains of one block each and no test between iterations: many iterations can
-    run in one launch (``sgmc_glm_fused_loop``)."""
-    return (bool(np.all(self.nblocks_chain == 1)) and self.exchange is None and
-            self.mh is None and self.mode in (_lib.MODE_SGLD, _lib.MODE_SGLD_RMS,
-                                              _lib.MODE_SGHMC, _lib.MODE_OBABO))
+    """Chains of one block each, no Metropolis-Hastings test, and no exchange or one
+    that runs only at its test iterations: many iterations can run in one launch
+    (``sgmc_glm_fused_loop``), cut at the exchange's test iterations."""
+    return (bool(np.all(self.nblocks_chain == 1)) and self.mh is None and
+            (self.exchange is None or self.exchange.sparse) and
+            self.mode in (_lib.MODE_SGLD, _lib.MODE_SGLD_RMS, _lib.MODE_SGHMC,
+                          _lib.MODE_OBABO))
 
   def _launch_loop(self, iterations):
     _lib.call("sgmc_glm_fused_loop", _lib.ptr(self.table), self.C, self.uniform_dclass,
@@ -259,59 +271,106 @@ class FusedGLM:
               self.table_len, _lib.ptr(self.save_slot), _lib.ptr(self.iter),
               _lib.ptr(self.done), int(iterations), _lib.stream())
 
+  def _plan(self, iterations, loop):
+    """Launch sequence of ``iterations`` iterations starting after host_iter: ("it",)
+    one iteration's evaluations, ("loop", k) k iterations in one loop launch, ("swap",)
+    the exchange, ("mh",) the Metropolis-Hastings test."""
+    ops, done, left = [], self.host_iter, int(iterations)
+    ex = self.exchange
+    while left > 0:
+      if loop:
+        k = min(1000, left)
+        t = ex.iterations_to_next_launch(done) if ex is not None else None
+        if t is not None:
+          k = min(k, t)
+        ops.append(("loop", k))
+      else:
+        k = 1
+        ops.append(("it",))
+      done += k
+      left -= k
+      if ex is not None and ex.needs_launch(done):
+        ops.append(("swap",))
+      if self.mh is not None:
+        ops.append(("mh",))
+    return ops
+
+  def _execute(self, ops):
+    for op in ops:
+      if op[0] == "it":
+        self._launch_steps()
+        self.host_iter += 1
+      elif op[0] == "loop":
+        self._launch_loop(op[1])
+        self.host_iter += op[1]
+      elif op[0] == "swap":
+        self.exchange.launch()
+      else:
+        self.mh.launch()
+
+  def _loop_arg(self, loop):
+    if loop is None:
+      loop = self.prefer_loop
+    if loop is None:
+      return self.loop_capable()
+    if loop and not self.loop_capable():
+      raise ValueError("loop launches need single-block chains, no MH test, and no "
+                       "exchange or one without saving pairs")
+    return bool(loop)
+
   def iteration(self):
     """Advances every chain by one reference iteration (steps launches, two per step in
     the OBABO modes), followed by the replica-exchange step or the Metropolis-Hastings
     test when one is attached."""
-    self._launch_steps()
-    if self.exchange is not None:
-      self.exchange.launch()
-    if self.mh is not None:
-      self.mh.launch()
+    self._execute(self._plan(1, False))
 
-  def capture(self, iterations_per_graph=1):
-    """Captures ``iterations_per_graph`` iterations into a HIP graph."""
+  def capture(self, iterations_per_graph=1, loop=None):
+    """Captures ``iterations_per_graph`` iterations into a HIP graph (loop launches
+    when the table allows them, unless ``loop=False``). The launch sequence depends on
+    where the iterations start relative to a sparse exchange's test iterations: replay
+    the returned graph (or run with graph_iterations), which checks that it still
+    fits, and captures again when it does not."""
+    loop = self._loop_arg(loop)
+    plan = self._plan(iterations_per_graph, loop)
     g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
+    # one capture stream per table, kept: streams created on every capture would move
+    # the hardware queues that later streams get
+    if self._capture_stream is None:
+      self._capture_stream = torch.cuda.Stream(device=self.device)
+    s = self._capture_stream
     s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
+    start = self.host_iter
+    with _lib.capture_guard(), torch.cuda.stream(s):
       with torch.cuda.graph(g, stream=s):
-        for _ in range(iterations_per_graph):
-          self.iteration()
+        self._execute(plan)
     torch.cuda.current_stream().wait_stream(s)
-    self._graph = g
-    self._graph_iters = iterations_per_graph
-    return g
+    self.host_iter = start  # capturing records the launches without running them
+    self._graph = EngineGraph(self, g, int(iterations_per_graph), loop, plan)
+    return self._graph
+
+  def replay(self, iterations_per_graph, loop=None):
+    """Replays the captured graph of ``iterations_per_graph`` iterations, capturing it
+    first when there is none or when its launch sequence does not fit this start."""
+    loop = self._loop_arg(loop)
+    g = self._graph
+    if (g is None or g.iterations != iterations_per_graph or g.loop != loop or
+        g.plan != self._plan(iterations_per_graph, loop)):
+      g = self.capture(iterations_per_graph, loop)
+    g.replay()
 
   def run(self, iterations, graph_iterations=0, loop=None):
     """Runs ``iterations`` iterations (eagerly, or by replaying a captured graph of
     ``graph_iterations`` iterations; the remainder runs eagerly). A table of
-    single-block chains with nothing between iterations runs them in loop launches of
-    up to 1000 iterations instead (``loop=False`` disables that)."""
-    if loop is None:
-      loop = self.loop_capable()
-    if loop:
-      if not self.loop_capable():
-        raise ValueError("loop launches need single-block chains and no exchange or MH")
-      done = 0
-      while done < iterations:
-        k = min(1000, iterations - done)
-        self._launch_loop(k)
-        done += k
-      return
+    single-block chains with no test between iterations (or only a sparse exchange)
+    runs them in loop launches of up to 1000 iterations (``loop=False`` disables
+    that)."""
+    loop = self._loop_arg(loop)
+    rest = int(iterations)
     if graph_iterations and iterations >= graph_iterations:
-      g = getattr(self, "_graph", None)
-      if g is None or self._graph_iters != graph_iterations:
-        # capturing records the launches without running them
-        g = self.capture(graph_iterations)
-      reps, rest = divmod(iterations, graph_iterations)
+      reps, rest = divmod(int(iterations), int(graph_iterations))
       for _ in range(reps):
-        g.replay()
-      for _ in range(rest):
-        self.iteration()
-    else:
-      for _ in range(iterations):
-        self.iteration()
+        self.replay(graph_iterations, loop)
+    self._execute(self._plan(rest, loop))
 
   # ------------------------------------------------------------- results -----
   def theta(self, i=None):
@@ -365,6 +424,7 @@ class FusedGLM:
     self.target_bytes = best[1]
     self._build_table()
     self._restore(snap)
+    self._graph = None
     self.tuned = {"target_bytes": best[1], "seconds_per_iteration": best[0],
                   "candidates": candidates}
     return self.tuned
@@ -384,6 +444,30 @@ class FusedGLM:
     torch.cuda.synchronize()
 
 
+class EngineGraph:
+  """A HIP graph of a FusedGLM's iterations with the launch sequence it recorded.
+  replay() runs it if that sequence still fits the engine's position (a sparse
+  exchange tests at fixed iteration counts) and raises otherwise."""
+
+  def __init__(self, engine, graph, iterations, loop, plan):
+    # a weak reference: the engine holds its graph, and a cycle would leave the graph
+    # to the cyclic collector
+    self._engine = weakref.ref(engine)
+    self.graph = graph
+    self.iterations, self.loop, self.plan = iterations, loop, plan
+
+  def replay(self):
+    e = self._engine()
+    if e is None:
+      raise RuntimeError("the engine of this graph no longer exists")
+    if e._plan(self.iterations, self.loop) != self.plan:
+      raise RuntimeError("this graph was recorded at another position relative to the "
+                         "exchange's test iterations: replay through FusedGLM.replay / run, "
+                         "which capture again when needed")
+    self.graph.replay()
+    e.host_iter += self.iterations
+
+
 class TableGroup:
   """Several chain tables advanced side by side, each on its own HIP stream.
 
@@ -398,16 +482,30 @@ class TableGroup:
   engines: FusedGLM tables. A replica exchange or MH test attached to a table acts on
   that table's chains only and runs on its stream after each of its iterations. Results
   are indexed over the group's chains in table order.
+  stream_of: the stream of each table (indices from 0; tables on one stream run one
+  after the other). Default: one stream per table. The group's streams come from a
+  per-device pool of streams with hardware queues of their own
+  (``_lib.dedicated_stream``): HIP gives a process few hardware queues
+  (GPU_MAX_HW_QUEUES, 4 by default) and shares them among ordinary streams, and the
+  kernels of two streams on one queue run one after the other. Two groups used at the
+  same time share the pool's streams.
   """
 
-  def __init__(self, engines):
+  def __init__(self, engines, stream_of=None):
     engines = list(engines)
     if not engines:
       raise ValueError("TableGroup needs at least one table")
     if len({str(e.device) for e in engines}) != 1:
       raise ValueError("the tables of a TableGroup share one device")
     self.engines = engines
-    self.streams = [torch.cuda.Stream(device=e.device) for e in engines]
+    if stream_of is None:
+      stream_of = list(range(len(engines)))
+    stream_of = [int(i) for i in stream_of]
+    if len(stream_of) != len(engines) or min(stream_of) < 0:
+      raise ValueError("stream_of needs one stream index >= 0 per table")
+    self.stream_of = stream_of
+    self.streams = [_lib.dedicated_stream(engines[0].device, i)
+                    for i in range(max(stream_of) + 1)]
     self.offsets = np.concatenate([[0], np.cumsum([e.C for e in engines])]).astype(np.int64)
     self.C = int(self.offsets[-1])
     self._graphs = None
@@ -439,9 +537,17 @@ class TableGroup:
     for s in self.streams:
       cur.wait_stream(s)
 
+  def _side_stream(self):
+    if getattr(self, "_side", None) is None:
+      self._side = torch.cuda.Stream(device=self.engines[0].device)
+    return self._side
+
+  def _table_streams(self):
+    return [self.streams[i] for i in self.stream_of]
+
   def _each(self, fn):
     self._fork()
-    for e, s in zip(self.engines, self.streams):
+    for e, s in zip(self.engines, self._table_streams()):
       with torch.cuda.stream(s):
         fn(e)
     self._join()
@@ -459,50 +565,98 @@ class TableGroup:
 
   def run(self, iterations, graph_iterations=0):
     """Runs ``iterations`` iterations of every table (by replaying graphs of
-    ``graph_iterations`` iterations when given; the remainder eagerly). The current
-    stream waits for all tables."""
+    ``graph_iterations`` iterations when given, captured again where a table's launch
+    sequence no longer fits; the remainder eagerly). The current stream waits for all
+    tables."""
     rest = iterations
     if graph_iterations and iterations >= graph_iterations:
-      if self._graphs is None or self._graph_iters != graph_iterations:
-        self.capture(graph_iterations)
       reps, rest = divmod(iterations, graph_iterations)
       self._fork()
       for _ in range(reps):
-        for g, s in zip(self._graphs, self.streams):
+        for e, s in zip(self.engines, self._table_streams()):
           with torch.cuda.stream(s):
-            g.replay()
+            e.replay(graph_iterations)
       self._join()
-    for _ in range(rest):
-      self.iteration()
+    if rest:
+      self._each(lambda e: e.run(rest))
 
-  def autotune(self, candidates=None, reps=3):
-    """Picks one bytes-per-block target for every table by timing their launches run
-    concurrently; restores the chain states afterwards."""
+  def autotune(self, candidates=None, reps=50, tables=None):
+    """Chooses the bytes-per-block target of the tables listed in ``tables`` (indices;
+    default all): first one target common to them, then each table's own, one table at
+    a time. Each candidate is timed as a HIP graph in which every table of the group
+    runs ``reps`` iterations' evaluations back to back on its stream (loop-capable
+    tables not being tuned in one loop launch, as they run), with HIP events around a
+    replay after a warm-up replay. Exchanges and tests are not launched; the chain
+    states are restored afterwards."""
+    tables = list(range(len(self.engines))) if tables is None else list(tables)
+    if not tables:
+      return None
+    eng = [self.engines[i] for i in tables]
     if candidates is None:
-      base = max(e.target_bytes for e in self.engines)
+      base = max(e.target_bytes for e in eng)
       candidates = sorted({max(16 * 1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4, 8, 16)})
     snaps = [e._snapshot() for e in self.engines]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    built = {}
+
+    def block():
+      cur = torch.cuda.current_stream()
+      for s in self.streams:
+        s.wait_stream(cur)
+      for i, (e, s) in enumerate(zip(self.engines, self._table_streams())):
+        with torch.cuda.stream(s):
+          if i not in tables and e.loop_capable():
+            e._launch_loop(reps)
+          else:
+            for _ in range(reps):
+              e._launch_steps()
+      for s in self.streams:
+        cur.wait_stream(s)
+
+    def timed(targets):
+      for e, tb in zip(eng, targets):
+        if built.get(id(e)) != tb:
+          e.target_bytes = tb
+          e._build_table()
+          built[id(e)] = tb
+      g = torch.cuda.CUDAGraph()
+      side = self._side_stream()
+      side.wait_stream(torch.cuda.current_stream())
+      with _lib.capture_guard(), torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+          block()
+      torch.cuda.current_stream().wait_stream(side)
+      g.replay()
+      e0.record()
+      g.replay()
+      e1.record()
+      e1.synchronize()
+      del g
+      return e0.elapsed_time(e1) / 1e3 / reps
+
     best = None
     for tb in candidates:
-      for e in self.engines:
-        e.target_bytes = tb
-        e._build_table()
-      self._each(lambda e: e._launch_steps())
-      torch.cuda.synchronize()
-      t0 = time.perf_counter()
-      for _ in range(reps):
-        self._each(lambda e: e._launch_steps())
-      torch.cuda.synchronize()
-      dt = (time.perf_counter() - t0) / reps
+      dt = timed([tb] * len(eng))
       if best is None or dt < best[0]:
-        best = (dt, tb)
-    for e, snap in zip(self.engines, snaps):
-      e.target_bytes = best[1]
+        best = (dt, [tb] * len(eng))
+    for j in range(len(eng)) if len(eng) > 1 else []:
+      for tb in candidates:
+        if tb == best[1][j]:
+          continue
+        trial = list(best[1])
+        trial[j] = tb
+        dt = timed(trial)
+        if dt < best[0]:
+          best = (dt, trial)
+    for e, tb in zip(eng, best[1]):
+      e.target_bytes = tb
       e._build_table()
+      e._graph = None
+    for e, snap in zip(self.engines, snaps):
       e._restore(snap)
     self._graphs = None
     self.tuned = {"target_bytes": best[1], "seconds_per_iteration": best[0],
-                  "candidates": list(candidates), "tables": len(self.engines)}
+                  "candidates": list(candidates), "tables": tables}
     return self.tuned
 
   def theta(self, i=None):
@@ -537,6 +691,12 @@ class ReplicaExchange:
   0 / 1 (even / odd rounds of a ladder); save: per pair, keep the lo chain's samples at
   the engine's save slots (taken after the exchange, as the reference saves the normal
   chain). Pairs tested in one round must not share a chain.
+
+  The swap kernel tests when the engine's iteration count (capped at the schedule's
+  length, as the step size index is) is a multiple of swap_every. Without saving
+  pairs the other launches do nothing, so the engine launches the kernel only at those
+  iterations (``sparse``); saving pairs copy the lo chain after every iteration, which
+  needs every launch.
   """
 
   def __init__(self, engine, pairs, temperatures, keys=None, swap_every=1, parity=None,
@@ -597,7 +757,28 @@ class ReplicaExchange:
         k += 1
     self.table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
     self.pairs = list(pairs)
+    self.sparse = not any(save)
     engine.exchange = self
+    engine._graph = None
+
+  def needs_launch(self, done):
+    """Whether the swap kernel has work after the engine's iteration ``done`` (count
+    of completed iterations)."""
+    if not self.sparse:
+      return True
+    d = min(int(done), self.engine.table_len)
+    return d >= 1 and d % self.swap_every == 0
+
+  def iterations_to_next_launch(self, done):
+    """Iterations after ``done`` up to and including the next one that needs the swap
+    kernel, or None when none does."""
+    if not self.sparse:
+      return 1
+    S, L = self.swap_every, self.engine.table_len
+    if done >= L:
+      return 1 if L % S == 0 else None
+    nxt = (int(done) // S + 1) * S
+    return nxt - int(done) if nxt <= L else None
 
   def launch(self):
     eng = self.engine
@@ -717,6 +898,7 @@ class AmagoldMH(_SharedDataTest):
     self._set_integrator_keys(int_words)
     self._full(self.U_full)
     engine.mh = self
+    engine._graph = None
 
   def launch(self):
     eng = self.engine
@@ -746,6 +928,7 @@ class SggmcMH(_SharedDataTest):
     self.U_samples = torch.zeros(max(1, C * self.n_slots), dtype=torch.float64, device=dev)
     self._full(self.U_full)
     engine.mh = self
+    engine._graph = None
 
   def launch(self):
     eng = self.engine

@@ -367,7 +367,7 @@ class FusedMLP:
     g = torch.cuda.CUDAGraph()
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
+    with _lib.capture_guard(), torch.cuda.stream(s):
       with torch.cuda.graph(g, stream=s):
         for _ in range(iterations_per_graph):
           self.iteration()

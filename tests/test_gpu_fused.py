@@ -166,21 +166,21 @@ def test_fused_autotune_preserves_state():
                              rtol=1e-4, atol=1e-5)
 
 
-def _ladder(R_=4, D=24, N=700, swap_every=3, iters=12, graph=0):
+def _ladder(R_=4, D=24, N=700, swap_every=3, iters=12, graph=0, saving=True, target_bytes=None):
   """One logistic problem, R_ rungs on a geometric temperature ladder, adjacent pairs
-  tested in alternating even / odd rounds."""
+  tested in alternating even / odd rounds (rung 0 saved when ``saving``)."""
   X, y = _data(N, D, "logistic", seed=42)
   Xd, yd = _dev(X), torch.as_tensor(y, device=DEV)
   temps = [20.0 ** (j / (R_ - 1)) for j in range(R_)]
   chains = [dict(X=Xd, y=yd, D=D, n=64, theta0=np.full(D, 0.05 * j, np.float32), key=(2, j),
                  dkey=(3, j), temperature=temps[j], save=False) for j in range(R_)]
   eng = fused.FusedGLM(chains, "logistic", "sgld", np.full(iters, 2e-3, np.float32),
-                       save_slots=np.arange(iters), n_slots=iters)
+                       save_slots=np.arange(iters), n_slots=iters, target_bytes=target_bytes)
   pairs = [(j, j + 1) for j in range(R_ - 1)]
   ex = fused.ReplicaExchange(eng, pairs, [(temps[j], temps[j + 1]) for j in range(R_ - 1)],
                              keys=[(5, j) for j in range(R_ - 1)], swap_every=swap_every,
                              parity=[j % 2 for j in range(R_ - 1)],
-                             save=[j == 0 for j in range(R_ - 1)])
+                             save=[saving and j == 0 for j in range(R_ - 1)])
   return eng, ex
 
 
@@ -232,6 +232,86 @@ def test_exchange_ladder_graph_equals_eager():
   assert torch.equal(a.theta_flat, b.theta_flat)
   assert torch.equal(exa.swaps, exb.swaps) and torch.equal(exa.ssq, exb.ssq)
   assert torch.equal(exa.samples, exb.samples)
+
+
+def _same_ladder(a, exa, b, exb):
+  assert a.iteration_count() == b.iteration_count()
+  assert torch.equal(a.theta_flat, b.theta_flat) and torch.equal(a.keys, b.keys)
+  assert torch.equal(a.dkeys, b.dkeys) and torch.equal(a.U, b.U)
+  assert torch.equal(exa.swaps, exb.swaps) and torch.equal(exa.ssq, exb.ssq)
+  assert torch.equal(exa.step, exb.step) and torch.equal(exa.keys, exb.keys)
+  assert torch.equal(a.samples, b.samples)
+
+
+@pytest.mark.parametrize("how,iters,swap_every,target_bytes", [
+    ("eager", 12, 3, None), ("graph", 24, 3, None), ("graph", 14, 4, None),
+    ("loop", 12, 3, 1 << 30), ("loop-graph", 24, 4, 1 << 30), ("eager", 20, 3, None),
+    ("loop", 20, 4, 1 << 30)])
+def test_sparse_exchange_equals_launch_after_every_iteration(how, iters, swap_every,
+                                                             target_bytes):
+  """Without saving pairs the swap kernel is launched only after its test iterations
+  (eagerly, from graphs, between loop launches); the chains, exchange statistics and
+  keys equal those of a launch after every iteration. The schedule has 12 entries, so
+  runs of 20 iterations cover the capped iteration count (12 % 3 == 0: a test after
+  every iteration past the schedule; 12 % 4 == 0 too; 20 / 4 with table 12)."""
+  print("sparse exchange", how, iters, swap_every, flush=True)
+  a, exa = _ladder(saving=False, swap_every=swap_every, target_bytes=target_bytes)
+  b, exb = _ladder(saving=False, swap_every=swap_every, target_bytes=target_bytes)
+  exb.sparse = False  # the reference: launch after every iteration
+  assert exa.sparse and a.table_len == 12
+  if how.startswith("loop"):
+    assert a.loop_capable() and not b.loop_capable()
+  if how == "eager":
+    a.run(iters, loop=False)
+  elif how == "graph":
+    a.run(iters, graph_iterations=6, loop=False)
+  elif how == "loop":
+    a.run(iters, loop=True)
+  else:
+    a.run(iters, graph_iterations=8, loop=True)
+  b.run(iters)
+  torch.cuda.synchronize()
+  assert int(exa.step.sum()) > 0
+  _same_ladder(a, exa, b, exb)
+
+
+def test_table_group_autotune_restores_state():
+  """TableGroup.autotune on one table of a group (the other, loop-capable, timed as it
+  runs): afterwards the group runs exactly as a fresh group built with the chosen
+  targets (chains, exchange statistics, keys)."""
+  print("table group autotune", flush=True)
+  a = [_ladder(saving=False, R_=4), _ladder(saving=False, R_=5, D=40, N=900)]
+  b = [_ladder(saving=False, R_=4), _ladder(saving=False, R_=5, D=40, N=900)]
+  ga = fused.TableGroup([e for e, _ in a])
+  tuned = ga.autotune(candidates=[4 << 10, 64 << 10], reps=3, tables=[0])
+  assert tuned["tables"] == [0] and len(tuned["target_bytes"]) == 1
+  e_b = b[0][0]
+  e_b.target_bytes = a[0][0].target_bytes
+  e_b._build_table()
+  assert list(e_b.nblocks_chain) == list(a[0][0].nblocks_chain)
+  gb = fused.TableGroup([e for e, _ in b])
+  ga.run(12, graph_iterations=6)
+  gb.run(12, graph_iterations=6)
+  torch.cuda.synchronize()
+  for (ea, xa), (eb, xb) in zip(a, b):
+    _same_ladder(ea, xa, eb, xb)
+
+
+def test_sparse_exchange_graph_recaptured_after_eager_iterations():
+  """A graph recorded at one position relative to the test iterations is not replayed
+  at another: EngineGraph.replay raises, run() captures again."""
+  print("sparse exchange graph phase", flush=True)
+  a, exa = _ladder(saving=False, swap_every=3)
+  b, exb = _ladder(saving=False, swap_every=3)
+  exb.sparse = False
+  g = a.capture(3, loop=False)
+  a.run(1, loop=False)
+  with pytest.raises(RuntimeError, match="another position"):
+    g.replay()
+  a.run(9, graph_iterations=3, loop=False)
+  b.run(10)
+  torch.cuda.synchronize()
+  _same_ladder(a, exa, b, exb)
 
 
 @pytest.mark.parametrize("mode", ["sgld", "sgld_rms", "sghmc"])
