@@ -295,11 +295,18 @@ __device__ __forceinline__ void upd_st(T v, T* p) {
   else *p = v;
 }
 
+// UM: the update rule compiled in -- UM_RMS: SGLD with rms_prop (state v in aux); or
+// UM_ANY: the rule of u.mode / u.rms at run time.
+constexpr int UM_ANY = 0;
+constexpr int UM_RMS = 1;
+
+template <int UM = UM_ANY>
 __device__ __forceinline__ void upd_load(const Upd& u, uint32_t i, float& w, float& aux) {
   w = upd_ld(u.th + i);
-  aux = u.aux ? upd_ld(u.aux + i) : 0.0f;
+  aux = (UM == UM_RMS || u.aux) ? upd_ld(u.aux + i) : 0.0f;
 }
 
+template <int UM = UM_ANY>
 __device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, float gsum,
                                          float xi, float& lp, float& ke) {
   float g = gsum;
@@ -308,14 +315,16 @@ __device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, flo
     lp = __builtin_fmaf(__builtin_fmaf(w * w, u.inv_var, u.log_norm), -0.5f, lp);
   }
   g = g * u.inv_T;
-  if (u.mode == MLP_SGLD) {
+  const int mode = UM == UM_RMS ? MLP_SGLD : u.mode;
+  if (mode == MLP_SGLD) {
     const float sg = (-u.eps) * g;
     const float sn = u.noise_scale * xi;
-    if (u.rms) {
+    if (UM == UM_RMS || u.rms) {
       const float vn = __builtin_fmaf(u.alpha, aux, u.one_m_alpha * (g * g));
       aux = vn;
-      const float G = __builtin_amdgcn_rcpf(u.lmbd + __builtin_amdgcn_sqrtf(vn));
-      w = w + __builtin_fmaf(__builtin_amdgcn_sqrtf(G), sn, G * sg);
+      // G = 1 / (lambda + sqrt(v)) and sqrt(G) from one reciprocal square root
+      const float rG = __builtin_amdgcn_rsqf(u.lmbd + __builtin_amdgcn_sqrtf(vn));
+      w = w + __builtin_fmaf(rG, sn, (rG * rG) * sg);
     } else {
       w = w + (sg + sn);
     }
@@ -332,18 +341,65 @@ __device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, flo
   }
 }
 
+// SH: the GEMM copy written -- SH_NONE, SH_BF16, SH_E4M3 (known from the operand
+// precision in the parameter-update GEMMs), or SH_ANY: whichever pointer is set.
+constexpr int SH_ANY = -1, SH_NONE = 0, SH_BF16 = 1, SH_E4M3 = 2;
+
+template <int UM = UM_ANY, int SH = SH_ANY>
 __device__ __forceinline__ void upd_store(const Upd& u, uint32_t i, float w, float aux,
                                           float& wmax) {
-  if (u.mode != MLP_OBABO_BO) {
+  if (UM == UM_RMS || u.mode != MLP_OBABO_BO) {
     upd_st(w, u.th + i);
-    if (u.sh8) {
+    if (SH == SH_E4M3 || (SH == SH_ANY && u.sh8)) {
       upd_st(f2q(w * u.q8), u.sh8 + i);
       wmax = fmaxf(wmax, fabsf(w));
-    } else if (u.sh) {
+    } else if (SH == SH_BF16 || (SH == SH_ANY && u.sh)) {
       upd_st(f2bf(w), u.sh + i);
     }
   }
-  if (u.aux) upd_st(aux, u.aux + i);
+  if (UM == UM_RMS || u.aux) upd_st(aux, u.aux + i);
+}
+
+// Buffer resources of one leaf's state (theta, v, GEMM copy) for the compiled-in
+// rms_prop update: a lane addresses its parameters with one 32-bit offset and the
+// per-element offsets are uniform (scalar offset field), so no per-element address
+// arithmetic runs on the vector ALU.
+constexpr int kRawBufferFlags = 0x00020000;  // word 3 of a raw buffer resource (CDNA)
+
+struct UpdRes {
+  __amdgpu_buffer_rsrc_t th, aux, sh;
+};
+
+template <int SH>
+__device__ __forceinline__ UpdRes make_res(const Upd& u, uint32_t count) {
+  UpdRes r;
+  r.th = __builtin_amdgcn_make_buffer_rsrc(u.th, (short)0, (int)(4 * count), kRawBufferFlags);
+  r.aux = __builtin_amdgcn_make_buffer_rsrc(u.aux, (short)0, (int)(4 * count), kRawBufferFlags);
+  if (SH == SH_E4M3)
+    r.sh = __builtin_amdgcn_make_buffer_rsrc(u.sh8, (short)0, (int)count, kRawBufferFlags);
+  else
+    r.sh = __builtin_amdgcn_make_buffer_rsrc(u.sh, (short)0, (int)(2 * count), kRawBufferFlags);
+  return r;
+}
+
+// element vo + so (vo per lane, so uniform)
+__device__ __forceinline__ void res_load(const UpdRes& r, uint32_t vo, uint32_t so, float& w,
+                                         float& aux) {
+  w = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r.th, 4 * vo, 4 * so, 0));
+  aux = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r.aux, 4 * vo, 4 * so, 0));
+}
+
+template <int SH>
+__device__ __forceinline__ void res_store(const Upd& u, const UpdRes& r, uint32_t vo,
+                                          uint32_t so, float w, float aux, float& wmax) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(w), r.th, 4 * vo, 4 * so, 0);
+  if (SH == SH_E4M3) {
+    __builtin_amdgcn_raw_buffer_store_b8(f2q(w * u.q8), r.sh, vo, so, 0);
+    wmax = fmaxf(wmax, fabsf(w));
+  } else if (SH == SH_BF16) {
+    __builtin_amdgcn_raw_buffer_store_b16(f2bf(w), r.sh, 2 * vo, 2 * so, 0);
+  }
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(aux), r.aux, 4 * vo, 4 * so, 0);
 }
 
 // Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
@@ -649,7 +705,7 @@ __device__ __forceinline__ float get_act(const Step& s, int c, int t, const void
   return bf2f(reinterpret_cast<const uint16_t*>(p)[i]);
 }
 
-template <int BK, int PREC, bool A_K, bool B_K, int EPI>
+template <int BK, int PREC, bool A_K, bool B_K, int EPI, int UM = UM_ANY>
 __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
     gemm_kernel(Gemm g, Step s) {
   static_assert(BN == 2 * BM, "tile shapes");
@@ -756,20 +812,28 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
       // written by this thread only), so its memory time overlaps the arithmetic.
       constexpr int CJ = kUpdChunk;
       float w0[CJ], a0[CJ], w1[CJ], a1[CJ];
+      constexpr int SH = PREC == P_BF16 ? SH_BF16 : PREC == P_FP8 ? SH_E4M3 : SH_NONE;
+      // compiled-in rms_prop: buffer-resource accesses, lane offset o0 plus the uniform
+      // row offset of element j (and oh for its band-1 pair)
+      const UpdRes rs = UM == UM_RMS ? make_res<SH>(up, (uint32_t)s.size[g.leaf]) : UpdRes{};
+      auto so = [&](int j) -> uint32_t { return (uint32_t)(8 * (j >> 2) + (j & 3)) * (uint32_t)g.N; };
+      auto load2 = [&](int j, float& wa, float& aa, float& wb, float& ab_) {
+        if constexpr (UM == UM_RMS) {
+          res_load(rs, o0, so(j), wa, aa);
+          res_load(rs, o0, so(j) + oh, wb, ab_);
+        } else {
+          upd_load<UM>(up, off(j), wa, aa);
+          upd_load<UM>(up, off(j) + oh, wb, ab_);
+        }
+      };
 #pragma unroll
-      for (int u = 0; u < CJ; ++u) {
-        upd_load(up, off(u), w0[u], a0[u]);
-        upd_load(up, off(u) + oh, w1[u], a1[u]);
-      }
+      for (int u = 0; u < CJ; ++u) load2(u, w0[u], a0[u], w1[u], a1[u]);
 #pragma unroll
       for (int j0 = 0; j0 < 16; j0 += CJ) {
         float nw0[CJ], na0[CJ], nw1[CJ], na1[CJ];
         if (j0 + CJ < 16) {
 #pragma unroll
-          for (int u = 0; u < CJ; ++u) {
-            upd_load(up, off(j0 + CJ + u), nw0[u], na0[u]);
-            upd_load(up, off(j0 + CJ + u) + oh, nw1[u], na1[u]);
-          }
+          for (int u = 0; u < CJ; ++u) load2(j0 + CJ + u, nw0[u], na0[u], nw1[u], na1[u]);
         }
 #pragma unroll
         for (int u = 0; u < CJ; ++u) {
@@ -777,10 +841,15 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
           const uint32_t i0 = off(j);
           uint32_t b0, b1;
           bits_pair(lk, S, i0, b0, b1);
-          upd_step(up, w0[u], a0[u], ab * acc0[j], bits_to_normal_fast(b0), lp, ke);
-          upd_step(up, w1[u], a1[u], ab * acc1[j], bits_to_normal_fast(b1), lp, ke);
-          upd_store(up, i0, w0[u], a0[u], wmax);
-          upd_store(up, i0 + oh, w1[u], a1[u], wmax);
+          upd_step<UM>(up, w0[u], a0[u], ab * acc0[j], bits_to_normal_fast(b0), lp, ke);
+          upd_step<UM>(up, w1[u], a1[u], ab * acc1[j], bits_to_normal_fast(b1), lp, ke);
+          if constexpr (UM == UM_RMS) {
+            res_store<SH>(up, rs, o0, so(j), w0[u], a0[u], wmax);
+            res_store<SH>(up, rs, o0, so(j) + oh, w1[u], a1[u], wmax);
+          } else {
+            upd_store<UM, SH>(up, i0, w0[u], a0[u], wmax);
+            upd_store<UM, SH>(up, i0 + oh, w1[u], a1[u], wmax);
+          }
         }
         if (j0 + CJ < 16) {
 #pragma unroll
@@ -825,7 +894,7 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
     const int64_t slot = (int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn;
     const float tot = block_sum<GT>(lp, red);
     if (threadIdx.x == 0) s.prior_part[slot] = tot;
-    if (EPI == EPI_UPDATE && s.mode != MLP_SGLD) {
+    if (EPI == EPI_UPDATE && UM != UM_RMS && s.mode != MLP_SGLD) {
       const float kt = block_sum<GT>(ke, red);
       if (threadIdx.x == 0) s.ke_part[slot] = kt;
     }
@@ -1388,6 +1457,16 @@ enum GemmSlot { SLOT_G1 = 0, SLOT_G2, SLOT_G3, SLOT_G4, SLOT_G5, kSlots };
 template <int BK, int PREC, bool A_K, bool B_K, int EPI>
 void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part,
                     int splits) {
+  // parameter updates of SGLD with rms_prop run the kernel with that rule compiled in
+  if constexpr (EPI == EPI_UPDATE) {
+    if (s.mode == MLP_SGLD && s.rms && splits <= 1) {
+      g.C = (int)grid.z;
+      g.splits = 1;
+      hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI, UM_RMS>), grid, dim3(GT), 0, st,
+                         g, s);
+      return;
+    }
+  }
   g.C = (int)grid.z;
   g.splits = splits;
   if (g.splits > 1) {

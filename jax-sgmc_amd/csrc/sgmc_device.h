@@ -241,30 +241,32 @@ __device__ __forceinline__ float bits_to_normal_fast(uint32_t bits) {
   const float f = unit_float(bits);
   const float x = fmaxf(lo, f * 2.0f + lo);
   const float x2 = x * x;
-  float w = -0.693147182f * __builtin_amdgcn_logf(1.0f - x2);
-  float p;
-  if (w < 5.0f) {
-    w = w - 2.5f;
-    p = 2.81022636e-08f;
-    p = __builtin_fmaf(p, w, 3.43273939e-07f);
-    p = __builtin_fmaf(p, w, -3.5233877e-06f);
-    p = __builtin_fmaf(p, w, -4.39150654e-06f);
-    p = __builtin_fmaf(p, w, 0.00021858087f);
-    p = __builtin_fmaf(p, w, -0.00125372503f);
-    p = __builtin_fmaf(p, w, -0.00417768164f);
-    p = __builtin_fmaf(p, w, 0.246640727f);
-    p = __builtin_fmaf(p, w, 1.50140941f);
-  } else {
-    w = __builtin_amdgcn_sqrtf(w) - 3.0f;
-    p = -0.000200214257f;
-    p = __builtin_fmaf(p, w, 0.000100950558f);
-    p = __builtin_fmaf(p, w, 0.00134934322f);
-    p = __builtin_fmaf(p, w, -0.00367342844f);
-    p = __builtin_fmaf(p, w, 0.00573950773f);
-    p = __builtin_fmaf(p, w, -0.0076224613f);
-    p = __builtin_fmaf(p, w, 0.00943887047f);
-    p = __builtin_fmaf(p, w, 1.00167406f);
-    p = __builtin_fmaf(p, w, 2.83297682f);
+  const float w = -0.693147182f * __builtin_amdgcn_logf(1.0f - x2);
+  // central branch (w < 5) on every lane; the tail (|x| > 0.9966, about 1 lane in 300)
+  // only when a lane of the wavefront needs it, a wave-uniform branch instead of both
+  // polynomials on every lane
+  const float wc = w - 2.5f;
+  float p = 2.81022636e-08f;
+  p = __builtin_fmaf(p, wc, 3.43273939e-07f);
+  p = __builtin_fmaf(p, wc, -3.5233877e-06f);
+  p = __builtin_fmaf(p, wc, -4.39150654e-06f);
+  p = __builtin_fmaf(p, wc, 0.00021858087f);
+  p = __builtin_fmaf(p, wc, -0.00125372503f);
+  p = __builtin_fmaf(p, wc, -0.00417768164f);
+  p = __builtin_fmaf(p, wc, 0.246640727f);
+  p = __builtin_fmaf(p, wc, 1.50140941f);
+  if (__builtin_expect(__any(w >= 5.0f), 0)) {
+    const float wt = __builtin_amdgcn_sqrtf(w) - 3.0f;
+    float pt = -0.000200214257f;
+    pt = __builtin_fmaf(pt, wt, 0.000100950558f);
+    pt = __builtin_fmaf(pt, wt, 0.00134934322f);
+    pt = __builtin_fmaf(pt, wt, -0.00367342844f);
+    pt = __builtin_fmaf(pt, wt, 0.00573950773f);
+    pt = __builtin_fmaf(pt, wt, -0.0076224613f);
+    pt = __builtin_fmaf(pt, wt, 0.00943887047f);
+    pt = __builtin_fmaf(pt, wt, 1.00167406f);
+    pt = __builtin_fmaf(pt, wt, 2.83297682f);
+    p = w < 5.0f ? p : pt;
   }
   return 1.41421354f * (p * x);
 }
