@@ -208,6 +208,50 @@ def test_glm_potential_block_handoff_repeatable():
   assert close(var.item(), np.var(likr), rtol=1e-4)[0]
 
 
+@pytest.mark.parametrize("blocks,D", [(257, 64), (1024, 1024), (4093, 64), (2048, 512)])
+def test_glm_block_handoff_stress(blocks, D):
+  """The cross-block hand-off (records written with agent-scope atomic stores, completed
+  before the arrival counter's increment, read by the last block with agent-scope atomic
+  loads) under stress: 150 launches with block counts that spread the last-arriving
+  block over all XCDs, L2-resident writes of unrelated data and a concurrent kernel on a
+  second stream between launches. Every launch must be bit-identical to the first and
+  the first must match the oracle."""
+  print("glm handoff stress", blocks, D, flush=True)
+  N, n = 30000, 4096
+  X, y = _glm_data(N, D, L.LIK_LOGISTIC, seed=blocks)
+  rng = np.random.default_rng(blocks)
+  theta = (0.3 * rng.standard_normal(D)).astype(np.float32)
+  idx = rng.integers(0, N, n).astype(np.int32)
+  Xd, ldx = _padded(X)
+  yd = torch.as_tensor(y, device=DEV)
+  idx_d = torch.as_tensor(idx, device=DEV)
+  th = torch.as_tensor(theta, device=DEV)
+  ws = torch.zeros(int(L.query("sgmc_glm_workspace_floats", n, D, blocks)), device=DEV)
+  U = torch.empty(1, device=DEV)
+  g = torch.empty(D, device=DEV)
+  var = torch.empty(1, device=DEV)
+  junk = torch.empty(1 << 20, device=DEV)
+  side = torch.cuda.Stream()
+  other = torch.empty(1 << 22, device=DEV)
+  first = None
+  for r in range(150):
+    junk.fill_(float(r))
+    with torch.cuda.stream(side):
+      other.mul_(0.5).add_(1.0)
+    L.call("sgmc_glm_potential", L.LIK_LOGISTIC, L.ptr(Xd), ldx, L.ptr(yd), L.ptr(idx_d), n, N,
+           D, L.ptr(th), 1.0, 1, 1.0, 1.0, L.ptr(U), L.ptr(g), None, L.ptr(var), L.ptr(ws),
+           ws.numel(), blocks, L.stream())
+    out = torch.cat([U, g, var])
+    if first is None:
+      first = out.clone()
+    else:
+      assert torch.equal(out, first), f"launch {r} differs"
+  torch.cuda.synchronize()
+  Ur, gr, likr = om.stochastic_potential(om.Family("logistic", X, y), theta, idx, 1.0, 1.0)
+  assert close(first[0].item(), Ur)[0]
+  assert close(first[1:1 + D].cpu().numpy(), gr)[0]
+
+
 @pytest.mark.parametrize("family", [L.LIK_LINEAR_GAUSSIAN, L.LIK_LOGISTIC])
 @pytest.mark.parametrize("N,D", [(1, 3), (1000, 32), (12345, 512)])
 def test_glm_full_potential_f64(family, N, D):
