@@ -3,6 +3,7 @@
 Every entry point of include/sgmc.h is declared here with its argument types.
 Loading fails loudly: there is no fallback implementation of any kernel.
 """
+import atexit
 import contextlib
 import ctypes
 import gc
@@ -244,13 +245,28 @@ def capture_guard():
 _DEDICATED = {}
 
 
+def _release_dedicated():
+  """At interpreter exit: waits for the pool's streams and destroys them, so that the
+  runtime's own teardown finds no queue of ours left."""
+  for (dev, _), st in list(_DEDICATED.items()):
+    try:
+      with torch.cuda.device(dev):
+        st.synchronize()
+        call("sgmc_stream_destroy", c_void_p(st.cuda_stream))
+    except Exception:  # exiting: nothing left to report to
+      pass
+  _DEDICATED.clear()
+
+
 def dedicated_stream(device, index):
   """Stream ``index`` of a per-device pool of streams with hardware queues of their own
-  (sgmc_stream_create), created on first use and kept until the process ends; wrapped
+  (sgmc_stream_create), created on first use and destroyed at interpreter exit; wrapped
   as a torch stream."""
   dev = torch.device(device)
   key = (dev.index if dev.index is not None else torch.cuda.current_device(), int(index))
   if key not in _DEDICATED:
+    if not _DEDICATED:
+      atexit.register(_release_dedicated)
     with torch.cuda.device(key[0]):
       h = c_void_p()
       call("sgmc_stream_create", ctypes.byref(h))
