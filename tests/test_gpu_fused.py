@@ -166,6 +166,38 @@ def test_fused_autotune_preserves_state():
                              rtol=1e-4, atol=1e-5)
 
 
+def test_fused_sghmc_logistic_c2_full_size():
+  """The headline configuration at its stated size: SGHMC on logistic regression with
+  N = 1e6, D = 512, n = 4096, 5 leapfrog steps, friction 0.5, 64 chains as two chain
+  tables on two streams (as the bench runs it). Two iterations (10 gradient evaluations
+  of 4096 rows per chain); chains 0 and 63 (one per table) against the float64 oracle:
+  positions, momenta and keys, within 1e-5 relative (floor 1e-5 max|x|)."""
+  print("fused SGHMC, C2 size", flush=True)
+  N, D, n, C, its, steps, fric = 1_000_000, 512, 4096, 64, 2, 5, 0.5
+  rng = np.random.default_rng(2024)
+  X = (rng.standard_normal((N, D), dtype=np.float32) / np.float32(np.sqrt(D)))
+  w = rng.standard_normal(D).astype(np.float32)
+  y = (rng.random(N) < 1 / (1 + np.exp(-(X @ w)))).astype(np.float32)
+  Xd, yd = torch.as_tensor(X, device=DEV), torch.as_tensor(y, device=DEV)
+  eps = np.array([2e-3, 1.5e-3], np.float32)
+  chains = [dict(X=Xd, y=yd, D=D, n=n, theta0=np.zeros(D, np.float32), key=(0, c),
+                 dkey=(2, c)) for c in range(C)]
+  grp = fused.TableGroup.split(chains, 2, "logistic", "sghmc", eps, steps=steps,
+                               friction=fric, prior_scale=1.0)
+  grp.run(its)
+  torch.cuda.synchronize()
+  fam = om.Family("logistic", X, y)
+  pot = lambda th, idx: om.stochastic_potential(fam, th, idx, 1.0)
+  for c in (0, C - 1):
+    r = ost.sghmc(pot, np.zeros(D), its, eps, steps, fric, N, n, [D],
+                  key=np.array([0, c], np.uint32), dkey=np.array([2, c], np.uint32))
+    _check(grp.theta(c).cpu().numpy(), r["theta"][-1], f"chain {c} theta")
+    _check(grp.momentum(c).cpu().numpy(), r["p"][-1], f"chain {c} momentum")
+    e, j = grp._locate(c)
+    np.testing.assert_array_equal(R.key_words(e.keys[j]), r["key"])
+    np.testing.assert_array_equal(R.key_words(e.dkeys[j]), r["dkey"])
+
+
 def _ladder(R_=4, D=24, N=700, swap_every=3, iters=12, graph=0, saving=True, target_bytes=None):
   """One logistic problem, R_ rungs on a geometric temperature ladder, adjacent pairs
   tested in alternating even / odd rounds (rung 0 saved when ``saving``)."""
