@@ -321,25 +321,38 @@ def ladder_problems(torch, R):
   return out
 
 
-def ladder_tables(torch, fused, R, probs, eps, G):
-  """The 32 ladders as one chain table per D class on its own stream (fused.TableGroup),
-  each table with its own replica exchange. Per class the kernel is compiled for that
-  row width only (fewer registers, more waves per SIMD than the mixed-class kernel);
-  the D = 16 and 64 tables keep one block per chain and run their iterations in loop
-  launches cut at the exchange's test iterations, concurrent with the D = 256 and 1024
-  tables, which take one launch per iteration. Keys are those of the chains' positions
-  in a single table ordered by problem, so every chain and exchange gives what it gives
-  there."""
+LADDER_GROUPS = ((16,), (64,), (256,), (1024,))
+LADDER_STREAMS = (0, 0, 1, 2)
+
+
+def ladder_tables(torch, fused, R, probs, eps, G, groups=LADDER_GROUPS,
+                  stream_of=LADDER_STREAMS):
+  """The 32 ladders as chain tables (fused.TableGroup), one per group of D classes, each
+  table with its own replica exchange. Default: one table per D class, so each class's
+  kernel is compiled for its row width only (fewer registers than the mixed-class
+  kernel); the D = 16 and 64 tables keep one block per chain and run their iterations in
+  loop launches cut at the exchange's test iterations, one after the other on one
+  stream, beside the D = 256 and 1024 tables (one launch per iteration, a stream each).
+  Keys are those of the chains' positions in a single table ordered by problem, so every
+  chain and exchange gives what it gives there."""
   C = LADDER_PROBLEMS * LADDER_RUNGS
   temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
   keys = R.key_words(R.split(R.PRNGKey(0), C))
   dkeys = R.key_words(R.split(R.PRNGKey(2), C))
   skeys = R.key_words(R.split(R.PRNGKey(4), LADDER_PROBLEMS * (LADDER_RUNGS - 1)))
   engines, exchanges, dims = [], [], []
-  for D in LADDER_DIMS:
+  for group in groups:
+    # a group is a tuple of D classes, or (classes, part, parts): every parts-th problem
+    # of those classes from the part-th on
+    classes, part, parts = (group, 0, 1) if isinstance(group[0], int) else group
     chains, pairs, ptemps, parity, sk = [], [], [], [], []
+    seen = 0
     for k, (X, y) in enumerate(probs):
-      if X.shape[1] != D:
+      D = X.shape[1]
+      if D not in classes:
+        continue
+      seen += 1
+      if (seen - 1) % parts != part:
         continue
       for j in range(LADDER_RUNGS):
         c = k * LADDER_RUNGS + j
@@ -351,17 +364,14 @@ def ladder_tables(torch, fused, R, probs, eps, G):
           ptemps.append((temps[j], temps[j + 1]))
           parity.append(j % 2)
           sk.append(tuple(skeys[k * (LADDER_RUNGS - 1) + j]))
-    small = D <= 64
+    small = max(classes) <= 64
     eng = fused.FusedGLM(chains, "logistic", "sgld", eps, prior_scale=1.0,
                          target_bytes=(1 << 30) if small else None)
     exchanges.append(fused.ReplicaExchange(eng, pairs, ptemps, keys=sk, swap_every=G,
                                            parity=parity))
     engines.append(eng)
-    dims.append(D)
-  # three streams (the null stream takes the fourth hardware queue): the D = 16 and 64
-  # loop tables share one, one after the other, well inside an iteration block of the
-  # others
-  return fused.TableGroup(engines, stream_of=[0, 0, 1, 2]), exchanges, dims
+    dims.append(max(classes))
+  return fused.TableGroup(engines, stream_of=list(stream_of)), exchanges, dims
 
 
 def run_ladder(args):
