@@ -60,6 +60,11 @@ constexpr int BN = 128;
 constexpr int NCLS = 10;
 constexpr int NLEAF = 6;     // W1, W2, W3, b1, b2, b3
 constexpr int kStrip = 32;   // rows per partial sum of the small leaves' gradients
+constexpr int kXcds = 8;         // accelerator complex dies, each with its own L2
+#ifndef SGMC_GATHER_BATCH
+#define SGMC_GATHER_BATCH 12
+#endif
+constexpr int kGatherBatch = SGMC_GATHER_BATCH;  // float4 row loads in flight per lane
 constexpr int kFinalBlocks = 64;
 // Update epilogue tuning: parameters pairs per prefetch chunk (1, 2, 4 or 8 of a lane's
 // 16 pairs) and the waves per SIMD the update GEMMs are compiled for (register budget).
@@ -111,6 +116,7 @@ struct Step {
   int64_t size[NLEAF];     // leaf sizes
   uint32_t* key;           // [C][2]
   uint32_t* dkey;          // [C][2]
+  uint32_t* lkey;          // [C][NLEAF][2] noise keys of this update, written by the gather
   // activations: bf16; e4m3 except h2 (bf16) in fp8 mode; float32 in fp32 mode
   void* Xb;                // [C][n][D_in]
   int32_t* lab;            // [C][n]
@@ -405,7 +411,7 @@ __device__ __forceinline__ void res_store(const Upd& u, const UpdRes& r, uint32_
 // Key of leaf `leaf` of the noise tree of this update: random_tree(sub) with
 // (key, sub) = split(key) for SGLD; s1 / s2 of (key, s1, s2) = split(key, 3) for the
 // two momentum refreshes of an OBABO step.
-__device__ __forceinline__ Key leaf_key(const Step& s, int chain, int leaf) {
+__device__ __forceinline__ Key derive_leaf_key(const Step& s, int chain, int leaf) {
   const Key k = load_key(s.key + 2 * chain);
   Key a, b, c;
   if (s.mode == MLP_SGLD) {
@@ -416,10 +422,19 @@ __device__ __forceinline__ Key leaf_key(const Step& s, int chain, int leaf) {
   return split_key(s.mode == MLP_OBABO_BA ? b : c, NLEAF, (uint32_t)leaf);
 }
 
+// The same key as stored by the step's gather kernel (the chain key does not change
+// until the step's last kernel), so the update epilogues load two words instead of
+// running four to seven Threefry blocks per wave.
+__device__ __forceinline__ Key leaf_key(const Step& s, int chain, int leaf) {
+  return load_key(s.lkey + 2 * (chain * NLEAF + leaf));
+}
+
 // ------------------------------------------------------------------ gather ----
 // Xb[c][j] = bf16(X[idx_j]), lab[c][j] = labels[idx_j]; one wavefront per row.
 __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
   const int c = blockIdx.y;
+  if (s.update && blockIdx.x == 0 && threadIdx.x < NLEAF)
+    store_key(s.lkey + 2 * (c * NLEAF + threadIdx.x), derive_leaf_key(s, c, threadIdx.x));
   const int j = blockIdx.x * (GT / kWave) + threadIdx.x / kWave;
   if (j >= s.n) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -437,37 +452,46 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
     r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, randint_mult(span));
     if (r < 0 || r >= s.N) r = 0;
   }
-  const float* src = s.X + r * s.ldx;
+  const float* __restrict__ src = s.X + r * s.ldx;
   const int64_t row = (int64_t)c * s.n + j;
+  // kGatherBatch float4 loads per lane go out before their conversions and stores, so a
+  // row costs D_in / (256 kGatherBatch) memory round trips instead of D_in / 256. The
+  // loads are unconditional: offsets past the row are clamped to its last float4 (and
+  // not stored), a row past N reads row 0 and is zeroed.
+  constexpr int kStep = kWave * 4;
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto rowcopy = [&](auto&& put) {
+    for (int f0 = lane * 4; f0 < s.D_in; f0 += kGatherBatch * kStep) {
+      float4 xs[kGatherBatch];
+#pragma unroll
+      for (int u = 0; u < kGatherBatch; ++u)
+        xs[u] = *reinterpret_cast<const float4*>(src + min(f0 + u * kStep, s.D_in - 4));
+#pragma unroll
+      for (int u = 0; u < kGatherBatch; ++u)
+        if (f0 + u * kStep < s.D_in) put(f0 + u * kStep, valid ? xs[u] : zero4);
+    }
+  };
   if (s.fp8) {
     const float qs = s.q[(int64_t)c * kQWords + Q_X];
-    uint8_t* dst = reinterpret_cast<uint8_t*>(s.Xb) + row * s.D_in;
+    uint8_t* __restrict__ dst = reinterpret_cast<uint8_t*>(s.Xb) + row * s.D_in;
     float amax = 0.f;
-    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
-      float4 x = *reinterpret_cast<const float4*>(src + f);
-      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
+    rowcopy([&](int f, float4 x) {
       amax = fmaxf(amax, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
       const uint32_t v = (uint32_t)f2q(x.x * qs) | ((uint32_t)f2q(x.y * qs) << 8) |
                          ((uint32_t)f2q(x.z * qs) << 16) | ((uint32_t)f2q(x.w * qs) << 24);
       *reinterpret_cast<uint32_t*>(dst + f) = v;
-    }
+    });
     q_note_amax(s, c, Q_X, amax);
   } else if (s.fp32) {
-    float* dst = reinterpret_cast<float*>(s.Xb) + row * s.D_in;
-    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
-      float4 x = *reinterpret_cast<const float4*>(src + f);
-      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
-      *reinterpret_cast<float4*>(dst + f) = x;
-    }
+    float* __restrict__ dst = reinterpret_cast<float*>(s.Xb) + row * s.D_in;
+    rowcopy([&](int f, float4 x) { *reinterpret_cast<float4*>(dst + f) = x; });
   } else {
-    uint16_t* dst = reinterpret_cast<uint16_t*>(s.Xb) + row * s.D_in;
-    for (int f = lane * 4; f < s.D_in; f += kWave * 4) {
-      float4 x = *reinterpret_cast<const float4*>(src + f);
-      if (!valid) x = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint16_t* __restrict__ dst = reinterpret_cast<uint16_t*>(s.Xb) + row * s.D_in;
+    rowcopy([&](int f, float4 x) {
       const uint32_t lo = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
       const uint32_t hi = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
       *reinterpret_cast<uint2*>(dst + f) = make_uint2(lo, hi);
-    }
+    });
   }
   if (lane == 0) s.lab[row] = valid ? s.labels[r] : -1;
 }
@@ -714,8 +738,18 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
   __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
   __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
   __shared__ float red[GT / kWave];
-  const int bm = blockIdx.x, bn = blockIdx.y;
-  const int c = blockIdx.z / g.splits, sp = blockIdx.z % g.splits;
+  // Blocks go to the XCDs round-robin by linear id. In the GEMMs without a parameter
+  // update, XCD x takes the contiguous range [x T / 8, (x + 1) T / 8) of logical tiles
+  // (bm fastest, then bn, then chain and split), so the blocks that share an operand tile
+  // read it through one L2 instead of fetching it from HBM once per XCD (Xb W1 of a
+  // 3072-512 net: 390 -> 75 MB per launch). The update GEMMs keep the linear order, whose
+  // state stream measured faster.
+  const uint32_t T = gridDim.x * gridDim.y * gridDim.z;
+  uint32_t lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (EPI != EPI_UPDATE && T % kXcds == 0) lin = (lin % kXcds) * (T / kXcds) + lin / kXcds;
+  const int bm = (int)(lin % gridDim.x), bn = (int)((lin / gridDim.x) % gridDim.y);
+  const int bz = (int)(lin / (gridDim.x * gridDim.y));
+  const int c = bz / g.splits, sp = bz % g.splits;
   const int kspan = g.K / g.splits;
   const int kb = sp * kspan, ke = kb + kspan;
   const int lane = threadIdx.x & (kWave - 1);
@@ -958,10 +992,27 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
   float z[NCLS];
 #pragma unroll
   for (int k = 0; k < NCLS; ++k) z[k] = 0.f;
-  for (int a = lane; a < s.H2; a += kWave) {
-    const float h = h2_at(s, h2row + a);
+  // columns a = lane + 64 u, in batches whose loads go out together (four columns, then
+  // pairs: H2 is a multiple of 128); the sums keep the ascending order of a
+  auto zbatch = [&](int a0, auto nb) {
+    constexpr int NB = decltype(nb)::value;
+    float h[NB], w3[NB][NCLS];
 #pragma unroll
-    for (int k = 0; k < NCLS; ++k) z[k] += h * W3[(int64_t)a * NCLS + k];
+    for (int u = 0; u < NB; ++u) {
+      const int a = a0 + u * kWave;
+      h[u] = h2_at(s, h2row + a);
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w3[u][k] = W3[(int64_t)a * NCLS + k];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) z[k] += h[u] * w3[u][k];
+  };
+  {
+    int a0 = lane;
+    for (; a0 + 3 * kWave < s.H2; a0 += 4 * kWave) zbatch(a0, std::integral_constant<int, 4>{});
+    for (; a0 < s.H2; a0 += 2 * kWave) zbatch(a0, std::integral_constant<int, 2>{});
   }
 #pragma unroll
   for (int k = 0; k < NCLS; ++k) z[k] = wave_sum(z[k]) + b3[k];
@@ -1000,43 +1051,83 @@ __global__ void __launch_bounds__(GT) head_kernel(Step s) {
   const int64_t row = (int64_t)c * s.n + j;
   const float qs = s.fp8 ? s.q[(int64_t)c * kQWords + Q_DA2] : 1.0f;
   float amax = 0.f;
-  for (int a = lane; a < s.H2; a += kWave) {
-    float d = 0.f;
+  auto dbatch = [&](int a0, auto nb) {
+    constexpr int NB = decltype(nb)::value;
+    float w3[NB][NCLS];
+    bool on[NB];
 #pragma unroll
-    for (int k = 0; k < NCLS; ++k) d += dl[k] * W3[(int64_t)a * NCLS + k];
-    const float v = h2_on(s, h2row + a) ? d : 0.0f;
-    amax = fmaxf(amax, fabsf(v));
-    put_act(s.da2, row * s.H2 + a, v, s.fp8 != 0, qs, s.fp32 != 0);
+    for (int u = 0; u < NB; ++u) {
+      const int a = a0 + u * kWave;
+      on[u] = h2_on(s, h2row + a);
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w3[u][k] = W3[(int64_t)a * NCLS + k];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      float d = 0.f;
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) d += dl[k] * w3[u][k];
+      const float v = on[u] ? d : 0.0f;
+      amax = fmaxf(amax, fabsf(v));
+      put_act(s.da2, row * s.H2 + a0 + u * kWave, v, s.fp8 != 0, qs, s.fp32 != 0);
+    }
+  };
+  {
+    int a0 = lane;
+    for (; a0 + 3 * kWave < s.H2; a0 += 4 * kWave) dbatch(a0, std::integral_constant<int, 4>{});
+    for (; a0 < s.H2; a0 += 2 * kWave) dbatch(a0, std::integral_constant<int, 2>{});
   }
   if (s.fp8) q_note_amax(s, c, Q_DA2, amax);
 }
 
 // ------------------------------------------------------- small-leaf sums -----
+// Element i of an activation tensor stored at precision PREC (e4m3 at scale qs).
+template <int PREC>
+__device__ __forceinline__ float act_of(const void* p, int64_t i, float qs) {
+  if constexpr (PREC == P_F32) return reinterpret_cast<const float*>(p)[i];
+  else if constexpr (PREC == P_FP8) return q2f(reinterpret_cast<const uint8_t*>(p)[i]) / qs;
+  else return bf2f(reinterpret_cast<const uint16_t*>(p)[i]);
+}
+
 // Row-strip partial sums of the small leaves' gradients (flat order W3 [H2][10], b1,
 // b2, b3): for strip t, part[t][.] = sum over its kStrip rows of h2^T dL, dA1, dA2, dL.
 // Grid x: ceil(H2 / GT) blocks for W3 and b2, ceil(H1 / GT) for b1, one for b3.
+// PREC: the evaluation's GEMM precision (h2 is bf16 in e4m3 mode). A thread loads its
+// column's kStrip rows before summing them (in ascending row order), one memory round
+// trip per strip; the strip's dL rows are staged in LDS.
+template <int PREC>
 __global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
+  constexpr int H2P = PREC == P_F32 ? P_F32 : P_BF16;
+  __shared__ float dls[kStrip * NCLS];
   const int c = blockIdx.z, t = blockIdx.y;
   const int nb2 = (s.H2 + GT - 1) / GT, nb1 = (s.H1 + GT - 1) / GT;
   const int64_t nW3 = (int64_t)s.H2 * NCLS;
   const int64_t total = nW3 + s.H1 + s.H2 + NCLS;
   float* out = s.col_part + ((int64_t)c * (s.n / kStrip) + t) * total;
   const int64_t r0 = (int64_t)c * s.n + (int64_t)t * kStrip;
+  const float* dL = s.dL + r0 * NCLS;
   const int bx = blockIdx.x;
   if (bx < nb2) {
+    for (int q = threadIdx.x; q < kStrip * NCLS; q += GT) dls[q] = dL[q];
+    __syncthreads();
     const int a = bx * GT + threadIdx.x;
     if (a >= s.H2) return;
+    const float qs = PREC == P_FP8 ? s.q[(int64_t)c * kQWords + Q_DA2] : 1.0f;
+    float h[kStrip], d2[kStrip];
+#pragma unroll
+    for (int r = 0; r < kStrip; ++r) {
+      h[r] = act_of<H2P>(s.h2, (r0 + r) * s.H2 + a, 1.0f);
+      d2[r] = act_of<PREC>(s.da2, (r0 + r) * s.H2 + a, qs);
+    }
     float w3[NCLS];
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) w3[k] = 0.f;
     float b2 = 0.f;
-#pragma unroll 4
-    for (int r = 0; r < kStrip; ++r) {
-      const float h = h2_at(s, (r0 + r) * s.H2 + a);
-      b2 += get_act(s, c, Q_DA2, s.da2, (r0 + r) * s.H2 + a);
-      const float* dl = s.dL + (r0 + r) * NCLS;
 #pragma unroll
-      for (int k = 0; k < NCLS; ++k) w3[k] += h * dl[k];
+    for (int r = 0; r < kStrip; ++r) {
+      b2 += d2[r];
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w3[k] += h[r] * dls[r * NCLS + k];
     }
 #pragma unroll
     for (int k = 0; k < NCLS; ++k) out[(int64_t)a * NCLS + k] = w3[k];
@@ -1044,16 +1135,23 @@ __global__ void __launch_bounds__(GT) colsum_kernel(Step s) {
   } else if (bx < nb2 + nb1) {
     const int i = (bx - nb2) * GT + threadIdx.x;
     if (i >= s.H1) return;
+    const float qs = PREC == P_FP8 ? s.q[(int64_t)c * kQWords + Q_DA1] : 1.0f;
+    float d1[kStrip];
+#pragma unroll
+    for (int r = 0; r < kStrip; ++r) d1[r] = act_of<PREC>(s.da1, (r0 + r) * s.H1 + i, qs);
     float b1 = 0.f;
-#pragma unroll 8
-    for (int r = 0; r < kStrip; ++r) b1 += get_act(s, c, Q_DA1, s.da1, (r0 + r) * s.H1 + i);
+#pragma unroll
+    for (int r = 0; r < kStrip; ++r) b1 += d1[r];
     out[nW3 + i] = b1;
   } else {
     const int k = threadIdx.x;
     if (k >= NCLS) return;
+    float dk[kStrip];
+#pragma unroll
+    for (int r = 0; r < kStrip; ++r) dk[r] = dL[r * NCLS + k];
     float b3 = 0.f;
-#pragma unroll 8
-    for (int r = 0; r < kStrip; ++r) b3 += s.dL[(r0 + r) * NCLS + k];
+#pragma unroll
+    for (int r = 0; r < kStrip; ++r) b3 += dk[r];
     out[nW3 + s.H1 + s.H2 + k] = b3;
   }
 }
@@ -1388,7 +1486,7 @@ using namespace sgmc::mlp;
 namespace {
 
 struct Layout {
-  int64_t xb, lab, h1, h2, da2, da1, dl, lik, prior, ke, facc, col, fin, split, total;
+  int64_t xb, lab, h1, h2, da2, da1, dl, lik, lkey, prior, ke, facc, col, fin, split, total;
   int nparts, g4_blocks, g5_blocks, small_blocks;
 };
 
@@ -1406,6 +1504,7 @@ Layout layout_of(int C, int n, int D_in, int H1, int H2) {
   L.da1 = o; o = align256(o + (int64_t)C * n * H1 * 4);
   L.dl = o; o = align256(o + (int64_t)C * n * NCLS * 4);
   L.lik = o; o = align256(o + (int64_t)C * n * 4);
+  L.lkey = o; o = align256(o + (int64_t)C * NLEAF * 2 * 4);
   L.g4_blocks = (H1 / BM) * (H2 / BN);
   L.g5_blocks = (D_in / BM) * (H1 / BN);
   const int64_t small_total = (int64_t)H2 * NCLS + H1 + H2 + NCLS;
@@ -1614,6 +1713,7 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
   s.da1 = w + L.da1;
   s.dL = reinterpret_cast<float*>(w + L.dl);
   s.lik = reinterpret_cast<float*>(w + L.lik);
+  s.lkey = reinterpret_cast<uint32_t*>(w + L.lkey);
   s.prior_part = reinterpret_cast<float*>(w + L.prior);
   s.ke_part = reinterpret_cast<float*>(w + L.ke);
   s.col_part = reinterpret_cast<float*>(w + L.col);
@@ -1721,9 +1821,10 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
     launch_gemm<false, false, EPI_GRAD>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_, SLOT_G4);
     launch_gemm<false, false, EPI_GRAD>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_, SLOT_G5);
   }
-  hipLaunchKernelGGL(colsum_kernel,
-                     dim3((q.H2 + GT - 1) / GT + (q.H1 + GT - 1) / GT + 1, n / kStrip, C),
-                     dim3(GT), 0, st_, s);
+  const dim3 col_grid((q.H2 + GT - 1) / GT + (q.H1 + GT - 1) / GT + 1, n / kStrip, C);
+  if (s.fp32) hipLaunchKernelGGL(colsum_kernel<P_F32>, col_grid, dim3(GT), 0, st_, s);
+  else if (s.fp8) hipLaunchKernelGGL(colsum_kernel<P_FP8>, col_grid, dim3(GT), 0, st_, s);
+  else hipLaunchKernelGGL(colsum_kernel<P_BF16>, col_grid, dim3(GT), 0, st_, s);
   hipLaunchKernelGGL(small_kernel, dim3(L.small_blocks, C), dim3(GT), 0, st_, s,
                      L.g4_blocks + L.g5_blocks);
   if (update && s.mode == MLP_SGLD && q.samples && q.save_slot) {
