@@ -11,7 +11,8 @@ constexpr int kNT = 256;
 __global__ void __launch_bounds__(kNT) draw_batch_kernel(const uint32_t* __restrict__ dkey_in,
                                                          uint32_t* __restrict__ dkey_out,
                                                          int32_t* __restrict__ idx, uint32_t n,
-                                                         uint32_t span, uint32_t mult) {
+                                                         uint32_t span, uint32_t mult,
+                                                         uint32_t magic) {
   Key next, sub, k1, k2;
   split2(load_key(dkey_in), next, sub);
   split2(sub, k1, k2);
@@ -21,8 +22,8 @@ __global__ void __launch_bounds__(kNT) draw_batch_kernel(const uint32_t* __restr
     uint32_t h0, h1, l0, l1;
     bits_pair(k1, n, j, h0, h1);
     bits_pair(k2, n, j, l0, l1);
-    idx[j] = (int32_t)randint_offset(h0, l0, span, mult);
-    if (j + m < n) idx[j + m] = (int32_t)randint_offset(h1, l1, span, mult);
+    idx[j] = (int32_t)randint_offset(h0, l0, span, mult, magic);
+    if (j + m < n) idx[j + m] = (int32_t)randint_offset(h1, l1, span, mult, magic);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) store_key(dkey_out, next);
 }
@@ -74,7 +75,7 @@ int sgmc_draw_batch(const uint32_t* dkey_in, uint32_t* dkey_out, int32_t* idx, i
   const uint32_t m = ((uint32_t)n + 1) >> 1;
   hipLaunchKernelGGL(draw_batch_kernel, dim3((m + kNT - 1) / kNT), dim3(kNT), 0,
                      as_stream(stream), dkey_in, dkey_out, idx, (uint32_t)n, span,
-                     randint_mult(span));
+                     randint_mult(span), randint_magic(span));
   return check_launch("sgmc_draw_batch");
 }
 

@@ -32,6 +32,7 @@ constexpr int kNT = 256;
 constexpr int kWaves = kNT / kWave;
 constexpr int kMaxD = 1024;  // fused path: features per row
 constexpr int kLdsRows = 4096;  // per-row likelihoods staged in LDS per block
+constexpr int kResWindow = 512;  // resident loop: step sizes / save slots staged per window
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -62,12 +63,12 @@ struct RowSource {
   const int32_t* idx;  // explicit indices, or null
   int draw;            // nonzero: in-kernel randint with (k1, k2)
   Key k1, k2;
-  uint32_t n, span, mult;
+  uint32_t n, span, mult, magic;
   int64_t rows;        // rows addressable through X (bounds guard)
   // Row index of minibatch row j, or -1 if it falls outside [0, rows).
   __device__ __forceinline__ int64_t row(int j) const {
     int64_t r;
-    if (draw) r = (int64_t)draw_index(k1, k2, n, (uint32_t)j, span, mult);
+    if (draw) r = (int64_t)draw_index(k1, k2, n, (uint32_t)j, span, mult, magic);
     else if (idx) r = (int64_t)idx[j];
     else r = (int64_t)j;
     return (r >= 0 && r < rows) ? r : -1;
@@ -312,11 +313,10 @@ __device__ __forceinline__ void accumulate_rows(gcfloat* __restrict__ X, int64_t
 }
 
 // Reduces acc over the segments of each wavefront and over the block's wavefronts;
-// writes the block's column sums to out[0..4*D4) and returns the block's lik sum.
-template <int LPR, int CPL>
-__device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float lik_acc, int D4,
-                                                    gfloat* __restrict__ out, float* lds_cols,
-                                                    float* lds_small) {
+// hands column sum c (0 <= c < 4 D4) to put(c, s) and returns the block's lik sum.
+template <int LPR, int CPL, bool LDS_SYNC = false, class Put>
+__device__ __forceinline__ float block_reduce(float4 (&acc)[CPL], float lik_acc, int D4,
+                                              float* lds_cols, float* lds_small, Put put) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const int q = lane % LPR;
@@ -330,7 +330,8 @@ __device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float li
       acc[k].w += __shfl_xor(acc[k].w, m, kWave);
     }
   }
-  __syncthreads();
+  if (LDS_SYNC) lds_sync();
+  else __syncthreads();
   if (lane < LPR) {
 #pragma unroll
     for (int k = 0; k < CPL; ++k) {
@@ -338,14 +339,23 @@ __device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float li
       if (c4 < D4) reinterpret_cast<float4*>(lds_cols + w * 4 * D4)[c4] = acc[k];
     }
   }
-  const float lsum = block_sum<kNT>(lik_acc, lds_small);  // contains __syncthreads
+  const float lsum = block_sum<kNT, LDS_SYNC>(lik_acc, lds_small);  // contains barriers
   for (int c = threadIdx.x; c < 4 * D4; c += kNT) {
     float s = 0.0f;
 #pragma unroll
     for (int ww = 0; ww < kWaves; ++ww) s += lds_cols[ww * 4 * D4 + c];
-    publish(out + c, s);
+    put(c, s);
   }
   return lsum;
+}
+
+// block_reduce with the column sums published to out[0..4*D4) for the chain's last block.
+template <int LPR, int CPL>
+__device__ __forceinline__ float block_reduce_write(float4 (&acc)[CPL], float lik_acc, int D4,
+                                                    gfloat* __restrict__ out, float* lds_cols,
+                                                    float* lds_small) {
+  return block_reduce<LPR, CPL>(acc, lik_acc, D4, lds_cols, lds_small,
+                                [&](int c, float s) { publish(out + c, s); });
 }
 
 // Splits rows [0, n) of a minibatch among the blocks of a chain and then among the
@@ -409,7 +419,7 @@ __global__ void __launch_bounds__(kNT) glm_potential_kernel(PotArgs a) {
   float lik_acc = 0.0f;
   int r0, r1, b0, b1;
   wave_rows(a.n, gridDim.x, blockIdx.x, r0, r1, b0, b1);
-  RowSource src{a.idx, 0, Key{0, 0}, Key{0, 0}, 0, 0, 0, a.idx ? a.N : (int64_t)a.n};
+  RowSource src{a.idx, 0, Key{0, 0}, Key{0, 0}, 0, 0, 0, 0, a.idx ? a.N : (int64_t)a.n};
   accumulate_rows<LPR, CPL>(to_global(a.X), a.ldx, to_global(a.y), a.D4, src, r0, r1, th, a.L,
                             acc, lik_acc, lds_lik, b0);
   // Block record: column sums, the block's lik sum and its lik sum of squared deviations
@@ -662,6 +672,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   src.n = (uint32_t)ch.n;
   src.span = (uint32_t)ch.N;
   src.mult = randint_mult(src.span);
+  src.magic = randint_magic(src.span);
   src.rows = ch.N;
 
   float4 acc[CPL];
@@ -900,6 +911,185 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   }
 }
 
+// ---- resident loop: one single-block SGLD chain, many iterations on chip ----
+// For chains of D <= 4 LPR (one float4 chunk per lane of a row segment) and minibatches
+// of at most 256 rows (one 64-row chunk per wavefront), the chain's state stays on chip
+// for the whole launch: theta in LDS (and its owner threads' registers), the keys in
+// registers, the column sums in LDS. The minibatch of the next evaluation (its indices
+// depend only on the data key) is drawn and its rows loaded right after this
+// evaluation's row groups are consumed, so their latency overlaps the reduction and the
+// update. The arithmetic is fused_body's for a chain of one block, in the same order
+// (row groups, reductions, the SGLD / rms_prop update, U, var), so both give the same
+// bits.
+template <int LPR, int MODE>
+__device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm_chain& ch,
+                                              int iterations, float* lds_cols, float* lds_small,
+                                              float* lds_lik, float* th_lds, float* col_lds,
+                                              float* eps_lds, int* slot_lds) {
+  constexpr int CPL = 1;
+  using RG = RowGroup<LPR, CPL>;
+  constexpr int G = (kWave / LPR) * RG::U;  // rows per group
+  constexpr int NG = kWave / G;             // groups of a 64-row chunk
+  const int mode = MODE >= 0 ? MODE : a.mode;
+  const bool rms = mode == SGMC_MODE_SGLD_RMS;
+  const int D = ch.D, D4 = (D + 3) >> 2, n = ch.n;
+  gfloat* g_theta = to_global(ch.theta);
+  gfloat* g_v = to_global(ch.v);
+  gfloat* g_samples = to_global(ch.samples);
+  const int it0 = a.iter[0];
+  Key key = load_key(ch.key), dkey = load_key(ch.dkey);
+  // Thread jp < m owns the noise pair (jp, jp + m), as in fused_body's update loop.
+  const uint32_t S = (uint32_t)D, m = (S + 1) >> 1;
+  const uint32_t jp = threadIdx.x;
+  const bool own = jp < m;
+  const bool two = own && jp + m < S;
+  const uint32_t jj[2] = {jp, two ? jp + m : jp};
+  float w[2] = {0.0f, 0.0f}, v[2] = {0.0f, 0.0f};
+  if (own) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      w[h] = g_theta[jj[h]];
+      if (rms) v[h] = g_v[jj[h]];
+    }
+  }
+  for (int c = threadIdx.x; c < 4 * LPR; c += kNT) th_lds[c] = c < D ? g_theta[c] : 0.0f;
+  int r0, r1, b0, b1;
+  wave_rows(n, 1, 0, r0, r1, b0, b1);
+  const int ng = r0 < r1 ? (r1 - r0 + G - 1) / G : 0;  // wavefront-uniform
+  RowSource src;
+  src.idx = nullptr;
+  src.draw = 1;
+  src.n = (uint32_t)n;
+  src.span = (uint32_t)ch.N;
+  src.mult = randint_mult(src.span);
+  src.magic = randint_magic(src.span);
+  src.rows = ch.N;
+  Key dnext;
+  RG buf[NG];
+  // (dkey', sub) = split(dkey); k1, k2 = split(sub); the wavefront's rows into buf
+  auto draw_rows = [&]() {
+    Key dsub;
+    split2_uniform(dkey, dnext, dsub);
+    split2_uniform(dsub, src.k1, src.k2);
+    const int rid = r0 < r1 ? chunk_rows(src, r0, r1) : -1;
+#pragma unroll
+    for (int gi = 0; gi < NG; ++gi)
+      if (gi < ng) load_group<LPR, CPL>(to_global(ch.X), ch.ldx, to_global(ch.y), D4, gi * G, rid,
+                                        buf[gi]);
+  };
+  draw_rows();
+  const float c = (float)(-ch.N) / (float)n;
+  const float Tn = ch.temperature;
+  float Uk = 0.0f;
+  // Barriers inside the loop wait for LDS only (lds_sync): the next minibatch's row
+  // loads stay in flight through the reduction and the update. The step sizes and save
+  // slots come from LDS (a window of kResWindow iterations), so no global load has to be
+  // waited on behind those row loads.
+  for (int t = 0; t < iterations; ++t) {
+    const int tw = t % kResWindow;
+    if (tw == 0) {
+      __syncthreads();
+      for (int u = threadIdx.x; u < kResWindow; u += kNT) {
+        const int iu = max(0, min(it0 + t + u, a.table_len - 1));
+        eps_lds[u] = a.eps_table[iu];
+        slot_lds[u] = a.save_slot ? a.save_slot[iu] : -1;
+      }
+      __syncthreads();
+    }
+    const float eps = eps_lds[tw];
+    const int slot = slot_lds[tw];
+    float4 th[CPL];
+    {
+      const int q = (threadIdx.x & (kWave - 1)) % LPR;
+      th[0] = make_float4(th_lds[4 * q], th_lds[4 * q + 1], th_lds[4 * q + 2], th_lds[4 * q + 3]);
+    }
+    float4 acc[CPL];
+    acc[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float lik_acc = 0.0f;
+#pragma unroll
+    for (int gi = 0; gi < NG; ++gi)
+      if (gi < ng) process_group<LPR, CPL>(buf[gi], r0 + gi * G, D4, th, a.L, acc, lik_acc, lds_lik, 0);
+    dkey = dnext;
+    if (t + 1 < iterations) draw_rows();
+    const float ls = block_reduce<LPR, CPL, true>(acc, lik_acc, D4, lds_cols, lds_small,
+                                                  [&](int cc, float sv) { col_lds[cc] = sv; });
+    lds_sync();
+    Key knext, ksub;
+    split2_uniform(key, knext, ksub);
+    const Key leaf = split_key(ksub, 1, 0);
+    float lp = 0.0f;
+    if (own) {
+      uint32_t wb0, wb1;
+      bits_pair(leaf, S, jp, wb0, wb1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h && !two) break;
+        const uint32_t j = jj[h];
+        const float xi = bits_to_normal(h ? wb1 : wb0);
+        const float wj = w[h];
+        // one published record: the last block's sum over records starts from +0
+        float g = c * (0.0f + col_lds[j]);
+        if (a.pr.has_prior) {
+          g = g + wj * a.pr.inv_var;
+          lp += (a.pr.log_norm + (wj * wj) * a.pr.inv_var) / -2.0f;
+        }
+        g = g / a.pot_temperature;
+        const float sg = (-eps) * g;
+        const float sn = sqrtf((2.0f * Tn) * eps) * xi;
+        float wn;
+        if (rms) {
+          const float vn = a.alpha * v[h] + (1.0f - a.alpha) * (g * g);
+          v[h] = vn;
+          const float Gs = 1.0f / (a.lmbd + sqrtf(vn));
+          wn = wj + ((0.0f + Gs * sg) + sqrtf(Gs) * sn);
+        } else {
+          wn = wj + (sg + sn);
+        }
+        w[h] = wn;
+        th_lds[j] = wn;
+        if (slot >= 0 && ch.samples) g_samples[(int64_t)slot * D + j] = wn;
+      }
+    }
+    key = knext;
+    lp = block_sum<kNT, true>(lp, lds_small);
+    // the block's lik sum is the only record: the sum over records is ls + 0
+    const float mean = (ls + 0.0f) / (float)n;
+    Uk = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
+    if (threadIdx.x == 0 && slot >= 0 && ch.samples_U) ch.samples_U[slot] = Uk;
+    if (ch.var && t == iterations - 1) {
+      // var(lik) of the last evaluation, as fused_body forms it from one record
+      const float mb = n > 0 ? ls / (float)n : 0.0f;
+      float s2 = 0.0f;
+      for (int jr = threadIdx.x; jr < n; jr += kNT) {
+        const float d = lds_lik[jr] - mb;
+        s2 += d * d;
+      }
+      const float m2 = block_sum<kNT, true>(s2, lds_small);
+      float tv = 0.0f;
+      if (threadIdx.x == 0 && n > 0) {
+        const float d = ls / (float)n - mean;
+        tv += m2 + (float)n * (d * d);
+      }
+      const float var_sum = block_sum<kNT, true>(tv, lds_small);
+      if (threadIdx.x == 0) ch.var[0] = var_sum / (float)n;
+    }
+    lds_sync();
+  }
+  if (own) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h && !two) break;
+      g_theta[jj[h]] = w[h];
+      if (rms) g_v[jj[h]] = v[h];
+    }
+  }
+  if (threadIdx.x == 0) {
+    ch.U[0] = Uk;
+    store_key(ch.key, key);
+    store_key(ch.dkey, dkey);
+  }
+}
+
 // Many iterations in one launch for tables whose chains have one block each: block c
 // runs chain c's iterations back to back (every leapfrog step of every iteration),
 // reading its parameters and keys back from memory after a barrier, so no launch and
@@ -910,6 +1100,10 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   __shared__ __attribute__((aligned(16))) float lds_cols[kWaves * kMaxD];
   __shared__ float lds_small[kWaves];
   __shared__ float lds_lik[kLdsRows];
+  __shared__ float res_th[64];
+  __shared__ float res_col[64];
+  __shared__ float res_eps[kResWindow];
+  __shared__ int res_slot[kResWindow];
   __shared__ int flag;
   // The grid has one block per chain and every chain one block (sgmc_glm_check_table).
   const int chain = blockIdx.x;
@@ -919,7 +1113,18 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   b.advance = -1;
   const int mode = MODE >= 0 ? MODE : a.mode;
   const int launches = mode == SGMC_MODE_OBABO ? 2 * a.steps : a.steps;
-  for (int t = 0; t < iterations; ++t) {
+  // SGLD chains of D <= 64 and minibatches of at most 256 rows without per-row
+  // likelihood output run resident (block-uniform choice)
+  bool resident = false;
+  if constexpr (DCLASS == 0 || DCLASS == 1)
+    resident = (mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS) && !ch.lik &&
+               ch.n <= kWaves * kWave;
+  if (resident) {
+    if constexpr (DCLASS == 0 || DCLASS == 1)
+      resident_loop<DCLASS == 0 ? 4 : 16, MODE>(a, ch, iterations, lds_cols, lds_small, lds_lik,
+                                                res_th, res_col, res_eps, res_slot);
+  }
+  for (int t = 0; t < (resident ? 0 : iterations); ++t) {
     b.iter_offset = t;
     for (int st = 0; st < launches; ++st) {
       b.step = st;

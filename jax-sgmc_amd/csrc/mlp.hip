@@ -117,6 +117,7 @@ struct Step {
   uint32_t* key;           // [C][2]
   uint32_t* dkey;          // [C][2]
   uint32_t* lkey;          // [C][NLEAF][2] noise keys of this update, written by the gather
+  uint32_t rmult, rmagic;  // randint_mult(N), randint_magic(N) of the index draw
   // activations: bf16; e4m3 except h2 (bf16) in fp8 mode; float32 in fp32 mode
   void* Xb;                // [C][n][D_in]
   int32_t* lab;            // [C][n]
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(GT) gather_kernel(Step s) {
     split2(load_key(s.dkey + 2 * c), next, sub);
     split2(sub, k1, k2);
     const uint32_t span = (uint32_t)s.N;
-    r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, randint_mult(span));
+    r = (int64_t)draw_index(k1, k2, (uint32_t)s.n, (uint32_t)j, span, s.rmult, s.rmagic);
     if (r < 0 || r >= s.N) r = 0;
   }
   const float* __restrict__ src = s.X + r * s.ldx;
@@ -1705,6 +1706,8 @@ int make_step(const sgmc_mlp_state* st, int update, Layout& L, Step& s) {
     o += sizes[l];
   }
   s.key = q.key; s.dkey = q.dkey;
+  s.rmult = randint_mult((uint32_t)q.N);
+  s.rmagic = randint_magic((uint32_t)q.N);
   s.Xb = w + L.xb;
   s.lab = reinterpret_cast<int32_t*>(w + L.lab);
   s.h1 = w + L.h1;

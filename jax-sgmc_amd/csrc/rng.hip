@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(kNT) draw_kernel(const uint32_t* __restrict__ 
 
 __global__ void __launch_bounds__(kNT) randint_kernel(const uint32_t* __restrict__ key,
                                                       int32_t* __restrict__ out, uint32_t S,
-                                                      int32_t minval, uint32_t span, uint32_t mult) {
+                                                      int32_t minval, uint32_t span, uint32_t mult,
+                                                      uint32_t magic) {
   const uint32_t m = (S + 1) >> 1;
   const uint32_t j = blockIdx.x * kNT + threadIdx.x;
   if (j >= m) return;
@@ -63,8 +64,8 @@ __global__ void __launch_bounds__(kNT) randint_kernel(const uint32_t* __restrict
   uint32_t h0, h1, l0, l1;
   bits_pair(k1, S, j, h0, h1);
   bits_pair(k2, S, j, l0, l1);
-  out[j] = minval + (int32_t)randint_offset(h0, l0, span, mult);
-  if (j + m < S) out[j + m] = minval + (int32_t)randint_offset(h1, l1, span, mult);
+  out[j] = minval + (int32_t)randint_offset(h0, l0, span, mult, magic);
+  if (j + m < S) out[j + m] = minval + (int32_t)randint_offset(h1, l1, span, mult, magic);
 }
 
 // random_tree: keys = split(key, L); leaf l gets normal(keys[l], size_l).
@@ -185,7 +186,7 @@ int sgmc_randint(const uint32_t* key, int32_t* out, int64_t n, int32_t minval, i
   const uint32_t S = (uint32_t)n;
   const uint32_t m = (S + 1) >> 1;
   hipLaunchKernelGGL(randint_kernel, dim3((m + kNT - 1) / kNT), dim3(kNT), 0, as_stream(stream),
-                     key, out, S, minval, span, randint_mult(span));
+                     key, out, S, minval, span, randint_mult(span), randint_magic(span));
   return check_launch("sgmc_randint");
 }
 

@@ -96,6 +96,17 @@ __device__ __forceinline__ void split2(Key k, Key& a, Key& b) {
   b = Key{q0, q1};
 }
 
+// split2 for a key that is the same on every lane: lanes 0 and 1 each run one of the two
+// Threefry blocks and the words are broadcast, so the split costs one block of latency.
+// Every lane of the wavefront must be active.
+__device__ __forceinline__ void split2_uniform(Key k, Key& a, Key& b) {
+  const uint32_t l = __lane_id() & 1u;
+  uint32_t p = l, q = l + 2u;
+  threefry2x32(k.k0, k.k1, p, q);
+  a = Key{(uint32_t)__builtin_amdgcn_readlane((int)p, 0), (uint32_t)__builtin_amdgcn_readlane((int)p, 1)};
+  b = Key{(uint32_t)__builtin_amdgcn_readlane((int)q, 0), (uint32_t)__builtin_amdgcn_readlane((int)q, 1)};
+}
+
 // split(key, 3) with three Threefry blocks.
 __device__ __forceinline__ void split3(Key k, Key& a, Key& b, Key& c) {
   uint32_t a0 = 0, b0 = 3, a1 = 1, b1 = 4, a2 = 2, b2 = 5;
@@ -284,17 +295,31 @@ __host__ __device__ __forceinline__ uint32_t randint_mult(uint32_t span) {
   return (uint32_t)(m * m) % span;
 }
 
-__device__ __forceinline__ uint32_t randint_offset(uint32_t hi, uint32_t lo, uint32_t span,
-                                                   uint32_t mult) {
-  return ((hi % span) * mult + (lo % span)) % span;
+// a % d without a division: magic = floor(2^32 / d) for d >= 2 (0 for d = 1). Since
+// magic > 2^32 / d - 1, q' = umulhi(a, magic) is q or q - 1 (q = floor(a / d)), so
+// r' = a - q' d lies in [0, 2d) (and r' <= a < 2^32): one conditional subtraction of d
+// gives a % d.
+__host__ __device__ __forceinline__ uint32_t randint_magic(uint32_t span) {
+  return span >= 2u ? (uint32_t)(0x100000000ull / span) : 0u;
+}
+__device__ __forceinline__ uint32_t mod_magic(uint32_t a, uint32_t d, uint32_t magic) {
+  if (d < 2u) return 0u;
+  const uint32_t r = a - __umulhi(a, magic) * d;
+  return r >= d ? r - d : r;
 }
 
-// Index j (0 <= j < n) of randint(sub, (n,), 0, N) given k1, k2 = split(sub).
+__device__ __forceinline__ uint32_t randint_offset(uint32_t hi, uint32_t lo, uint32_t span,
+                                                   uint32_t mult, uint32_t magic) {
+  return mod_magic(mod_magic(hi, span, magic) * mult + mod_magic(lo, span, magic), span, magic);
+}
+
+// Index j (0 <= j < n) of randint(sub, (n,), 0, N) given k1, k2 = split(sub); mult and
+// magic: randint_mult(N), randint_magic(N).
 __device__ __forceinline__ int32_t draw_index(Key k1, Key k2, uint32_t n, uint32_t j,
-                                              uint32_t span, uint32_t mult) {
+                                              uint32_t span, uint32_t mult, uint32_t magic) {
   const uint32_t hi = bits_word(k1, n, j);
   const uint32_t lo = bits_word(k2, n, j);
-  return (int32_t)randint_offset(hi, lo, span, mult);
+  return (int32_t)randint_offset(hi, lo, span, mult, magic);
 }
 
 __device__ __forceinline__ Key load_key(const uint32_t* p) { return Key{p[0], p[1]}; }
@@ -318,13 +343,24 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 }
 
 // Sum over a block of NT threads (NT a multiple of 64); every thread gets the total.
-template <int NT>
+// Workgroup barrier for LDS hand-offs only: waits for this wavefront's LDS (and scalar)
+// accesses, not for its global loads, which stay in flight across the barrier (their
+// registers are waited on where they are first used).
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// LDS_SYNC: barriers that leave global loads in flight (lds_sync) instead of
+// __syncthreads; for blocks that share only LDS between the two barriers.
+template <int NT, bool LDS_SYNC = false>
 __device__ __forceinline__ float block_sum(float v, float* lds /* NT/64 floats */) {
   v = wave_sum(v);
   const int w = threadIdx.x / kWave;
-  __syncthreads();
+  if (LDS_SYNC) lds_sync();
+  else __syncthreads();
   if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = v;
-  __syncthreads();
+  if (LDS_SYNC) lds_sync();
+  else __syncthreads();
   float t = 0.0f;
 #pragma unroll
   for (int i = 0; i < NT / kWave; ++i) t += lds[i];

@@ -346,19 +346,37 @@ def test_sparse_exchange_graph_recaptured_after_eager_iterations():
   _same_ladder(a, exa, b, exb)
 
 
-@pytest.mark.parametrize("mode", ["sgld", "sgld_rms", "sghmc"])
-def test_loop_launch_equals_per_evaluation_launches(mode):
+_LOOP_SHAPES = {
+    # mixed D classes: the generic loop body
+    "mixed": [(500, 8, 32), (800, 33, 64), (300, 100, 17)],
+    # D <= 16 and D <= 64 tables: SGLD chains run resident (minibatches up to 256 rows;
+    # the 300-row chain of the second table takes the generic body in the same launch)
+    "d16": [(500, 8, 32), (700, 16, 256), (300, 3, 17)],
+    "d64": [(800, 33, 64), (600, 64, 200), (900, 20, 300)],
+}
+
+
+@pytest.mark.parametrize("mode,shapes,family", [
+    ("sgld", "mixed", "logistic"), ("sgld_rms", "mixed", "logistic"),
+    ("sghmc", "mixed", "logistic"), ("sgld", "d16", "logistic"),
+    ("sgld_rms", "d16", "linear_gaussian"), ("sgld", "d64", "linear_gaussian"),
+    ("sgld_rms", "d64", "logistic"), ("sghmc", "d64", "logistic")])
+def test_loop_launch_equals_per_evaluation_launches(mode, shapes, family):
   """Single-block chains: many iterations in one loop launch give bit for bit the
-  chains of one launch per evaluation (saved samples, keys, iteration counter)."""
-  print("loop launch", mode, flush=True)
+  chains of one launch per evaluation (positions, rms_prop state, saved samples and
+  potentials, U, var(lik), keys, iteration counter)."""
+  print("loop launch", mode, shapes, family, flush=True)
   rng = np.random.default_rng(12)
   tables = []
   for _ in range(2):
     chains = []
-    for c, (N, D, n) in enumerate([(500, 8, 32), (800, 33, 64), (300, 100, 17)]):
+    for c, (N, D, n) in enumerate(_LOOP_SHAPES[shapes]):
       X = (rng.standard_normal((N, D)) / np.sqrt(D)).astype(np.float32) if not tables else \
           tables[0][c][0]
-      y = (rng.random(N) < 0.5).astype(np.float32) if not tables else tables[0][c][1]
+      if family == "logistic":
+        y = (rng.random(N) < 0.5).astype(np.float32) if not tables else tables[0][c][1]
+      else:
+        y = rng.standard_normal(N).astype(np.float32) if not tables else tables[0][c][1]
       chains.append((X, y, D, n))
     tables.append(chains)
   its = 23
@@ -372,9 +390,10 @@ def test_loop_launch_equals_per_evaluation_launches(mode):
       spec.append(dict(X=torch.as_tensor(Xp, device=DEV), y=torch.as_tensor(y, device=DEV), D=D,
                        n=n,
                        theta0=np.zeros(D, np.float32), key=(1, c), dkey=(2, c)))
-    eng = fused.FusedGLM(spec, "logistic", mode, eps, steps=3, friction=0.4,
+    kw = dict(sigma=0.7, prior_scale=2.0) if family == "linear_gaussian" else {}
+    eng = fused.FusedGLM(spec, family, mode, eps, steps=3, friction=0.4,
                       save_slots=np.arange(its, dtype=np.int32), n_slots=its,
-                      target_bytes=1 << 30)
+                      target_bytes=1 << 30, **kw)
     assert eng.loop_capable()
     engines.append(eng)
   engines[0].run(its, loop=False)
@@ -382,6 +401,11 @@ def test_loop_launch_equals_per_evaluation_launches(mode):
   torch.cuda.synchronize()
   a, b = engines
   assert a.iteration_count() == b.iteration_count() == its
+  assert torch.isfinite(a.theta_flat).all()
+  assert torch.equal(a.theta_flat, b.theta_flat)
+  assert torch.equal(a.U, b.U) and torch.equal(a.var, b.var)
+  if a.v_flat is not None:
+    assert torch.equal(a.v_flat, b.v_flat)
   for c in range(3):
     np.testing.assert_array_equal(a.sample_trace(c).cpu().numpy(), b.sample_trace(c).cpu().numpy())
     np.testing.assert_array_equal(a.sample_potentials(c).cpu().numpy(),
