@@ -575,7 +575,14 @@ def tune_tiles(eng, args):
   """Times the GEMM tilings on the live problem (FusedMLP.autotune) unless disabled;
   returns the plan and its provenance for the bench line's config."""
   names = ("G1 Xb.W1", "G2 h1.W2", "G3 dA2.W2^T", "G4 h1^T.dA2", "G5 Xb^T.dA1")
-  if args.no_autotune:
+  if args.tiles:
+    # a fixed plan: "k x s" per GEMM, e.g. 128x2,128x1,32x1,32x1,128x1
+    pairs = [t.split("x") for t in args.tiles.split(",")]
+    if len(pairs) != 5:
+      raise SystemExit("--tiles needs five k-tile x split entries (G1..G5)")
+    eng.tune = [int(k) * 16 + int(sp) for k, sp in pairs]
+    plan, how = eng.gemm_plan(), {"source": "fixed (--tiles " + args.tiles + ")"}
+  elif args.no_autotune:
     plan, how = eng.gemm_plan(), {"source": "built-in"}
   else:
     log("autotune: GEMM k-tile depth and split-K on the live problem")
@@ -1178,6 +1185,9 @@ def main():
   ap.add_argument("--warmup", type=int, default=50)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--tiles", default=None,
+                  help="MLP workloads: a fixed GEMM plan instead of the tuner, five 'k x s' "
+                       "entries for G1..G5 (e.g. 128x2,128x1,32x1,32x1,128x1)")
   ap.add_argument("--no-autotune", action="store_true",
                   help="MLP workloads: keep the built-in GEMM tiling instead of timing the "
                        "candidates on the live problem before the timed region")
