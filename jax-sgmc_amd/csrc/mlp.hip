@@ -79,6 +79,11 @@ constexpr int kFinalBlocks = 64;
 #endif
 constexpr int kUpdChunk = SGMC_UPD_CHUNK;
 static_assert(16 % kUpdChunk == 0, "chunks of the 16 pairs");
+// The compiled-in rms_prop update at deep k-tiles (two waves per SIMD either way) has the
+// registers for chunks of four pairs (pSGLD 3072-512-512 step 294.9 -> 292.7 us; the
+// run-time rule of the OBABO updates loses a wave with them)
+constexpr int kUpdChunkDeep = 4;
+static_assert(16 % kUpdChunkDeep == 0, "chunks of the 16 pairs");
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -845,7 +850,7 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
       // Chunks of kUpdChunk pairs, software-pipelined: the next chunk's loads go out
       // before this chunk's noise arithmetic and stores (each parameter is read and
       // written by this thread only), so its memory time overlaps the arithmetic.
-      constexpr int CJ = kUpdChunk;
+      constexpr int CJ = (BK >= 64 && UM == UM_RMS) ? kUpdChunkDeep : kUpdChunk;
       float w0[CJ], a0[CJ], w1[CJ], a1[CJ];
       constexpr int SH = PREC == P_BF16 ? SH_BF16 : PREC == P_FP8 ? SH_E4M3 : SH_NONE;
       // compiled-in rms_prop: buffer-resource accesses, lane offset o0 plus the uniform
