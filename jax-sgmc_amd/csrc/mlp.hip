@@ -307,15 +307,25 @@ __device__ __forceinline__ void upd_st(T v, T* p) {
   else *p = v;
 }
 
-// UM: the update rule compiled in -- UM_RMS: SGLD with rms_prop (state v in aux); or
+// UM: the update rule compiled in -- UM_RMS: SGLD with rms_prop (state v in aux);
+// UM_OBA / UM_OBO: the O B A and B O halves of an OBABO step (momentum in aux); or
 // UM_ANY: the rule of u.mode / u.rms at run time.
 constexpr int UM_ANY = 0;
 constexpr int UM_RMS = 1;
+constexpr int UM_OBA = 2;
+constexpr int UM_OBO = 3;
+
+// the rule's mode, and whether it carries state in aux / writes theta
+template <int UM>
+__device__ __forceinline__ int um_mode(const Upd& u) {
+  return UM == UM_RMS ? MLP_SGLD : UM == UM_OBA ? MLP_OBABO_BA : UM == UM_OBO ? MLP_OBABO_BO
+                                                                               : u.mode;
+}
 
 template <int UM = UM_ANY>
 __device__ __forceinline__ void upd_load(const Upd& u, uint32_t i, float& w, float& aux) {
   w = upd_ld(u.th + i);
-  aux = (UM == UM_RMS || u.aux) ? upd_ld(u.aux + i) : 0.0f;
+  aux = (UM != UM_ANY || u.aux) ? upd_ld(u.aux + i) : 0.0f;
 }
 
 template <int UM = UM_ANY>
@@ -327,7 +337,7 @@ __device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, flo
     lp = __builtin_fmaf(__builtin_fmaf(w * w, u.inv_var, u.log_norm), -0.5f, lp);
   }
   g = g * u.inv_T;
-  const int mode = UM == UM_RMS ? MLP_SGLD : u.mode;
+  const int mode = um_mode<UM>(u);
   if (mode == MLP_SGLD) {
     const float sg = (-u.eps) * g;
     const float sn = u.noise_scale * xi;
@@ -340,7 +350,7 @@ __device__ __forceinline__ void upd_step(const Upd& u, float& w, float& aux, flo
     } else {
       w = w + (sg + sn);
     }
-  } else if (u.mode == MLP_OBABO_BA) {
+  } else if (mode == MLP_OBABO_BA) {
     const float p1 = __builtin_fmaf(u.decay, aux, u.noise * xi);
     ke = __builtin_fmaf(p1, p1, ke);
     const float p2 = __builtin_fmaf(-u.half, g, p1);
@@ -360,7 +370,7 @@ constexpr int SH_ANY = -1, SH_NONE = 0, SH_BF16 = 1, SH_E4M3 = 2;
 template <int UM = UM_ANY, int SH = SH_ANY>
 __device__ __forceinline__ void upd_store(const Upd& u, uint32_t i, float w, float aux,
                                           float& wmax) {
-  if (UM == UM_RMS || u.mode != MLP_OBABO_BO) {
+  if (um_mode<UM>(u) != MLP_OBABO_BO) {
     upd_st(w, u.th + i);
     if (SH == SH_E4M3 || (SH == SH_ANY && u.sh8)) {
       upd_st(f2q(w * u.q8), u.sh8 + i);
@@ -369,7 +379,7 @@ __device__ __forceinline__ void upd_store(const Upd& u, uint32_t i, float w, flo
       upd_st(f2bf(w), u.sh + i);
     }
   }
-  if (UM == UM_RMS || u.aux) upd_st(aux, u.aux + i);
+  if (UM != UM_ANY || u.aux) upd_st(aux, u.aux + i);
 }
 
 // Buffer resources of one leaf's state (theta, v, GEMM copy) for the compiled-in
@@ -401,15 +411,18 @@ __device__ __forceinline__ void res_load(const UpdRes& r, uint32_t vo, uint32_t 
   aux = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r.aux, 4 * vo, 4 * so, 0));
 }
 
-template <int SH>
+// TH: the rule changes theta (and so its GEMM copy); the B O half of OBABO does not.
+template <int SH, bool TH = true>
 __device__ __forceinline__ void res_store(const Upd& u, const UpdRes& r, uint32_t vo,
                                           uint32_t so, float w, float aux, float& wmax) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(w), r.th, 4 * vo, 4 * so, 0);
-  if (SH == SH_E4M3) {
-    __builtin_amdgcn_raw_buffer_store_b8(f2q(w * u.q8), r.sh, vo, so, 0);
-    wmax = fmaxf(wmax, fabsf(w));
-  } else if (SH == SH_BF16) {
-    __builtin_amdgcn_raw_buffer_store_b16(f2bf(w), r.sh, 2 * vo, 2 * so, 0);
+  if (TH) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(w), r.th, 4 * vo, 4 * so, 0);
+    if (SH == SH_E4M3) {
+      __builtin_amdgcn_raw_buffer_store_b8(f2q(w * u.q8), r.sh, vo, so, 0);
+      wmax = fmaxf(wmax, fabsf(w));
+    } else if (SH == SH_BF16) {
+      __builtin_amdgcn_raw_buffer_store_b16(f2bf(w), r.sh, 2 * vo, 2 * so, 0);
+    }
   }
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(aux), r.aux, 4 * vo, 4 * so, 0);
 }
@@ -855,10 +868,10 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
       constexpr int SH = PREC == P_BF16 ? SH_BF16 : PREC == P_FP8 ? SH_E4M3 : SH_NONE;
       // compiled-in rms_prop: buffer-resource accesses, lane offset o0 plus the uniform
       // row offset of element j (and oh for its band-1 pair)
-      const UpdRes rs = UM == UM_RMS ? make_res<SH>(up, (uint32_t)s.size[g.leaf]) : UpdRes{};
+      const UpdRes rs = UM != UM_ANY ? make_res<SH>(up, (uint32_t)s.size[g.leaf]) : UpdRes{};
       auto so = [&](int j) -> uint32_t { return (uint32_t)(8 * (j >> 2) + (j & 3)) * (uint32_t)g.N; };
       auto load2 = [&](int j, float& wa, float& aa, float& wb, float& ab_) {
-        if constexpr (UM == UM_RMS) {
+        if constexpr (UM != UM_ANY) {
           res_load(rs, o0, so(j), wa, aa);
           res_load(rs, o0, so(j) + oh, wb, ab_);
         } else {
@@ -883,9 +896,9 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
           bits_pair(lk, S, i0, b0, b1);
           upd_step<UM>(up, w0[u], a0[u], ab * acc0[j], bits_to_normal_fast(b0), lp, ke);
           upd_step<UM>(up, w1[u], a1[u], ab * acc1[j], bits_to_normal_fast(b1), lp, ke);
-          if constexpr (UM == UM_RMS) {
-            res_store<SH>(up, rs, o0, so(j), w0[u], a0[u], wmax);
-            res_store<SH>(up, rs, o0, so(j) + oh, w1[u], a1[u], wmax);
+          if constexpr (UM != UM_ANY) {
+            res_store<SH, UM != UM_OBO>(up, rs, o0, so(j), w0[u], a0[u], wmax);
+            res_store<SH, UM != UM_OBO>(up, rs, o0, so(j) + oh, w1[u], a1[u], wmax);
           } else {
             upd_store<UM, SH>(up, i0, w0[u], a0[u], wmax);
             upd_store<UM, SH>(up, i0 + oh, w1[u], a1[u], wmax);
@@ -984,98 +997,136 @@ __global__ void __launch_bounds__(GT) splitk_finish_kernel(Gemm g, Step s) {
 }
 
 // ------------------------------------------------------------------ head ------
-// One wavefront per row: logits = h2 W3 + b3; p = softmax; lik = logits[label] - lse;
-// dL = c (onehot - p); dA2 = (dL W3^T) * (h2 > 0).
+// logits = h2 W3 + b3; p = softmax; lik = logits[label] - lse; dL = c (onehot - p);
+// dA2 = (dL W3^T) * (h2 > 0). A wavefront takes R rows of one chain. W3S: W3 is staged in
+// LDS (row stride kW3Stride, so lanes reading consecutive rows of W3 hit distinct banks)
+// and each W3 row read serves all of the wavefront's rows; otherwise W3 comes from global
+// memory. For every row the sums run in the same order either way: lane-partial sums
+// over the columns a = lane + 64 u in ascending u, then wave sums.
+constexpr int kW3Stride = NCLS + 1;
+
+template <bool W3S, int R>
 __global__ void __launch_bounds__(GT) head_kernel(Step s) {
+  extern __shared__ float w3s[];  // W3S: [H2][kW3Stride]
   const int c = blockIdx.y;
-  const int j = blockIdx.x * (GT / kWave) + threadIdx.x / kWave;
-  if (j >= s.n) return;
   const int lane = threadIdx.x & (kWave - 1);
   const float* th = s.theta + (int64_t)c * s.P;
   const float* W3 = th + s.off[2];
   const float* b3 = th + s.off[5];
-  const int64_t h2row = ((int64_t)c * s.n + j) * s.H2;
-  float z[NCLS];
+  if (W3S) {
+    // 16 loads per thread go out before their LDS stores (indices past W3 are clamped
+    // to its last element and not stored)
+    const int total = s.H2 * NCLS;
+    for (int i0 = threadIdx.x; i0 < total; i0 += 16 * GT) {
+      float v[16];
 #pragma unroll
-  for (int k = 0; k < NCLS; ++k) z[k] = 0.f;
-  // columns a = lane + 64 u, in batches whose loads go out together (four columns, then
-  // pairs: H2 is a multiple of 128); the sums keep the ascending order of a
+      for (int u = 0; u < 16; ++u) v[u] = W3[min(i0 + u * GT, total - 1)];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int i = i0 + u * GT;
+        if (i < total) w3s[(i / NCLS) * kW3Stride + i % NCLS] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+  auto w3at = [&](int a, int k) -> float {
+    return W3S ? w3s[a * kW3Stride + k] : W3[(int64_t)a * NCLS + k];
+  };
+  const int j0 = (blockIdx.x * (GT / kWave) + threadIdx.x / kWave) * R;
+  if (j0 >= s.n) return;  // s.n is a multiple of 4 R: rows j0 .. j0 + R - 1 exist
+  const int64_t row0 = (int64_t)c * s.n + j0;
+  float z[R][NCLS];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) z[r][k] = 0.f;
+  // columns in batches of four per lane (then pairs: H2 is a multiple of 128); the h2
+  // loads of a batch go out together
   auto zbatch = [&](int a0, auto nb) {
     constexpr int NB = decltype(nb)::value;
-    float h[NB], w3[NB][NCLS];
+    float h[R][NB], w[NB][NCLS];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int a = a0 + u * kWave;
-      h[u] = h2_at(s, h2row + a);
 #pragma unroll
-      for (int k = 0; k < NCLS; ++k) w3[u][k] = W3[(int64_t)a * NCLS + k];
+      for (int r = 0; r < R; ++r) h[r][u] = h2_at(s, (row0 + r) * s.H2 + a0 + u * kWave);
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w[u][k] = w3at(a0 + u * kWave, k);
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
-      for (int k = 0; k < NCLS; ++k) z[k] += h[u] * w3[u][k];
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < NCLS; ++k) z[r][k] += h[r][u] * w[u][k];
   };
   {
     int a0 = lane;
     for (; a0 + 3 * kWave < s.H2; a0 += 4 * kWave) zbatch(a0, std::integral_constant<int, 4>{});
     for (; a0 < s.H2; a0 += 2 * kWave) zbatch(a0, std::integral_constant<int, 2>{});
   }
-#pragma unroll
-  for (int k = 0; k < NCLS; ++k) z[k] = wave_sum(z[k]) + b3[k];
-  float mx = z[0];
-#pragma unroll
-  for (int k = 1; k < NCLS; ++k) mx = fmaxf(mx, z[k]);
-  float se = 0.f;
-#pragma unroll
-  for (int k = 0; k < NCLS; ++k) se += expf(z[k] - mx);
-  const float lse = mx + logf(se);
-  int lab = s.lab[(int64_t)c * s.n + j];
-  if (s.full) {  // likelihood only; padding rows (label -1) contribute 0
-    const int lb = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
-    float zl = z[0];
-#pragma unroll
-    for (int k = 1; k < NCLS; ++k)
-      if (k == lb) zl = z[k];
-    if (lane == 0) s.lik[(int64_t)c * s.n + j] = lab < 0 ? 0.0f : zl - lse;
-    return;
-  }
-  lab = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
   const float cscale = ((float)(-s.N) / (float)s.n) / s.temperature;
-  float dl[NCLS];
-  float zl = 0.f;
+  float dl[R][NCLS];
 #pragma unroll
-  for (int k = 0; k < NCLS; ++k) {
-    const float p = expf(z[k] - lse);
-    dl[k] = cscale * ((k == lab ? 1.0f : 0.0f) - p);
-    if (k == lab) zl = z[k];
-  }
-  if (lane == 0) {
-    s.lik[(int64_t)c * s.n + j] = zl - lse;
+  for (int r = 0; r < R; ++r) {
+    float zr[NCLS];
 #pragma unroll
-    for (int k = 0; k < NCLS; ++k) s.dL[((int64_t)c * s.n + j) * NCLS + k] = dl[k];
+    for (int k = 0; k < NCLS; ++k) zr[k] = wave_sum(z[r][k]) + b3[k];
+    float mx = zr[0];
+#pragma unroll
+    for (int k = 1; k < NCLS; ++k) mx = fmaxf(mx, zr[k]);
+    float se = 0.f;
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) se += expf(zr[k] - mx);
+    const float lse = mx + logf(se);
+    int lab = s.lab[row0 + r];
+    if (s.full) {  // likelihood only; padding rows (label -1) contribute 0
+      const int lb = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
+      float zl = zr[0];
+#pragma unroll
+      for (int k = 1; k < NCLS; ++k)
+        if (k == lb) zl = zr[k];
+      if (lane == 0) s.lik[row0 + r] = lab < 0 ? 0.0f : zl - lse;
+      continue;
+    }
+    lab = lab < 0 ? 0 : (lab >= NCLS ? NCLS - 1 : lab);
+    float zl = 0.f;
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) {
+      const float p = expf(zr[k] - lse);
+      dl[r][k] = cscale * ((k == lab ? 1.0f : 0.0f) - p);
+      if (k == lab) zl = zr[k];
+    }
+    if (lane == 0) {
+      s.lik[row0 + r] = zl - lse;
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) s.dL[(row0 + r) * NCLS + k] = dl[r][k];
+    }
   }
-  const int64_t row = (int64_t)c * s.n + j;
+  if (s.full) return;
   const float qs = s.fp8 ? s.q[(int64_t)c * kQWords + Q_DA2] : 1.0f;
   float amax = 0.f;
   auto dbatch = [&](int a0, auto nb) {
     constexpr int NB = decltype(nb)::value;
-    float w3[NB][NCLS];
-    bool on[NB];
+    float w[NB][NCLS];
+    bool on[R][NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int a = a0 + u * kWave;
-      on[u] = h2_on(s, h2row + a);
 #pragma unroll
-      for (int k = 0; k < NCLS; ++k) w3[u][k] = W3[(int64_t)a * NCLS + k];
+      for (int r = 0; r < R; ++r) on[r][u] = h2_on(s, (row0 + r) * s.H2 + a0 + u * kWave);
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) w[u][k] = w3at(a0 + u * kWave, k);
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      float d = 0.f;
 #pragma unroll
-      for (int k = 0; k < NCLS; ++k) d += dl[k] * w3[u][k];
-      const float v = on[u] ? d : 0.0f;
-      amax = fmaxf(amax, fabsf(v));
-      put_act(s.da2, row * s.H2 + a0 + u * kWave, v, s.fp8 != 0, qs, s.fp32 != 0);
+      for (int r = 0; r < R; ++r) {
+        float d = 0.f;
+#pragma unroll
+        for (int k = 0; k < NCLS; ++k) d += dl[r][k] * w[u][k];
+        const float v = on[r][u] ? d : 0.0f;
+        amax = fmaxf(amax, fabsf(v));
+        put_act(s.da2, (row0 + r) * s.H2 + a0 + u * kWave, v, s.fp8 != 0, qs, s.fp32 != 0);
+      }
     }
   };
   {
@@ -1562,14 +1613,27 @@ enum GemmSlot { SLOT_G1 = 0, SLOT_G2, SLOT_G3, SLOT_G4, SLOT_G5, kSlots };
 template <int BK, int PREC, bool A_K, bool B_K, int EPI>
 void launch_gemm_bk(dim3 grid, Gemm g, const Step& s, hipStream_t st, float* part,
                     int splits) {
-  // parameter updates of SGLD with rms_prop run the kernel with that rule compiled in
+  // parameter updates of SGLD with rms_prop and of the two OBABO halves run the kernel
+  // with their rule compiled in
   if constexpr (EPI == EPI_UPDATE) {
-    if (s.mode == MLP_SGLD && s.rms && splits <= 1) {
+    if (splits <= 1) {
       g.C = (int)grid.z;
       g.splits = 1;
-      hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI, UM_RMS>), grid, dim3(GT), 0, st,
-                         g, s);
-      return;
+      if (s.mode == MLP_SGLD && s.rms) {
+        hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI, UM_RMS>), grid, dim3(GT), 0, st,
+                           g, s);
+        return;
+      }
+      if (s.mode == MLP_OBABO_BA) {
+        hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI, UM_OBA>), grid, dim3(GT), 0, st,
+                           g, s);
+        return;
+      }
+      if (s.mode == MLP_OBABO_BO) {
+        hipLaunchKernelGGL((gemm_kernel<BK, PREC, A_K, B_K, EPI, UM_OBO>), grid, dim3(GT), 0, st,
+                           g, s);
+        return;
+      }
     }
   }
   g.C = (int)grid.z;
@@ -1764,6 +1828,24 @@ const void* gemm_weight(const Step& s, const sgmc_mlp_state& q, int l) {
   return q.shadow + s.off[l];
 }
 
+// The head over rows [0, n) of every chain: W3 through LDS while it fits in 64 KB; four
+// rows per wavefront when that still leaves two blocks per CU (measured: 3072-1024-1024,
+// 8192 rows, 46.8 against 55.4 us), else one (3072-512-512, 4096 rows: 17.3 against 31.1).
+void launch_head(const Step& s, hipStream_t st) {
+  const size_t lds = (size_t)s.H2 * kW3Stride * sizeof(float);
+  const int rows_per_block4 = (GT / kWave) * 4;
+  const bool four = (int64_t)s.C * s.n / rows_per_block4 >= 512;
+  const int per_block = (GT / kWave) * (four ? 4 : 1);
+  const dim3 grid(s.n / per_block, s.C);
+  if (lds <= 65536) {
+    if (four) hipLaunchKernelGGL((head_kernel<true, 4>), grid, dim3(GT), lds, st, s);
+    else hipLaunchKernelGGL((head_kernel<true, 1>), grid, dim3(GT), lds, st, s);
+  } else {
+    if (four) hipLaunchKernelGGL((head_kernel<false, 4>), grid, dim3(GT), 0, st, s);
+    else hipLaunchKernelGGL((head_kernel<false, 1>), grid, dim3(GT), 0, st, s);
+  }
+}
+
 // gather, forward GEMMs and the head (likelihood, and its backward unless s.full).
 void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
   const int C = q.C, n = q.n;
@@ -1779,7 +1861,7 @@ void launch_forward(const Step& s, const sgmc_mlp_state& q, hipStream_t st_) {
           s.h2, q.H2, (int64_t)n * q.H2, q.theta + s.off[4], q.P, nullptr, 0, 0, 0, 0};
   launch_gemm<true, false, EPI_BIAS_RELU>(dim3(n / BM, q.H2 / BN, C), with_q(g2, Q_H1, Q_W2, -1),
                                           s, st_, SLOT_G2);
-  hipLaunchKernelGGL(head_kernel, rows_grid, dim3(GT), 0, st_, s);
+  launch_head(s, st_);
 }
 
 }  // namespace

@@ -275,3 +275,29 @@ def test_mlp_fp8_sggmc_graph_replay_matches_eager():
   assert torch.isfinite(a.theta).all()
   assert torch.equal(a.theta, b.theta) and torch.equal(a.U_full, b.U_full)
   assert torch.equal(a.q, b.q)
+
+
+def test_mlp_head_rows_per_wavefront_bit_equal():
+  """The output-layer kernel takes four rows per wavefront once a launch has 8192 rows
+  and one row below that; every row's sums run in the same order either way. Chain 0 of
+  a 16-chain engine (8192 rows: four per wavefront) equals the same chain alone (512
+  rows: one per wavefront) bit for bit, potential and gradient, with the GEMM plan
+  fixed; and it stays within the bf16 bounds of the float64 oracle."""
+  print("mlp head rows per wavefront", flush=True)
+  X, lab = data()
+  n = 512
+  dk = [(3, 100 + c) for c in range(16)]
+  big, th = engine(X, lab, C=16, n=n, dkeys=dk)
+  one, _ = engine(X, lab, C=1, n=n, dkeys=dk[:1])
+  fixed = [128 * 16 + 1, 128 * 16 + 1, 128 * 16 + 1, 128 * 16 + 1, 128 * 16 + 1]
+  big.tune = list(fixed)
+  one.tune = list(fixed)
+  Ub, gb = big.potential_and_grad()
+  Uo, go = one.potential_and_grad()
+  torch.cuda.synchronize()
+  assert torch.equal(Ub[0], Uo[0]) and torch.equal(gb[0], go[0])
+  for c in (0, 15):
+    Ur, gr, _ = oracle_potential(X, lab, th[c], dk[c], n)
+    assert abs(Ub[c].item() - Ur) <= 1e-2 * abs(Ur), (c, Ub[c].item(), Ur)
+    gk = gb[c].cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(gk - gr) / np.linalg.norm(gr) < 3e-2
