@@ -915,17 +915,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
 // For chains of D <= 4 LPR (one float4 chunk per lane of a row segment) and minibatches
 // of at most 256 rows (one 64-row chunk per wavefront), the chain's state stays on chip
 // for the whole launch: theta in LDS (and its owner threads' registers), the keys in
-// registers, the column sums in LDS. The minibatch of the next evaluation (its indices
-// depend only on the data key) is drawn and its rows loaded right after this
-// evaluation's row groups are consumed, so their latency overlaps the reduction and the
-// update. The arithmetic is fused_body's for a chain of one block, in the same order
-// (row groups, reductions, the SGLD / rms_prop update, U, var), so both give the same
-// bits.
+// registers, the wavefronts' column partials in LDS. The step's noise is drawn before
+// the rows are processed; the minibatch of the next evaluation (its indices depend only
+// on the data key) is drawn and its rows loaded right after this evaluation's row groups
+// are consumed, so their latency overlaps the reduction and the update. The arithmetic
+// is fused_body's for a chain of one block, in the same order (row groups, reductions,
+// the SGLD / rms_prop update, U, var), so both give the same bits.
 template <int LPR, int MODE>
 __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm_chain& ch,
                                               int iterations, float* lds_cols, float* lds_small,
-                                              float* lds_lik, float* th_lds, float* col_lds,
-                                              float* eps_lds, int* slot_lds) {
+                                              float* lds_lik, float* th_lds, float* eps_lds,
+                                              int* slot_lds) {
   constexpr int CPL = 1;
   using RG = RowGroup<LPR, CPL>;
   constexpr int G = (kWave / LPR) * RG::U;  // rows per group
@@ -984,7 +984,8 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   // Barriers inside the loop wait for LDS only (lds_sync): the next minibatch's row
   // loads stay in flight through the reduction and the update. The step sizes and save
   // slots come from LDS (a window of kResWindow iterations), so no global load has to be
-  // waited on behind those row loads.
+  // waited on behind those row loads. An iteration has two barriers (the wavefronts'
+  // reduction records, the new theta), four more where U or var(lik) is kept.
   for (int t = 0; t < iterations; ++t) {
     const int tw = t % kResWindow;
     if (tw == 0) {
@@ -998,6 +999,18 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
     }
     const float eps = eps_lds[tw];
     const int slot = slot_lds[tw];
+    // the step's noise (independent of theta): (key', sub) = split(key), leaf of sub
+    Key knext, ksub;
+    split2_uniform(key, knext, ksub);
+    const Key leaf = split_key(ksub, 1, 0);
+    float xi[2] = {0.0f, 0.0f};
+    if (own) {
+      uint32_t wb0, wb1;
+      bits_pair(leaf, S, jp, wb0, wb1);
+      xi[0] = bits_to_normal(wb0);
+      if (two) xi[1] = bits_to_normal(wb1);
+    }
+    key = knext;
     float4 th[CPL];
     {
       const int q = (threadIdx.x & (kWave - 1)) % LPR;
@@ -1011,31 +1024,46 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
       if (gi < ng) process_group<LPR, CPL>(buf[gi], r0 + gi * G, D4, th, a.L, acc, lik_acc, lds_lik, 0);
     dkey = dnext;
     if (t + 1 < iterations) draw_rows();
-    const float ls = block_reduce<LPR, CPL, true>(acc, lik_acc, D4, lds_cols, lds_small,
-                                                  [&](int cc, float sv) { col_lds[cc] = sv; });
+    // block_reduce's wavefront part: segment sums to the first LPR lanes, the wave's
+    // column partials and lik sum to LDS; one barrier, then each owner thread adds its
+    // two columns over the wavefronts in block_reduce's order
+    {
+      const int lane = threadIdx.x & (kWave - 1);
+      const int wv = threadIdx.x / kWave;
+#pragma unroll
+      for (int m = LPR; m < kWave; m <<= 1) {
+        acc[0].x += __shfl_xor(acc[0].x, m, kWave);
+        acc[0].y += __shfl_xor(acc[0].y, m, kWave);
+        acc[0].z += __shfl_xor(acc[0].z, m, kWave);
+        acc[0].w += __shfl_xor(acc[0].w, m, kWave);
+      }
+      const float lw = wave_sum(lik_acc);
+      if (lane < LPR && lane < D4) reinterpret_cast<float4*>(lds_cols + wv * 4 * D4)[lane] = acc[0];
+      if (lane == 0) lds_small[wv] = lw;
+    }
     lds_sync();
-    Key knext, ksub;
-    split2_uniform(key, knext, ksub);
-    const Key leaf = split_key(ksub, 1, 0);
+    float ls = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) ls += lds_small[i];
     float lp = 0.0f;
     if (own) {
-      uint32_t wb0, wb1;
-      bits_pair(leaf, S, jp, wb0, wb1);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (h && !two) break;
         const uint32_t j = jj[h];
-        const float xi = bits_to_normal(h ? wb1 : wb0);
+        float sc = 0.0f;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) sc += lds_cols[ww * 4 * D4 + j];
         const float wj = w[h];
         // one published record: the last block's sum over records starts from +0
-        float g = c * (0.0f + col_lds[j]);
+        float g = c * (0.0f + sc);
         if (a.pr.has_prior) {
           g = g + wj * a.pr.inv_var;
           lp += (a.pr.log_norm + (wj * wj) * a.pr.inv_var) / -2.0f;
         }
         g = g / a.pot_temperature;
         const float sg = (-eps) * g;
-        const float sn = sqrtf((2.0f * Tn) * eps) * xi;
+        const float sn = sqrtf((2.0f * Tn) * eps) * xi[h];
         float wn;
         if (rms) {
           const float vn = a.alpha * v[h] + (1.0f - a.alpha) * (g * g);
@@ -1050,28 +1078,31 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
         if (slot >= 0 && ch.samples) g_samples[(int64_t)slot * D + j] = wn;
       }
     }
-    key = knext;
-    lp = block_sum<kNT, true>(lp, lds_small);
-    // the block's lik sum is the only record: the sum over records is ls + 0
-    const float mean = (ls + 0.0f) / (float)n;
-    Uk = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
-    if (threadIdx.x == 0 && slot >= 0 && ch.samples_U) ch.samples_U[slot] = Uk;
-    if (ch.var && t == iterations - 1) {
-      // var(lik) of the last evaluation, as fused_body forms it from one record
-      const float mb = n > 0 ? ls / (float)n : 0.0f;
-      float s2 = 0.0f;
-      for (int jr = threadIdx.x; jr < n; jr += kNT) {
-        const float d = lds_lik[jr] - mb;
-        s2 += d * d;
+    // U (and var(lik)) only where they are kept: a save slot or the last iteration
+    const bool last = t == iterations - 1;
+    if ((slot >= 0 && ch.samples_U) || last) {
+      lp = block_sum<kNT, true>(lp, lds_small);
+      // the block's lik sum is the only record: the sum over records is ls + 0
+      const float mean = (ls + 0.0f) / (float)n;
+      Uk = ((float)(-ch.N) * mean - lp) / a.pot_temperature;
+      if (threadIdx.x == 0 && slot >= 0 && ch.samples_U) ch.samples_U[slot] = Uk;
+      if (ch.var && last) {
+        // var(lik) of the last evaluation, as fused_body forms it from one record
+        const float mb = n > 0 ? ls / (float)n : 0.0f;
+        float s2 = 0.0f;
+        for (int jr = threadIdx.x; jr < n; jr += kNT) {
+          const float d = lds_lik[jr] - mb;
+          s2 += d * d;
+        }
+        const float m2 = block_sum<kNT, true>(s2, lds_small);
+        float tv = 0.0f;
+        if (threadIdx.x == 0 && n > 0) {
+          const float d = ls / (float)n - mean;
+          tv += m2 + (float)n * (d * d);
+        }
+        const float var_sum = block_sum<kNT, true>(tv, lds_small);
+        if (threadIdx.x == 0) ch.var[0] = var_sum / (float)n;
       }
-      const float m2 = block_sum<kNT, true>(s2, lds_small);
-      float tv = 0.0f;
-      if (threadIdx.x == 0 && n > 0) {
-        const float d = ls / (float)n - mean;
-        tv += m2 + (float)n * (d * d);
-      }
-      const float var_sum = block_sum<kNT, true>(tv, lds_small);
-      if (threadIdx.x == 0) ch.var[0] = var_sum / (float)n;
     }
     lds_sync();
   }
@@ -1101,7 +1132,6 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   __shared__ float lds_small[kWaves];
   __shared__ float lds_lik[kLdsRows];
   __shared__ float res_th[64];
-  __shared__ float res_col[64];
   __shared__ float res_eps[kResWindow];
   __shared__ int res_slot[kResWindow];
   __shared__ int flag;
@@ -1122,7 +1152,7 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   if (resident) {
     if constexpr (DCLASS == 0 || DCLASS == 1)
       resident_loop<DCLASS == 0 ? 4 : 16, MODE>(a, ch, iterations, lds_cols, lds_small, lds_lik,
-                                                res_th, res_col, res_eps, res_slot);
+                                                res_th, res_eps, res_slot);
   }
   for (int t = 0; t < (resident ? 0 : iterations); ++t) {
     b.iter_offset = t;
