@@ -967,11 +967,27 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   Key dnext;
   RG buf[NG];
   // (dkey', sub) = split(dkey); k1, k2 = split(sub); the wavefront's rows into buf
+  // a wavefront of at most 32 rows draws their two randint words on the two lane halves
+  // (lanes 0-31 the words of k1, lanes 32-63 those of k2), one Threefry block per lane
+  const bool halves = r1 - r0 <= kWave / 2;
   auto draw_rows = [&]() {
     Key dsub;
     split2_uniform(dkey, dnext, dsub);
     split2_uniform(dsub, src.k1, src.k2);
-    const int rid = r0 < r1 ? chunk_rows(src, r0, r1) : -1;
+    int rid = -1;
+    if (halves) {
+      const int lane = threadIdx.x & (kWave - 1);
+      const int rr = lane & (kWave / 2 - 1);
+      const Key kk = lane < kWave / 2 ? src.k1 : src.k2;
+      const uint32_t word = bits_word(kk, src.n, (uint32_t)(r0 + rr));
+      const uint32_t lo = (uint32_t)__shfl((int)word, rr + kWave / 2, kWave);
+      if (lane < kWave / 2 && r0 + rr < r1) {
+        const int64_t r = (int64_t)randint_offset(word, lo, src.span, src.mult, src.magic);
+        if (r >= 0 && r < src.rows) rid = (int)r;
+      }
+    } else if (r0 < r1) {
+      rid = chunk_rows(src, r0, r1);
+    }
 #pragma unroll
     for (int gi = 0; gi < NG; ++gi)
       if (gi < ng) load_group<LPR, CPL>(to_global(ch.X), ch.ldx, to_global(ch.y), D4, gi * G, rid,
