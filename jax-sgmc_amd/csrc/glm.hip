@@ -925,7 +925,7 @@ template <int LPR, int MODE>
 __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm_chain& ch,
                                               int iterations, float* lds_cols, float* lds_small,
                                               float* lds_lik, float* th_lds, float* eps_lds,
-                                              int* slot_lds) {
+                                              int* slot_lds, uint32_t* kp, float* xis) {
   constexpr int CPL = 1;
   using RG = RowGroup<LPR, CPL>;
   constexpr int G = (kWave / LPR) * RG::U;  // rows per group
@@ -937,7 +937,8 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   gfloat* g_v = to_global(ch.v);
   gfloat* g_samples = to_global(ch.samples);
   const int it0 = a.iter[0];
-  Key key = load_key(ch.key), dkey = load_key(ch.dkey);
+  const int wv = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   // Thread jp < m owns the noise pair (jp, jp + m), as in fused_body's update loop.
   const uint32_t S = (uint32_t)D, m = (S + 1) >> 1;
   const uint32_t jp = threadIdx.x;
@@ -964,19 +965,27 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   src.mult = randint_mult(src.span);
   src.magic = randint_magic(src.span);
   src.rows = ch.N;
-  Key dnext;
   RG buf[NG];
-  // (dkey', sub) = split(dkey); k1, k2 = split(sub); the wavefront's rows into buf
-  // a wavefront of at most 32 rows draws their two randint words on the two lane halves
-  // (lanes 0-31 the words of k1, lanes 32-63 those of k2), one Threefry block per lane
+  // The theta-independent key work runs ahead of its use, one job per wavefront, handed
+  // over through LDS (kp, xis; slot = iteration parity) across the loop's barriers:
+  // - wavefront 0: the data-key chain d' = split(d)[0], (k1, k2) = split(split(d)[1]),
+  //   two iterations ahead (kp[4 + 4 slot ..]);
+  // - wavefront 1: the noise-key chain key' = split(key)[0], leaf = split_key(
+  //   split(key)[1], 1, 0), two iterations ahead (kp[2 slot ..]);
+  // - wavefronts 2 and 3: the normals of noise pair jp (words 0 and 1 of its Threefry
+  //   block) one iteration ahead (xis[64 slot + 2 jp + h]).
+  // kp[12 ..]: the keys after the launch's last iteration.
+  auto put_key = [&](int at, Key k) { kp[at] = k.k0; kp[at + 1] = k.k1; };
+  auto get_key = [&](int at) { return Key{kp[at], kp[at + 1]}; };
+  // the wavefront's rows of a minibatch with randint keys (k1, k2) into buf; a wavefront
+  // of at most 32 rows draws their two randint words on the two lane halves (lanes 0-31
+  // the words of k1, lanes 32-63 those of k2), one Threefry block per lane
   const bool halves = r1 - r0 <= kWave / 2;
-  auto draw_rows = [&]() {
-    Key dsub;
-    split2_uniform(dkey, dnext, dsub);
-    split2_uniform(dsub, src.k1, src.k2);
+  auto draw_rows = [&](Key k1, Key k2) {
+    src.k1 = k1;
+    src.k2 = k2;
     int rid = -1;
     if (halves) {
-      const int lane = threadIdx.x & (kWave - 1);
       const int rr = lane & (kWave / 2 - 1);
       const Key kk = lane < kWave / 2 ? src.k1 : src.k2;
       const uint32_t word = bits_word(kk, src.n, (uint32_t)(r0 + rr));
@@ -993,7 +1002,34 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
       if (gi < ng) load_group<LPR, CPL>(to_global(ch.X), ch.ldx, to_global(ch.y), D4, gi * G, rid,
                                         buf[gi]);
   };
-  draw_rows();
+  Key kc, dc;  // wavefront 1: key_{t+2}; wavefront 0: d_{t+2} (at iteration t)
+  {
+    // keys of iterations 0 and 1 (every thread; uniform values)
+    Key key1, d1, ks, dsub, k1, k2;
+    split2_uniform(load_key(ch.key), key1, ks);
+    const Key leaf0 = split_key(ks, 1, 0);
+    split2_uniform(key1, kc, ks);
+    const Key leaf1 = split_key(ks, 1, 0);
+    split2_uniform(load_key(ch.dkey), d1, dsub);
+    split2_uniform(dsub, k1, k2);
+    Key k1n, k2n;
+    split2_uniform(d1, dc, dsub);
+    split2_uniform(dsub, k1n, k2n);
+    if (threadIdx.x == 0) {
+      put_key(2, leaf1);
+      put_key(8, k1n);
+      put_key(10, k2n);
+      put_key(12, key1);  // the keys after one iteration (later ones overwrite)
+      put_key(14, d1);
+    }
+    if (own) {  // the normals of iteration 0
+      uint32_t wb0, wb1;
+      bits_pair(leaf0, S, jp, wb0, wb1);
+      xis[2 * jp] = bits_to_normal(wb0);
+      if (two) xis[2 * jp + 1] = bits_to_normal(wb1);
+    }
+    draw_rows(k1, k2);
+  }
   const float c = (float)(-ch.N) / (float)n;
   const float Tn = ch.temperature;
   float Uk = 0.0f;
@@ -1015,18 +1051,33 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
     }
     const float eps = eps_lds[tw];
     const int slot = slot_lds[tw];
-    // the step's noise (independent of theta): (key', sub) = split(key), leaf of sub
-    Key knext, ksub;
-    split2_uniform(key, knext, ksub);
-    const Key leaf = split_key(ksub, 1, 0);
-    float xi[2] = {0.0f, 0.0f};
-    if (own) {
-      uint32_t wb0, wb1;
-      bits_pair(leaf, S, jp, wb0, wb1);
-      xi[0] = bits_to_normal(wb0);
-      if (two) xi[1] = bits_to_normal(wb1);
+    const int par = t & 1;
+    if (wv == 0) {  // (k1, k2) of iteration t + 2, and d_{t+3}
+      if (t + 2 == iterations && lane == 0) put_key(14, dc);
+      Key dn, dsub, k1, k2;
+      split2_uniform(dc, dn, dsub);
+      split2_uniform(dsub, k1, k2);
+      if (lane == 0) {
+        put_key(4 + 4 * par, k1);
+        put_key(6 + 4 * par, k2);
+      }
+      dc = dn;
+    } else if (wv == 1) {  // leaf of iteration t + 2, and key_{t+3}
+      if (t + 2 == iterations && lane == 0) put_key(12, kc);
+      Key kn, ks;
+      split2_uniform(kc, kn, ks);
+      const Key lf = split_key(ks, 1, 0);
+      if (lane == 0) put_key(2 * par, lf);
+      kc = kn;
+    } else if (t + 1 < iterations) {  // normals of iteration t + 1 (word wv - 2 of each pair)
+      const int h = wv - 2;
+      const uint32_t q = (uint32_t)lane;
+      if (q < m && (h == 0 || q + m < S)) {
+        uint32_t wb0, wb1;
+        bits_pair(get_key(2 * (par ^ 1)), S, q, wb0, wb1);
+        xis[64 * (par ^ 1) + 2 * q + h] = bits_to_normal(h ? wb1 : wb0);
+      }
     }
-    key = knext;
     float4 th[CPL];
     {
       const int q = (threadIdx.x & (kWave - 1)) % LPR;
@@ -1038,20 +1089,17 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
 #pragma unroll
     for (int gi = 0; gi < NG; ++gi)
       if (gi < ng) process_group<LPR, CPL>(buf[gi], r0 + gi * G, D4, th, a.L, acc, lik_acc, lds_lik, 0);
-    dkey = dnext;
-    if (t + 1 < iterations) draw_rows();
+    if (t + 1 < iterations) draw_rows(get_key(4 + 4 * (par ^ 1)), get_key(6 + 4 * (par ^ 1)));
     // block_reduce's wavefront part: segment sums to the first LPR lanes, the wave's
     // column partials and lik sum to LDS; one barrier, then each owner thread adds its
     // two columns over the wavefronts in block_reduce's order
     {
-      const int lane = threadIdx.x & (kWave - 1);
-      const int wv = threadIdx.x / kWave;
 #pragma unroll
-      for (int m = LPR; m < kWave; m <<= 1) {
-        acc[0].x += __shfl_xor(acc[0].x, m, kWave);
-        acc[0].y += __shfl_xor(acc[0].y, m, kWave);
-        acc[0].z += __shfl_xor(acc[0].z, m, kWave);
-        acc[0].w += __shfl_xor(acc[0].w, m, kWave);
+      for (int mm = LPR; mm < kWave; mm <<= 1) {
+        acc[0].x += __shfl_xor(acc[0].x, mm, kWave);
+        acc[0].y += __shfl_xor(acc[0].y, mm, kWave);
+        acc[0].z += __shfl_xor(acc[0].z, mm, kWave);
+        acc[0].w += __shfl_xor(acc[0].w, mm, kWave);
       }
       const float lw = wave_sum(lik_acc);
       if (lane < LPR && lane < D4) reinterpret_cast<float4*>(lds_cols + wv * 4 * D4)[lane] = acc[0];
@@ -1079,7 +1127,7 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
         }
         g = g / a.pot_temperature;
         const float sg = (-eps) * g;
-        const float sn = sqrtf((2.0f * Tn) * eps) * xi[h];
+        const float sn = sqrtf((2.0f * Tn) * eps) * xis[64 * par + 2 * jp + h];
         float wn;
         if (rms) {
           const float vn = a.alpha * v[h] + (1.0f - a.alpha) * (g * g);
@@ -1132,8 +1180,8 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   }
   if (threadIdx.x == 0) {
     ch.U[0] = Uk;
-    store_key(ch.key, key);
-    store_key(ch.dkey, dkey);
+    store_key(ch.key, get_key(12));
+    store_key(ch.dkey, get_key(14));
   }
 }
 
@@ -1150,6 +1198,8 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   __shared__ float res_th[64];
   __shared__ float res_eps[kResWindow];
   __shared__ int res_slot[kResWindow];
+  __shared__ uint32_t res_kp[16];
+  __shared__ float res_xi[128];
   __shared__ int flag;
   // The grid has one block per chain and every chain one block (sgmc_glm_check_table).
   const int chain = blockIdx.x;
@@ -1168,7 +1218,7 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   if (resident) {
     if constexpr (DCLASS == 0 || DCLASS == 1)
       resident_loop<DCLASS == 0 ? 4 : 16, MODE>(a, ch, iterations, lds_cols, lds_small, lds_lik,
-                                                res_th, res_eps, res_slot);
+                                                res_th, res_eps, res_slot, res_kp, res_xi);
   }
   for (int t = 0; t < (resident ? 0 : iterations); ++t) {
     b.iter_offset = t;
