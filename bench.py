@@ -571,6 +571,15 @@ def mlp_posterior(torch, M, R, X, labels, th0, keys, dkeys, dims, n, prec, integ
   return out
 
 
+def tile_label(names, key):
+  """'G1 k128 s2' for a (slot, code) timing, 'G4+G5 k32 k128' for a joint one."""
+  if len(key) == 2:
+    i, c = key
+    return f"{names[i].split()[0]} k{c >> 4} s{c & 15}"
+  _, c4, c5 = key
+  return f"G4+G5 k{c4 >> 4} k{c5 >> 4}"
+
+
 def tune_tiles(eng, args):
   """Times the GEMM tilings on the live problem (FusedMLP.autotune) unless disabled;
   returns the plan and its provenance for the bench line's config."""
@@ -589,10 +598,11 @@ def tune_tiles(eng, args):
     plan = eng.autotune()
     b, t = eng.tune_gain
     how = {"source": "autotune", "unit_ms_builtin": b, "unit_ms_tuned": t,
-           "unit_ms": {f"{names[i].split()[0]} k{c >> 4} s{c & 15}": round(ms, 5)
-                       for (i, c), ms in sorted(eng.tune_ms.items())}}
+           "unit_ms": {tile_label(names, key): round(ms, 5)
+                       for key, ms in sorted(eng.tune_ms.items())}}
   tiles = {nm: {"BM": 64, "BN": 128, "BK": k, "split_k": s} for nm, (k, s) in zip(names, plan)}
-  return dict(how, tiles=tiles)
+  # gW2 runs in gW1's launch when both take the same k-tile
+  return dict(how, tiles=tiles, gw1_gw2_one_launch=plan[3] == plan[4])
 
 
 def run_mlp(args):

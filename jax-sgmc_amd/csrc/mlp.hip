@@ -748,26 +748,24 @@ __device__ __forceinline__ float get_act(const Step& s, int c, int t, const void
   return bf2f(reinterpret_cast<const uint16_t*>(p)[i]);
 }
 
-template <int BK, int PREC, bool A_K, bool B_K, int EPI, int UM = UM_ANY>
-__global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
-    gemm_kernel(Gemm g, Step s) {
+// One block's tile of GEMM g: block lin (linear id) of a gx x gy x gz grid.
+template <int BK, int PREC, bool A_K, bool B_K, int EPI, int UM>
+__device__ __forceinline__ void gemm_tile(const Gemm& g, const Step& s, uint32_t lin,
+                                          uint32_t gx, uint32_t gy, uint32_t gz, uint8_t* As,
+                                          uint8_t* Bs, float* red) {
   static_assert(BN == 2 * BM, "tile shapes");
   constexpr bool F8 = PREC == P_FP8;
   constexpr int EB = elem_bytes<PREC>();
-  __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
-  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
-  __shared__ float red[GT / kWave];
   // Blocks go to the XCDs round-robin by linear id. In the GEMMs without a parameter
   // update, XCD x takes the contiguous range [x T / 8, (x + 1) T / 8) of logical tiles
   // (bm fastest, then bn, then chain and split), so the blocks that share an operand tile
   // read it through one L2 instead of fetching it from HBM once per XCD (Xb W1 of a
   // 3072-512 net: 390 -> 75 MB per launch). The update GEMMs keep the linear order, whose
   // state stream measured faster.
-  const uint32_t T = gridDim.x * gridDim.y * gridDim.z;
-  uint32_t lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const uint32_t T = gx * gy * gz;
   if (EPI != EPI_UPDATE && T % kXcds == 0) lin = (lin % kXcds) * (T / kXcds) + lin / kXcds;
-  const int bm = (int)(lin % gridDim.x), bn = (int)((lin / gridDim.x) % gridDim.y);
-  const int bz = (int)(lin / (gridDim.x * gridDim.y));
+  const int bm = (int)(lin % gx), bn = (int)((lin / gx) % gy);
+  const int bz = (int)(lin / (gx * gy));
   const int c = bz / g.splits, sp = bz % g.splits;
   const int kspan = g.K / g.splits;
   const int kb = sp * kspan, ke = kb + kspan;
@@ -944,7 +942,7 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
       }
     }
     if (EPI == EPI_UPDATE && s.fp8 && g.leaf < 2) q_note_amax(s, c, Q_W1 + g.leaf, wmax);
-    const int64_t slot = (int64_t)c * s.nparts + g.part0 + bm * gridDim.y + bn;
+    const int64_t slot = (int64_t)c * s.nparts + g.part0 + bm * gy + bn;
     const float tot = block_sum<GT>(lp, red);
     if (threadIdx.x == 0) s.prior_part[slot] = tot;
     if (EPI == EPI_UPDATE && UM != UM_RMS && s.mode != MLP_SGLD) {
@@ -952,6 +950,37 @@ __global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
       if (threadIdx.x == 0) s.ke_part[slot] = kt;
     }
   }
+}
+
+template <int BK, int PREC, bool A_K, bool B_K, int EPI, int UM = UM_ANY>
+__global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
+    gemm_kernel(Gemm g, Step s) {
+  constexpr int EB = elem_bytes<PREC>();
+  __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
+  __shared__ float red[GT / kWave];
+  gemm_tile<BK, PREC, A_K, B_K, EPI, UM>(
+      g, s, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x,
+      gridDim.y, gridDim.z, As, Bs, red);
+}
+
+// The two parameter GEMMs of an evaluation in one launch (one k-tile depth): a 1-D grid
+// whose first ga.gx ga.gy C blocks take gW1's tiles (ga) in their linear order and the
+// rest gW2's (gb), so gW2's blocks fill the last round of gW1's instead of a launch and
+// a latency-bound round of their own.
+template <int BK, int PREC, bool A_K, bool B_K, int EPI, int UM = UM_ANY>
+__global__ void __launch_bounds__(GT, EPI == EPI_UPDATE ? SGMC_UPD_WAVES : 1)
+    gemm_pair_kernel(Gemm ga, Gemm gb, Step s, uint32_t gxa, uint32_t gya, uint32_t gxb,
+                     uint32_t gyb) {
+  constexpr int EB = elem_bytes<PREC>();
+  __shared__ __attribute__((aligned(16))) uint8_t As[BM * BK * EB];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN * BK * EB];
+  __shared__ float red[GT / kWave];
+  const uint32_t na = gxa * gya * (uint32_t)s.C;
+  if (blockIdx.x < na)
+    gemm_tile<BK, PREC, A_K, B_K, EPI, UM>(ga, s, blockIdx.x, gxa, gya, s.C, As, Bs, red);
+  else
+    gemm_tile<BK, PREC, A_K, B_K, EPI, UM>(gb, s, blockIdx.x - na, gxb, gyb, s.C, As, Bs, red);
 }
 
 // Split-K finish: out = epilogue(ab * sum over splits of the partial products), the
@@ -1667,6 +1696,57 @@ void launch_gemm(dim3 grid, const Gemm& g, const Step& s, hipStream_t st, int sl
   }
 }
 
+// gW1 (ga, grid ra) and gW2 (gb, grid rb) in one launch at k-tile BK, with the update rule
+// compiled in as launch_gemm_bk chooses it.
+template <int BK, int PREC, int EPI>
+void launch_pair_bk(Gemm ga, dim3 ra, Gemm gb, dim3 rb, const Step& s, hipStream_t st) {
+  ga.C = gb.C = (int)ra.z;
+  ga.splits = gb.splits = 1;
+  const dim3 grid(ra.x * ra.y * ra.z + rb.x * rb.y * rb.z);
+  if constexpr (EPI == EPI_UPDATE) {
+    if (s.mode == MLP_SGLD && s.rms) {
+      hipLaunchKernelGGL((gemm_pair_kernel<BK, PREC, false, false, EPI, UM_RMS>), grid, dim3(GT),
+                         0, st, ga, gb, s, ra.x, ra.y, rb.x, rb.y);
+      return;
+    }
+    if (s.mode == MLP_OBABO_BA) {
+      hipLaunchKernelGGL((gemm_pair_kernel<BK, PREC, false, false, EPI, UM_OBA>), grid, dim3(GT),
+                         0, st, ga, gb, s, ra.x, ra.y, rb.x, rb.y);
+      return;
+    }
+    if (s.mode == MLP_OBABO_BO) {
+      hipLaunchKernelGGL((gemm_pair_kernel<BK, PREC, false, false, EPI, UM_OBO>), grid, dim3(GT),
+                         0, st, ga, gb, s, ra.x, ra.y, rb.x, rb.y);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((gemm_pair_kernel<BK, PREC, false, false, EPI>), grid, dim3(GT), 0, st, ga,
+                     gb, s, ra.x, ra.y, rb.x, rb.y);
+}
+
+// The parameter GEMMs of an evaluation: gW2 (g4) and gW1 (g5) in one launch when their
+// plans agree (the same k-tile; they never split), else one launch each.
+template <int EPI>
+void launch_param_gemms(const Gemm& g4, dim3 r4, const Gemm& g5, dim3 r5, const Step& s,
+                        hipStream_t st) {
+  const int t4 = s.plan[SLOT_G4], t5 = s.plan[SLOT_G5];
+  if (t4 != t5 || (t5 & 15) != 1) {
+    launch_gemm<false, false, EPI>(r4, g4, s, st, SLOT_G4);
+    launch_gemm<false, false, EPI>(r5, g5, s, st, SLOT_G5);
+    return;
+  }
+  const int bk = t5 >> 4;
+  if (s.fp32) {
+    launch_pair_bk<32, P_F32, EPI>(g5, r5, g4, r4, s, st);
+  } else if (s.fp8) {
+    if (bk == 128) launch_pair_bk<128, P_FP8, EPI>(g5, r5, g4, r4, s, st);
+    else launch_pair_bk<64, P_FP8, EPI>(g5, r5, g4, r4, s, st);
+  } else {
+    if (bk == 128) launch_pair_bk<128, P_BF16, EPI>(g5, r5, g4, r4, s, st);
+    else launch_pair_bk<32, P_BF16, EPI>(g5, r5, g4, r4, s, st);
+  }
+}
+
 // Checks tune[] against the shapes: each nonzero entry is 16 k + s with k a k-tile of
 // the precision (bf16 32 / 128, e4m3 64 / 128, float32 32), s in {1, 2, 4} (s > 1 for
 // the forward GEMMs G1-G3 only) and the GEMM's K a multiple of k s.
@@ -1904,13 +1984,9 @@ int sgmc_mlp_step(const sgmc_mlp_state* st, int update, void* stream) {
           nullptr, 0, 0, nullptr, 0, nullptr, 0, 0, 0, L.g4_blocks};
   g4 = with_q(g4, Q_H1, Q_DA2, -1);
   g5 = with_q(g5, Q_X, Q_DA1, -1);
-  if (update) {
-    launch_gemm<false, false, EPI_UPDATE>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_, SLOT_G4);
-    launch_gemm<false, false, EPI_UPDATE>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_, SLOT_G5);
-  } else {
-    launch_gemm<false, false, EPI_GRAD>(dim3(q.H1 / BM, q.H2 / BN, C), g4, s, st_, SLOT_G4);
-    launch_gemm<false, false, EPI_GRAD>(dim3(q.D_in / BM, q.H1 / BN, C), g5, s, st_, SLOT_G5);
-  }
+  const dim3 r4(q.H1 / BM, q.H2 / BN, C), r5(q.D_in / BM, q.H1 / BN, C);
+  if (update) launch_param_gemms<EPI_UPDATE>(g4, r4, g5, r5, s, st_);
+  else launch_param_gemms<EPI_GRAD>(g4, r4, g5, r5, s, st_);
   const dim3 col_grid((q.H2 + GT - 1) / GT + (q.H1 + GT - 1) / GT + 1, n / kStrip, C);
   if (s.fp32) hipLaunchKernelGGL(colsum_kernel<P_F32>, col_grid, dim3(GT), 0, st_, s);
   else if (s.fp8) hipLaunchKernelGGL(colsum_kernel<P_FP8>, col_grid, dim3(GT), 0, st_, s);

@@ -263,12 +263,14 @@ class FusedMLP:
     """Chooses the k-tile depth and split-K factor of each GEMM by timing the live
     problem (HIP events on the launch stream around `reps` back-to-back units after one
     warm-up unit): whole iterations for SGLD, one OBABO step (two evaluations, the unit
-    OBABO and SGGMC iterations repeat) otherwise. One GEMM at a time, starting from the
+    OBABO and SGGMC iterations repeat) otherwise. The forward GEMMs one at a time, then the
+    two parameter GEMMs jointly (they share a launch at equal depths), starting from the
     built-in choice; a candidate replaces the current one only if it is faster by more
     than `min_gain`. Each timing starts from a snapshot of the engine's state (the
     iteration counter runs at most reps + 1 past it; the schedule tables clamp it),
     which is restored at the end, so the chains continue where they were. Sets
-    self.tune (and self.tune_ms: the time per unit of every plan tried, self.tune_gain:
+    self.tune (and self.tune_ms: the time per unit of every plan tried, keyed (slot, code)
+    and (3, gW2 code, gW1 code), self.tune_gain:
     (built-in ms, tuned ms)) and returns gemm_plan()."""
     unit = self.iteration if self.integrator == "sgld" else self.obabo_step
     tensors = self._mutable()
@@ -294,7 +296,7 @@ class FusedMLP:
     self.tune_ms = {}
     timed()  # the first timing after other work runs slow by a few percent
     best = builtin = timed()
-    for slot in range(5):
+    for slot in range(3):
       best_code = self.tune[slot]
       self.tune_ms[(slot, best_code)] = best
       for code in self.tile_candidates(slot):
@@ -306,6 +308,20 @@ class FusedMLP:
         if t < best * (1.0 - min_gain):
           best, best_code = t, code
       self.tune[slot] = best_code
+    # the parameter GEMMs jointly: gW2 runs in gW1's launch when both take the same
+    # k-tile, so their depths are not independent (tune_ms[(3, c4, c5)])
+    best_pair = (self.tune[3], self.tune[4])
+    self.tune_ms[(3,) + best_pair] = best
+    for c4 in self.tile_candidates(3):
+      for c5 in self.tile_candidates(4):
+        if (c4, c5) == best_pair:
+          continue
+        self.tune[3], self.tune[4] = c4, c5
+        t = timed()
+        self.tune_ms[(3, c4, c5)] = t
+        if t < best * (1.0 - min_gain):
+          best, best_pair = t, (c4, c5)
+    self.tune[3], self.tune[4] = best_pair
     self.tune_gain = (builtin, best)
     restore()
     torch.cuda.synchronize()
