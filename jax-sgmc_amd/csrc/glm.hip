@@ -1105,11 +1105,26 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
       if (lane < LPR && lane < D4) reinterpret_cast<float4*>(lds_cols + wv * 4 * D4)[lane] = acc[0];
       if (lane == 0) lds_small[wv] = lw;
     }
+    // the owners' terms that do not need the column sums, before the barrier: the prior
+    // gradient and log density, the noise term
+    float lp = 0.0f, pg[2] = {0.0f, 0.0f}, sn[2] = {0.0f, 0.0f};
+    if (own) {
+      const float nscale = sqrtf((2.0f * Tn) * eps);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h && !two) break;
+        const float wj = w[h];
+        if (a.pr.has_prior) {
+          pg[h] = wj * a.pr.inv_var;
+          lp += (a.pr.log_norm + (wj * wj) * a.pr.inv_var) / -2.0f;
+        }
+        sn[h] = nscale * xis[64 * par + 2 * jp + h];
+      }
+    }
     lds_sync();
     float ls = 0.0f;
 #pragma unroll
     for (int i = 0; i < kWaves; ++i) ls += lds_small[i];
-    float lp = 0.0f;
     if (own) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1121,21 +1136,17 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
         const float wj = w[h];
         // one published record: the last block's sum over records starts from +0
         float g = c * (0.0f + sc);
-        if (a.pr.has_prior) {
-          g = g + wj * a.pr.inv_var;
-          lp += (a.pr.log_norm + (wj * wj) * a.pr.inv_var) / -2.0f;
-        }
-        g = g / a.pot_temperature;
+        if (a.pr.has_prior) g = g + pg[h];
+        if (a.pot_temperature != 1.0f) g = g / a.pot_temperature;  // x / 1 is x
         const float sg = (-eps) * g;
-        const float sn = sqrtf((2.0f * Tn) * eps) * xis[64 * par + 2 * jp + h];
         float wn;
         if (rms) {
           const float vn = a.alpha * v[h] + (1.0f - a.alpha) * (g * g);
           v[h] = vn;
           const float Gs = 1.0f / (a.lmbd + sqrtf(vn));
-          wn = wj + ((0.0f + Gs * sg) + sqrtf(Gs) * sn);
+          wn = wj + ((0.0f + Gs * sg) + sqrtf(Gs) * sn[h]);
         } else {
-          wn = wj + (sg + sn);
+          wn = wj + (sg + sn[h]);
         }
         w[h] = wn;
         th_lds[j] = wn;
