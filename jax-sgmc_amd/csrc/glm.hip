@@ -1223,13 +1223,15 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
   // SGLD chains of D <= 64 and minibatches of at most 256 rows without per-row
   // likelihood output run resident (block-uniform choice)
   bool resident = false;
-  if constexpr (DCLASS == 0 || DCLASS == 1)
+  constexpr bool kResident = DCLASS == 0 || DCLASS == 5 || DCLASS == 1;
+  if constexpr (kResident)
     resident = (mode == SGMC_MODE_SGLD || mode == SGMC_MODE_SGLD_RMS) && !ch.lik &&
                ch.n <= kWaves * kWave;
   if (resident) {
-    if constexpr (DCLASS == 0 || DCLASS == 1)
-      resident_loop<DCLASS == 0 ? 4 : 16, MODE>(a, ch, iterations, lds_cols, lds_small, lds_lik,
-                                                res_th, res_eps, res_slot, res_kp, res_xi);
+    if constexpr (kResident)
+      resident_loop<DCLASS == 0 ? 4 : DCLASS == 5 ? 8 : 16, MODE>(
+          a, ch, iterations, lds_cols, lds_small, lds_lik, res_th, res_eps, res_slot, res_kp,
+          res_xi);
   }
   for (int t = 0; t < (resident ? 0 : iterations); ++t) {
     b.iter_offset = t;
@@ -1238,6 +1240,7 @@ __global__ void __launch_bounds__(kNT) glm_loop_kernel(FusedArgs a, int iteratio
       switch (dc) {
         case 0: fused_body<4, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
         case 1: fused_body<16, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
+        case 5: fused_body<8, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
         case 2: fused_body<64, 1, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
         case 3: fused_body<64, 2, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
         default: fused_body<64, 4, MODE>(b, ch, 0, lds_cols, lds_small, lds_lik, &flag); break;
@@ -1274,6 +1277,7 @@ __global__ void __launch_bounds__(kNT) glm_fused_kernel(FusedArgs a) {
   switch (dc) {
     case 0: fused_body<4, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
     case 1: fused_body<16, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
+    case 5: fused_body<8, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
     case 2: fused_body<64, 1, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
     case 3: fused_body<64, 2, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
     default: fused_body<64, 4, MODE>(a, ch, bchain, lds_cols, lds_small, lds_lik, &flag); break;
@@ -1291,6 +1295,7 @@ void launch_fused(int dclass, int nblocks, hipStream_t s, const FusedArgs& a) {
     case 2: hipLaunchKernelGGL((glm_fused_kernel<2, MODE>), grid, block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((glm_fused_kernel<3, MODE>), grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL((glm_fused_kernel<4, MODE>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((glm_fused_kernel<5, MODE>), grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL((glm_fused_kernel<-1, MODE>), grid, block, 0, s, a); break;
   }
 }
@@ -1304,6 +1309,7 @@ void launch_loop(int dclass, int nchains, int iterations, hipStream_t s, const F
     case 2: hipLaunchKernelGGL((glm_loop_kernel<2, MODE>), grid, block, 0, s, a, iterations); break;
     case 3: hipLaunchKernelGGL((glm_loop_kernel<3, MODE>), grid, block, 0, s, a, iterations); break;
     case 4: hipLaunchKernelGGL((glm_loop_kernel<4, MODE>), grid, block, 0, s, a, iterations); break;
+    case 5: hipLaunchKernelGGL((glm_loop_kernel<5, MODE>), grid, block, 0, s, a, iterations); break;
     default:
       hipLaunchKernelGGL((glm_loop_kernel<-1, MODE>), grid, block, 0, s, a, iterations);
       break;
@@ -1312,6 +1318,7 @@ void launch_loop(int dclass, int nchains, int iterations, hipStream_t s, const F
 
 static int dclass_of(int D) {
   if (D <= 16) return 0;
+  if (D <= 32) return 5;  // 8 lanes per row (added after 0-4: the numbers are labels)
   if (D <= 64) return 1;
   if (D <= 256) return 2;
   if (D <= 512) return 3;
@@ -1367,6 +1374,7 @@ int launch_potential(const char* who, int family, const float* X, int64_t ldx, c
   switch (dclass_of(D)) {
     case 0: hipLaunchKernelGGL((glm_potential_kernel<4, 1>), grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL((glm_potential_kernel<16, 1>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((glm_potential_kernel<8, 1>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((glm_potential_kernel<64, 1>), grid, block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((glm_potential_kernel<64, 2>), grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL((glm_potential_kernel<64, 4>), grid, block, 0, s, a); break;
@@ -1520,6 +1528,7 @@ int sgmc_glm_full_potential(int family, const float* X, int64_t ldx, const float
   switch (dclass_of(D)) {
     case 0: hipLaunchKernelGGL((glm_full_kernel<4, 1>), grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL((glm_full_kernel<16, 1>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((glm_full_kernel<8, 1>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((glm_full_kernel<64, 1>), grid, block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((glm_full_kernel<64, 2>), grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL((glm_full_kernel<64, 4>), grid, block, 0, s, a); break;

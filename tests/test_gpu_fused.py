@@ -349,17 +349,20 @@ def test_sparse_exchange_graph_recaptured_after_eager_iterations():
 _LOOP_SHAPES = {
     # mixed D classes: the generic loop body
     "mixed": [(500, 8, 32), (800, 33, 64), (300, 100, 17)],
-    # D <= 16 and D <= 64 tables: SGLD chains run resident (minibatches up to 256 rows;
-    # the 300-row chain of the second table takes the generic body in the same launch)
+    # D <= 16, D <= 32 and D <= 64 tables: SGLD chains run resident (minibatches up to
+    # 256 rows; the 300-row chains take the generic body in the same launch)
     "d16": [(500, 8, 32), (700, 16, 256), (300, 3, 17)],
-    "d64": [(800, 33, 64), (600, 64, 200), (900, 20, 300)],
+    "d32": [(600, 17, 128), (700, 32, 256), (900, 24, 300), (400, 29, 40)],
+    "d64": [(800, 33, 64), (600, 64, 200), (900, 40, 300)],
 }
 
 
 @pytest.mark.parametrize("mode,shapes,family", [
     ("sgld", "mixed", "logistic"), ("sgld_rms", "mixed", "logistic"),
     ("sghmc", "mixed", "logistic"), ("sgld", "d16", "logistic"),
-    ("sgld_rms", "d16", "linear_gaussian"), ("sgld", "d64", "linear_gaussian"),
+    ("sgld_rms", "d16", "linear_gaussian"), ("sgld", "d32", "linear_gaussian"),
+    ("sgld_rms", "d32", "logistic"), ("sghmc", "d32", "linear_gaussian"),
+    ("sgld", "d64", "linear_gaussian"),
     ("sgld_rms", "d64", "logistic"), ("sghmc", "d64", "logistic")])
 def test_loop_launch_equals_per_evaluation_launches(mode, shapes, family):
   """Single-block chains: many iterations in one loop launch give bit for bit the
