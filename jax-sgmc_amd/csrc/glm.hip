@@ -966,14 +966,17 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   src.magic = randint_magic(src.span);
   src.rows = ch.N;
   RG buf[NG];
-  // The theta-independent key work runs ahead of its use, one job per wavefront, handed
-  // over through LDS (kp, xis; slot = iteration parity) across the loop's barriers:
-  // - wavefront 0: the data-key chain d' = split(d)[0], (k1, k2) = split(split(d)[1]),
-  //   two iterations ahead (kp[4 + 4 slot ..]);
-  // - wavefront 1: the noise-key chain key' = split(key)[0], leaf = split_key(
-  //   split(key)[1], 1, 0), two iterations ahead (kp[2 slot ..]);
-  // - wavefronts 2 and 3: the normals of noise pair jp (words 0 and 1 of its Threefry
-  //   block) one iteration ahead (xis[64 slot + 2 jp + h]).
+  // The theta-independent key work runs ahead of its use, after the reduction barrier
+  // (beside wavefront 0's update), handed over through LDS (kp, xis; slot = iteration
+  // parity) across the loop's barriers. With key_{i+1}, ks_i = split(key_i), leaf_i =
+  // split_key(ks_i, 1, 0) (the noise of iteration i), d_{i+1}, dsub_i = split(d_i) and
+  // (k1_i, k2_i) = split(dsub_i) (the minibatch of iteration i), at iteration t:
+  // - wavefront 3 runs one Threefry block per lane for all key steps at once: lanes 0-1
+  //   split(key_{t+3}), lane 2 split_key(ks_{t+2}), lanes 3-4 split(d_{t+3}), lanes 5-6
+  //   split(dsub_{t+2}); leaf_{t+2} goes to kp[2 slot ..], (k1, k2)_{t+2} to
+  //   kp[4 + 4 slot ..], the chain states stay in its registers;
+  // - wavefront 2 draws the normals of iteration t + 1 (lane l: word l / 32 of noise
+  //   pair l % 32) into xis[64 slot' + 2 jp + h].
   // kp[12 ..]: the keys after the launch's last iteration.
   auto put_key = [&](int at, Key k) { kp[at] = k.k0; kp[at + 1] = k.k1; };
   auto get_key = [&](int at) { return Key{kp[at], kp[at + 1]}; };
@@ -1002,25 +1005,28 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
       if (gi < ng) load_group<LPR, CPL>(to_global(ch.X), ch.ldx, to_global(ch.y), D4, gi * G, rid,
                                         buf[gi]);
   };
-  Key kc, dc;  // wavefront 1: key_{t+2}; wavefront 0: d_{t+2} (at iteration t)
+  // wavefront 3's chain states at iteration t: key_{t+3}, ks_{t+2}, d_{t+3}, dsub_{t+2}
+  Key kc, ksn, dc, dsn;
   {
-    // keys of iterations 0 and 1 (every thread; uniform values)
-    Key key1, d1, ks, dsub, k1, k2;
-    split2_uniform(load_key(ch.key), key1, ks);
-    const Key leaf0 = split_key(ks, 1, 0);
-    split2_uniform(key1, kc, ks);
-    const Key leaf1 = split_key(ks, 1, 0);
-    split2_uniform(load_key(ch.dkey), d1, dsub);
-    split2_uniform(dsub, k1, k2);
-    Key k1n, k2n;
-    split2_uniform(d1, dc, dsub);
-    split2_uniform(dsub, k1n, k2n);
+    // keys of iterations 0 to 2 (every thread; uniform values)
+    Key key1, key2, ks0, ks1, d1, d2, ds0, ds1, k1, k2, k1n, k2n;
+    split2_uniform(load_key(ch.key), key1, ks0);
+    split2_uniform(key1, key2, ks1);
+    split2_uniform(key2, kc, ksn);
+    const Key leaf0 = split_key(ks0, 1, 0);
+    const Key leaf1 = split_key(ks1, 1, 0);
+    split2_uniform(load_key(ch.dkey), d1, ds0);
+    split2_uniform(d1, d2, ds1);
+    split2_uniform(d2, dc, dsn);
+    split2_uniform(ds0, k1, k2);
+    split2_uniform(ds1, k1n, k2n);
     if (threadIdx.x == 0) {
       put_key(2, leaf1);
       put_key(8, k1n);
       put_key(10, k2n);
-      put_key(12, key1);  // the keys after one iteration (later ones overwrite)
-      put_key(14, d1);
+      // the keys after the launch (iterations >= 3: set again in the loop)
+      put_key(12, iterations == 1 ? key1 : iterations == 2 ? key2 : kc);
+      put_key(14, iterations == 1 ? d1 : iterations == 2 ? d2 : dc);
     }
     if (own) {  // the normals of iteration 0
       uint32_t wb0, wb1;
@@ -1052,32 +1058,6 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
     const float eps = eps_lds[tw];
     const int slot = slot_lds[tw];
     const int par = t & 1;
-    if (wv == 0) {  // (k1, k2) of iteration t + 2, and d_{t+3}
-      if (t + 2 == iterations && lane == 0) put_key(14, dc);
-      Key dn, dsub, k1, k2;
-      split2_uniform(dc, dn, dsub);
-      split2_uniform(dsub, k1, k2);
-      if (lane == 0) {
-        put_key(4 + 4 * par, k1);
-        put_key(6 + 4 * par, k2);
-      }
-      dc = dn;
-    } else if (wv == 1) {  // leaf of iteration t + 2, and key_{t+3}
-      if (t + 2 == iterations && lane == 0) put_key(12, kc);
-      Key kn, ks;
-      split2_uniform(kc, kn, ks);
-      const Key lf = split_key(ks, 1, 0);
-      if (lane == 0) put_key(2 * par, lf);
-      kc = kn;
-    } else if (t + 1 < iterations) {  // normals of iteration t + 1 (word wv - 2 of each pair)
-      const int h = wv - 2;
-      const uint32_t q = (uint32_t)lane;
-      if (q < m && (h == 0 || q + m < S)) {
-        uint32_t wb0, wb1;
-        bits_pair(get_key(2 * (par ^ 1)), S, q, wb0, wb1);
-        xis[64 * (par ^ 1) + 2 * q + h] = bits_to_normal(h ? wb1 : wb0);
-      }
-    }
     float4 th[CPL];
     {
       const int q = (threadIdx.x & (kWave - 1)) % LPR;
@@ -1122,6 +1102,47 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
       }
     }
     lds_sync();
+    if (wv == 3) {  // every key step of this iteration in one Threefry block per lane
+      Key k = kc;
+      uint32_t x0 = lane & 1, x1 = (lane & 1) + 2u;  // lanes 0, 1: split(key_{t+3})
+      if (lane == 2) {  // split_key(ks_{t+2}, 1, 0)
+        k = ksn;
+        x0 = 0u;
+        x1 = 1u;
+      } else if (lane == 3 || lane == 4) {  // split(d_{t+3})
+        k = dc;
+        x0 = lane - 3;
+        x1 = lane - 1;
+      } else if (lane >= 5) {  // lanes 5, 6: split(dsub_{t+2})
+        k = dsn;
+        x0 = (lane - 5) & 1;
+        x1 = x0 + 2u;
+      }
+      threefry2x32(k.k0, k.k1, x0, x1);
+      auto rl = [](uint32_t v, int src) {
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
+      };
+      if (t + 3 == iterations && lane == 0) {
+        put_key(12, kc);
+        put_key(14, dc);
+      }
+      if (lane == 0) {
+        put_key(2 * par, Key{rl(x0, 2), rl(x1, 2)});
+        put_key(4 + 4 * par, Key{rl(x0, 5), rl(x0, 6)});
+        put_key(6 + 4 * par, Key{rl(x1, 5), rl(x1, 6)});
+      }
+      kc = Key{rl(x0, 0), rl(x0, 1)};
+      ksn = Key{rl(x1, 0), rl(x1, 1)};
+      dc = Key{rl(x0, 3), rl(x0, 4)};
+      dsn = Key{rl(x1, 3), rl(x1, 4)};
+    } else if (wv == 2 && t + 1 < iterations) {  // normals of iteration t + 1
+      const uint32_t q = (uint32_t)lane & 31u, h = (uint32_t)lane >> 5;
+      if (q < m && (h == 0 || q + m < S)) {
+        uint32_t wb0, wb1;
+        bits_pair(get_key(2 * (par ^ 1)), S, q, wb0, wb1);
+        xis[64 * (par ^ 1) + 2 * q + h] = bits_to_normal(h ? wb1 : wb0);
+      }
+    }
     float ls = 0.0f;
 #pragma unroll
     for (int i = 0; i < kWaves; ++i) ls += lds_small[i];

@@ -417,6 +417,44 @@ def test_loop_launch_equals_per_evaluation_launches(mode, shapes, family):
   np.testing.assert_array_equal(a.dkeys.cpu().numpy(), b.dkeys.cpu().numpy())
 
 
+@pytest.mark.parametrize("shapes", ["d16", "d32", "d64"])
+def test_short_loop_launches_equal_per_evaluation_launches(shapes):
+  """Loop launches of 1, 2, 3 and 4 iterations (the key pipeline's start and end
+  cases) give the chains, keys and saved samples of one launch per evaluation."""
+  print("short loop launches", shapes, flush=True)
+  rng = np.random.default_rng(5)
+  its = 10
+  eps = np.linspace(0.01, 0.004, its).astype(np.float32)
+  data = []
+  for c, (N, D, n) in enumerate(_LOOP_SHAPES[shapes]):
+    X = (rng.standard_normal((N, D)) / np.sqrt(D)).astype(np.float32)
+    y = (rng.random(N) < 0.5).astype(np.float32)
+    data.append((X, y, D, n))
+  engines = []
+  for _ in range(2):
+    spec = []
+    for c, (X, y, D, n) in enumerate(data):
+      Xp = np.zeros((X.shape[0], (D + 3) // 4 * 4), np.float32)
+      Xp[:, :D] = X
+      spec.append(dict(X=torch.as_tensor(Xp, device=DEV), y=torch.as_tensor(y, device=DEV), D=D,
+                       n=n, theta0=np.zeros(D, np.float32), key=(3, c), dkey=(4, c)))
+    eng = fused.FusedGLM(spec, "logistic", "sgld", eps, save_slots=np.arange(its, dtype=np.int32),
+                         n_slots=its, target_bytes=1 << 30)
+    assert eng.loop_capable()
+    engines.append(eng)
+  a, b = engines
+  a.run(its, loop=False)
+  for k in (1, 2, 3, 4):
+    b.run(k, loop=True)
+  torch.cuda.synchronize()
+  assert a.iteration_count() == b.iteration_count() == its
+  assert torch.equal(a.theta_flat, b.theta_flat) and torch.equal(a.U, b.U)
+  for c in range(len(data)):
+    np.testing.assert_array_equal(a.sample_trace(c).cpu().numpy(), b.sample_trace(c).cpu().numpy())
+  np.testing.assert_array_equal(a.keys.cpu().numpy(), b.keys.cpu().numpy())
+  np.testing.assert_array_equal(a.dkeys.cpu().numpy(), b.dkeys.cpu().numpy())
+
+
 @pytest.mark.parametrize("tables,graph", [(2, 0), (3, 4)])
 def test_table_group_equals_one_table(tables, graph):
   """Chains split over tables on separate streams give bit for bit the single table's
