@@ -930,6 +930,7 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
   using RG = RowGroup<LPR, CPL>;
   constexpr int G = (kWave / LPR) * RG::U;  // rows per group
   constexpr int NG = kWave / G;             // groups of a 64-row chunk
+  constexpr int kSegs = kWave / LPR;         // row segments of a wavefront
   const int mode = MODE >= 0 ? MODE : a.mode;
   const bool rms = mode == SGMC_MODE_SGLD_RMS;
   const int D = ch.D, D4 = (D + 3) >> 2, n = ch.n;
@@ -1070,20 +1071,17 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
     for (int gi = 0; gi < NG; ++gi)
       if (gi < ng) process_group<LPR, CPL>(buf[gi], r0 + gi * G, D4, th, a.L, acc, lik_acc, lds_lik, 0);
     if (t + 1 < iterations) draw_rows(get_key(4 + 4 * (par ^ 1)), get_key(6 + 4 * (par ^ 1)));
-    // block_reduce's wavefront part: segment sums to the first LPR lanes, the wave's
-    // column partials and lik sum to LDS; one barrier, then each owner thread adds its
-    // two columns over the wavefronts in block_reduce's order
+    // Every lane's column partials go to LDS unreduced (one barrier); each owner thread
+    // then adds its two columns over the row segments in the order of block_reduce's
+    // segment shuffles (pairs of segments, pairs of pairs, ...) and over the wavefronts
+    // in block_reduce's order. The lik sum is only formed where U is kept.
+    const bool want_u = (slot >= 0 && ch.samples_U) || t == iterations - 1;
     {
-#pragma unroll
-      for (int mm = LPR; mm < kWave; mm <<= 1) {
-        acc[0].x += __shfl_xor(acc[0].x, mm, kWave);
-        acc[0].y += __shfl_xor(acc[0].y, mm, kWave);
-        acc[0].z += __shfl_xor(acc[0].z, mm, kWave);
-        acc[0].w += __shfl_xor(acc[0].w, mm, kWave);
+      reinterpret_cast<float4*>(lds_cols)[wv * kWave + lane] = acc[0];
+      if (want_u) {
+        const float lw = wave_sum(lik_acc);
+        if (lane == 0) lds_small[wv] = lw;
       }
-      const float lw = wave_sum(lik_acc);
-      if (lane < LPR && lane < D4) reinterpret_cast<float4*>(lds_cols + wv * 4 * D4)[lane] = acc[0];
-      if (lane == 0) lds_small[wv] = lw;
     }
     // the owners' terms that do not need the column sums, before the barrier: the prior
     // gradient and log density, the noise term
@@ -1143,9 +1141,6 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
         xis[64 * (par ^ 1) + 2 * q + h] = bits_to_normal(h ? wb1 : wb0);
       }
     }
-    float ls = 0.0f;
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) ls += lds_small[i];
     if (own) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1153,7 +1148,16 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
         const uint32_t j = jj[h];
         float sc = 0.0f;
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) sc += lds_cols[ww * 4 * D4 + j];
+        for (int ww = 0; ww < kWaves; ++ww) {
+          float sv[kSegs];
+#pragma unroll
+          for (int sg = 0; sg < kSegs; ++sg) sv[sg] = lds_cols[(ww * kWave + sg * LPR) * 4 + j];
+#pragma unroll
+          for (int w2 = 1; w2 < kSegs; w2 <<= 1)
+#pragma unroll
+            for (int i = 0; i < kSegs; i += 2 * w2) sv[i] = sv[i] + sv[i + w2];
+          sc += sv[0];
+        }
         const float wj = w[h];
         // one published record: the last block's sum over records starts from +0
         float g = c * (0.0f + sc);
@@ -1176,7 +1180,10 @@ __device__ __forceinline__ void resident_loop(const FusedArgs& a, const sgmc_glm
     }
     // U (and var(lik)) only where they are kept: a save slot or the last iteration
     const bool last = t == iterations - 1;
-    if ((slot >= 0 && ch.samples_U) || last) {
+    if (want_u) {
+      float ls = 0.0f;  // read before block_sum's barrier lets lds_small be reused
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) ls += lds_small[i];
       lp = block_sum<kNT, true>(lp, lds_small);
       // the block's lik sum is the only record: the sum over records is ls + 0
       const float mean = (ls + 0.0f) / (float)n;
