@@ -98,11 +98,12 @@ __device__ __forceinline__ float seg_sum(float v) {
   return v;
 }
 
-// Rows in flight per segment per pipeline stage: 4 (2 for D > 512); a stage of RPW-row
-// loads never exceeds one 64-row chunk.
+// Rows in flight per segment per pipeline stage: 4, also for D > 512 (eight 4 KB rows in
+// flight per wavefront there; 2 rows per stage left the row loop latency-bound); a stage
+// of RPW-row loads never exceeds one 64-row chunk.
 template <int LPR, int CPL>
 struct Unroll {
-  static constexpr int U0 = CPL >= 4 ? 2 : 4;
+  static constexpr int U0 = 4;
   static constexpr int U = U0 < LPR ? U0 : LPR;
 };
 
