@@ -291,6 +291,9 @@ typedef struct sgmc_glm_chain {
   float* mom_old;      /* [D] AMAGOLD, SGGMC: momentum at the iteration's start, or NULL */
 } sgmc_glm_chain;
 
+/* Row-width class of a chain of D features (the kernel instantiation that reads its rows:
+ * lanes per row and float4 chunks per lane). A label, not an order: compare classes for
+ * equality only. No reference counterpart. */
 int sgmc_glm_dclass(int D);
 /* Host-side check of a chain table before it is uploaded (no reference counterpart:
  * solver.py:162-171 runs chains one by one and has no table). chains and block_chain
