@@ -1748,7 +1748,7 @@ void launch_param_gemms(const Gemm& g4, dim3 r4, const Gemm& g5, dim3 r5, const 
 }
 
 // Checks tune[] against the shapes: each nonzero entry is 16 k + s with k a k-tile of
-// the precision (bf16 32 / 128, e4m3 64 / 128, float32 32), s in {1, 2, 4} (s > 1 for
+// the precision (bf16 32 / 128, e4m3 64 / 128, float32 32), s in {1, 2, 3, 4} (s > 1 for
 // the forward GEMMs G1-G3 only) and the GEMM's K a multiple of k s.
 int check_tune(int D_in, int H1, int H2, int n, int precision, const int32_t* tune) {
   if (!tune) return SGMC_OK;
@@ -1759,7 +1759,7 @@ int check_tune(int D_in, int H1, int H2, int n, int precision, const int32_t* tu
     const int bk = tv >> 4, sp = tv & 15;
     const bool bk_ok = precision == 2 ? bk == 32 : precision == 1 ? (bk == 64 || bk == 128)
                                                                 : (bk == 32 || bk == 128);
-    SGMC_REQUIRE(bk_ok && (sp == 1 || sp == 2 || sp == 4) && (sp == 1 || i < SLOT_G4) &&
+    SGMC_REQUIRE(bk_ok && sp >= 1 && sp <= 4 && (sp == 1 || i < SLOT_G4) &&
                      K[i] % (bk * sp) == 0,
                  SGMC_E_ARG, "sgmc_mlp: tune[%d] = %d (k-tile %d, split %d) does not fit K = %d",
                  i, tv, bk, sp, K[i]);

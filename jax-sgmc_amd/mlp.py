@@ -250,7 +250,7 @@ class FusedMLP:
     D_in, H1, H2 = self.dims
     K = (D_in, H1, H2, self.n, self.n)[slot]
     depths = {"fp32": (32,), "fp8": (64, 128), "bf16": (32, 128)}[self.precision]
-    splits = (1, 2, 4) if slot < 3 else (1,)
+    splits = (1, 2, 3, 4) if slot < 3 else (1,)
     return [k * 16 + s for k in depths for s in splits if K % (k * s) == 0]
 
   def _mutable(self):

@@ -45,7 +45,7 @@ def test_tuned_entries_override_only_their_gemm():
     ("fp8", [32 * 16 + 1, 0, 0, 0, 0], "k-tile 32"),         # e4m3 takes 64 or 128
     ("fp32", [128 * 16 + 1, 0, 0, 0, 0], "k-tile 128"),      # float32 takes 32
     ("bf16", [0, 0, 0, 32 * 16 + 2, 0], "split 2"),          # parameter GEMMs never split
-    ("bf16", [32 * 16 + 3, 0, 0, 0, 0], "split 3"),
+    ("bf16", [32 * 16 + 5, 0, 0, 0, 0], "split 5"),
     ("bf16", [0, 128 * 16 + 8, 0, 0, 0], "split 8"),
 ])
 def test_rejects_tunings_that_do_not_fit(prec, tune, what):

@@ -22,16 +22,16 @@ DIMS, N, n, C = (256, 256, 256), 2048, 128, 2
 SPLIT_TOL = {"fp32": 1e-5, "bf16": 2e-3, "fp8": 2e-2}
 
 
-def engine(precision, integrator="sgld", rms=True):
+def engine(precision, integrator="sgld", rms=True, dims=DIMS):
   rng = np.random.default_rng(11)
-  X = torch.as_tensor(rng.random((N, DIMS[0]), dtype=np.float32), device=DEV)
+  X = torch.as_tensor(rng.random((N, dims[0]), dtype=np.float32), device=DEV)
   labels = torch.as_tensor(rng.integers(0, 10, N).astype(np.int32), device=DEV)
-  th0 = np.stack([M.init_params(40 + c, *DIMS) for c in range(C)])
+  th0 = np.stack([M.init_params(40 + c, *dims) for c in range(C)])
   keys = np.array([[0, 3 + c] for c in range(C)], np.uint32)
   dkeys = np.array([[1, 8 + c] for c in range(C)], np.uint32)
   eps = np.full(16, 1e-4, np.float32)
   kw = dict(rms=rms) if integrator == "sgld" else dict(integrator=integrator, steps=2)
-  return M.FusedMLP(X, labels, DIMS, n, th0, eps, keys=keys, dkeys=dkeys,
+  return M.FusedMLP(X, labels, dims, n, th0, eps, keys=keys, dkeys=dkeys,
                     precision=precision, save_slots=np.arange(16), **kw)
 
 
@@ -44,10 +44,11 @@ def restore(eng, snap):
     t.copy_(s)
 
 
+@pytest.mark.parametrize("dims", [DIMS, (384, 256, 256)])  # 384 = 3 x 128: a three-way split
 @pytest.mark.parametrize("precision", ["bf16", "fp8", "fp32"])
-def test_tile_depth_bit_equal_and_split_within_rounding(precision):
-  print("MLP GEMM tilings, gradient mode:", precision, flush=True)
-  eng = engine(precision)
+def test_tile_depth_bit_equal_and_split_within_rounding(precision, dims):
+  print("MLP GEMM tilings, gradient mode:", precision, dims, flush=True)
+  eng = engine(precision, dims=dims)
   eng.grad = torch.empty_like(eng.theta)  # before the snapshot: part of the state list
   snap = snapshot(eng)
   for slot in range(5):
