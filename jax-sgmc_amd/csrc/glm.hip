@@ -916,10 +916,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
 // For chains of D <= 4 LPR (one float4 chunk per lane of a row segment) and minibatches
 // of at most 256 rows (one 64-row chunk per wavefront), the chain's state stays on chip
 // for the whole launch: theta in LDS (and its owner threads' registers), the keys in
-// registers, the wavefronts' column partials in LDS. The step's noise is drawn before
-// the rows are processed; the minibatch of the next evaluation (its indices depend only
-// on the data key) is drawn and its rows loaded right after this evaluation's row groups
-// are consumed, so their latency overlaps the reduction and the update. The arithmetic
+// registers, the wavefronts' column partials in LDS. The step's noise is drawn one
+// iteration ahead, beside the previous update; the minibatch of the next evaluation (its
+// indices depend only on the data key) is drawn and its rows loaded right after this
+// evaluation's row groups are consumed, so their latency overlaps the reduction and the
+// update. The arithmetic
 // is fused_body's for a chain of one block, in the same order (row groups, reductions,
 // the SGLD / rms_prop update, U, var), so both give the same bits.
 template <int LPR, int MODE>
