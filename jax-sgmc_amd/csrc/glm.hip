@@ -675,6 +675,21 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
   src.mult = randint_mult(src.span);
   src.magic = randint_magic(src.span);
   src.rows = ch.N;
+  // Noise key of this step (used by the chain's last block only; derived here, where
+  // it overlaps the wait for the evaluation point, instead of after the arrival).
+  // OBABO: (key', s1, s2) = split(key, 3) at both launches of a leapfrog step; the even
+  // one draws from s1, the odd one from s2 and stores key'.
+  Key knext, leaf;
+  if (ob) {
+    Key k0, s1, s2;
+    split3(key0, k0, s1, s2);
+    leaf = split_key(sub ? s2 : s1, 1, 0);
+    knext = sub ? k0 : key0;
+  } else {
+    Key ksub;
+    split2(key_noise, knext, ksub);
+    leaf = split_key(ksub, 1, 0);
+  }
 
   float4 acc[CPL];
 #pragma unroll
@@ -728,19 +743,6 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const sgmc_glm_ch
     if (ch.var) rec_m2 = consume(g_partial + (int64_t)threadIdx.x * stride + 4 * D4 + 1);
   }
 
-  // Noise key of this step. OBABO: (key', s1, s2) = split(key, 3) at both launches of
-  // a leapfrog step; the even one draws from s1, the odd one from s2 and stores key'.
-  Key knext, leaf;
-  if (ob) {
-    Key k0, s1, s2;
-    split3(key0, k0, s1, s2);
-    leaf = split_key(sub ? s2 : s1, 1, 0);
-    knext = sub ? k0 : key0;
-  } else {
-    Key ksub;
-    split2(key_noise, knext, ksub);
-    leaf = split_key(ksub, 1, 0);
-  }
   // AMAGOLD and SGGMC samples are saved after the Metropolis-Hastings test, not here.
   const int slot = (a.save_slot && !amg && !ob_mh) ? a.save_slot[it] : -1;
   const bool last_step = !has_mom || a.step == launches - 1;
