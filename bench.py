@@ -73,11 +73,13 @@ def chain_keys(R, C):
 
 
 # ---------------------------------------------------------------- CPU baseline ----
-# Protocol (bounded so that a default run stays within minutes):
-# one calibration call, a warm-up of up to 100 iterations (about warm_s seconds), then
-# the median of 5 timed runs of about run_s seconds each, on the host's CPU share
-# (OMP_NUM_THREADS, else os.cpu_count()). The C port used is the baseline build
-# (vectorized float32 sums with FMA contraction, AVX-512 where the host has it).
+# Protocol (100 warm-up iterations, then the median of 5 timed runs, bounded so that a
+# default run stays within minutes): one calibration call, a warm-up of up to 100
+# iterations (about warm_s seconds), further warm-up chunks while the time per iteration
+# is still falling (at most settle_s seconds), then the median of 5 timed runs of about
+# run_s seconds each, on the host's CPU share (OMP_NUM_THREADS, else os.cpu_count()).
+# The C port used is the baseline build (vectorized float32 sums with FMA contraction,
+# AVX-512 where the host has it).
 
 def host_info():
   model = "unknown"
@@ -100,22 +102,45 @@ def cpu_threads():
 
 
 def cpu_protocol(step, warm_s=3.0, run_s=3.0, runs=5, max_warm=100, min_iters=1,
-                 max_iters=1_000_000):
+                 max_iters=1_000_000, settle_s=10.0, chunk_s=1.0, settle_tol=0.03):
   """step(k) runs k more iterations of the workload. Returns the iterations per timed
-  run, the warm-up iterations and the median and all run times in seconds."""
-  t0 = time.perf_counter()
+  run, the warm-up iterations and seconds, and the median and all run times in seconds.
+
+  The warm-up runs chunks of about chunk_s seconds until it has run max_warm
+  iterations or warm_s seconds. After it, the host keeps running chunks of about chunk_s seconds while
+  a chunk's time per iteration is more than settle_tol below the previous chunk's (the
+  host's clocks and page placement are still settling), for at most settle_s seconds,
+  so that the timed runs measure the settled host and not its ramp."""
+  w0 = time.perf_counter()
   step(1)
-  t1 = max(time.perf_counter() - t0, 1e-6)
-  warm = int(min(max_warm, max(0, warm_s / t1)))
-  if warm:
-    step(warm)
-  per = int(min(max_iters, max(min_iters, run_s / t1)))
+  prev = max(time.perf_counter() - w0, 1e-6)
+  warm = 0
+  while warm < max_warm and time.perf_counter() - w0 < warm_s:
+    k = int(max(1, min(max_warm - warm, chunk_s / prev)))
+    t0 = time.perf_counter()
+    step(k)
+    prev = max((time.perf_counter() - t0) / k, 1e-6)
+    warm += k
+  settle = 0
+  while time.perf_counter() - w0 < warm_s + settle_s:
+    k = int(max(1, chunk_s / prev))
+    t0 = time.perf_counter()
+    step(k)
+    cur = max((time.perf_counter() - t0) / k, 1e-6)
+    settle += k
+    falling = cur < prev * (1.0 - settle_tol)
+    prev = cur
+    if not falling:
+      break
+  warm_total = time.perf_counter() - w0
+  per = int(min(max_iters, max(min_iters, run_s / prev)))
   times = []
   for _ in range(runs):
     t0 = time.perf_counter()
     step(per)
     times.append(time.perf_counter() - t0)
-  return {"warmup_iterations": 1 + warm, "iterations_per_run": per, "runs": runs,
+  return {"warmup_iterations": 1 + warm + settle, "warmup_s": round(warm_total, 2),
+          "iterations_per_run": per, "runs": runs,
           "median_s": float(np.median(times)), "run_s": [round(t, 4) for t in times]}
 
 
@@ -123,7 +148,8 @@ def baseline_line(chains_per_iter, prot, what, threads):
   """cpu_baseline object: chain-steps/s of the median run."""
   value = chains_per_iter * prot["iterations_per_run"] / prot["median_s"]
   return {"value": value, "unit": "chain-steps/s", "cores": threads, "kind": "port",
-          "sample": f"{what}; warm-up {prot['warmup_iterations']} iterations, then the median of "
+          "sample": f"{what}; warm-up {prot['warmup_iterations']} iterations ({prot['warmup_s']:.1f} s, "
+                    f"until the time per iteration stopped falling), then the median of "
                     f"{prot['runs']} timed runs of {prot['iterations_per_run']} iterations "
                     f"({prot['median_s']:.2f} s) on {threads} threads",
           "protocol": prot, "host": host_info()}
@@ -290,7 +316,7 @@ def cpu_baseline(X_host, y_host, keys, dkeys):
     cport.sghmc(1, X_host, y_host, BATCH, th, kk, dd, e, LEAPFROG, FRICTION, threads=threads,
                 fast=True)
 
-  prot = cpu_protocol(step, warm_s=3.0, run_s=3.0, max_warm=100)
+  prot = cpu_protocol(step, warm_s=10.0, run_s=3.0, max_warm=1_000_000)
   return baseline_line(C, prot, f"all {C} chains of the same workload (oracle/cport.c SGHMC "
                        "logistic, baseline build, OpenMP over chains)", threads)
 
