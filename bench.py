@@ -241,8 +241,9 @@ def laplace_logistic(torch, X, y, newton=10):
   formed on the GPU in float32 and solved in float64. At N = 1e6 rows and D = 512 the
   posterior is close to this Gaussian (Bernstein-von Mises)."""
   dev = X.device
-  w = torch.zeros(D, dtype=torch.float64, device=dev)
-  eye = torch.eye(D, dtype=torch.float64, device=dev)
+  Dx = X.shape[1]
+  w = torch.zeros(Dx, dtype=torch.float64, device=dev)
+  eye = torch.eye(Dx, dtype=torch.float64, device=dev)
   for _ in range(newton + 1):
     p = torch.sigmoid(X @ w.float())
     g = (X.T @ (y - p)).double() - w
@@ -352,7 +353,7 @@ LADDER_STREAMS = (0, 0, 1, 2)
 
 
 def ladder_tables(torch, fused, R, probs, eps, G, groups=LADDER_GROUPS,
-                  stream_of=LADDER_STREAMS):
+                  stream_of=LADDER_STREAMS, theta0s=None):
   """The 32 ladders as chain tables (fused.TableGroup), one per group of D classes, each
   table with its own replica exchange. Default: one table per D class, so each class's
   kernel is compiled for its row width only (fewer registers than the mixed-class
@@ -382,7 +383,8 @@ def ladder_tables(torch, fused, R, probs, eps, G, groups=LADDER_GROUPS,
         continue
       for j in range(LADDER_RUNGS):
         c = k * LADDER_RUNGS + j
-        chains.append(dict(X=X, y=y, D=D, n=LADDER_BATCH, theta0=np.zeros(D, np.float32),
+        th0 = np.zeros(D, np.float32) if theta0s is None else theta0s[k]
+        chains.append(dict(X=X, y=y, D=D, n=LADDER_BATCH, theta0=th0,
                            key=tuple(keys[c]), dkey=tuple(dkeys[c]), temperature=temps[j],
                            save=False))
         if j + 1 < LADDER_RUNGS:
@@ -398,6 +400,55 @@ def ladder_tables(torch, fused, R, probs, eps, G, groups=LADDER_GROUPS,
     engines.append(eng)
     dims.append(max(classes))
   return fused.TableGroup(engines, stream_of=list(stream_of)), exchanges, dims
+
+
+def ladder_posterior(torch, fused, R, probs, G, its=2000, burn=500):
+  """Posterior check of the cold (T = 1) rungs: each problem's Laplace approximation
+  (Newton MAP, inverse Hessian) is the reference. The ladders start at the MAPs and run
+  the workload's kernels, exchanges included, with the step sizes 1e-3 -> 1e-4 over
+  the check's iterations; each cold chain's state is read after every exchange block of
+  G iterations past the burn-in. Errors are in posterior standard deviations."""
+  laps = [laplace_logistic(torch, X, y) for X, y in probs]
+  from jax_sgmc_amd import scheduler
+  eps = scheduler.polynomial_table(1e-3, 1e-4, its + G)
+  grp, _, _ = ladder_tables(torch, fused, R, probs, eps, G,
+                            theta0s=[mu.float().cpu().numpy() for mu, _, _ in laps])
+  grp.capture(G)
+  # cold chain of problem k: position of rung 0 of k in its class's table
+  where = {}
+  for e, eng in enumerate(grp.engines):
+    ks = [k for k, (X, _) in enumerate(probs) if int(X.shape[1]) in
+          {int(d) for d in eng.D}]
+    for m, k in enumerate(ks):
+      where[k] = (e, m * LADDER_RUNGS)
+  snaps = [[] for _ in probs]
+  for b in range(its // G):
+    grp.run(G, G)
+    torch.cuda.synchronize()
+    if (b + 1) * G > burn:
+      for k, (e, c) in where.items():
+        snaps[k].append(grp.engines[e].theta(c).double().clone())
+  z_all, var_ratio, worst = [], [], []
+  finite = True
+  for k, (mu, cov, _) in enumerate(laps):
+    S = torch.stack(snaps[k])
+    finite = finite and bool(torch.isfinite(S).all().item())
+    sd = torch.sqrt(torch.diagonal(cov))
+    z = (S.mean(0) - mu) / sd
+    z_all.append(z)
+    var_ratio.append(float((S.var(0) / sd ** 2).median().item()))
+    worst.append(float(z.abs().max().item()))
+  z = torch.cat(z_all)
+  n_s = len(snaps[0])
+  return {"reference": "Laplace approximation of each problem's logistic posterior",
+          "iterations": its, "burn_in": burn, "start": "MAP",
+          "samples_per_problem": n_s,
+          "step_size": f"polynomial 1e-3 -> 1e-4 over the {its} iterations",
+          "cold_chain_states_finite": finite,
+          "mean_error_rms_sd": float(torch.sqrt(torch.mean(z * z)).item()),
+          "mean_error_max_sd_per_problem": [round(w, 3) for w in worst],
+          "independent_sample_standard_error_sd": 1.0 / float(np.sqrt(n_s)),
+          "variance_over_laplace_median_per_problem": [round(v, 3) for v in var_ratio]}
 
 
 def run_ladder(args):
@@ -464,7 +515,13 @@ def run_ladder(args):
                                 "glm_loop_kernel<1>, <0>, swap_kernel)",
                       "bytes_per_iteration": bytes_it},
          "exchanges": int(sum(int(ex.swaps.sum().item()) for ex in exchanges)),
-         "tests": int(sum(int(ex.step.sum().item()) for ex in exchanges))}
+         "tests": int(sum(int(ex.step.sum().item()) for ex in exchanges)),
+         "chains_finite_after_timed_run": int(sum(
+             sum(bool(torch.isfinite(e.theta(c)).all().item()) for c in range(e.C))
+             for e in grp.engines))}
+  if not args.no_posterior:
+    log("posterior check (cold rungs against each problem's Laplace approximation)")
+    out["posterior"] = ladder_posterior(torch, fused, R, probs, G)
   if not args.no_cpu:
     log("CPU baseline (oracle C port per chain)")
     temps = [20.0 ** (j / (LADDER_RUNGS - 1)) for j in range(LADDER_RUNGS)]
@@ -1221,6 +1278,8 @@ def main():
   ap.add_argument("--warmup", type=int, default=50)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--no-posterior", action="store_true",
+                  help="resgld-ladder: skip the cold-rung posterior check")
   ap.add_argument("--tiles", default=None,
                   help="MLP workloads: a fixed GEMM plan instead of the tuner, five 'k x s' "
                        "entries for G1..G5 (e.g. 128x2,128x1,32x1,32x1,128x1)")
