@@ -1163,6 +1163,30 @@ def mh_cpu_baseline(kind, X_host, y_host, theta0, keys, dkeys, eps):
                        "over chains)", threads)
 
 
+def mh_posterior(torch, eng, X, y, done, theta0):
+  """Posterior moments of the saved post-test states (every iteration of warm-up and
+  timed region, the first half dropped as burn-in), pooled over the chains, against the
+  Laplace approximation of the logistic posterior, in posterior standard deviations."""
+  mu, cov, last_step = laplace_logistic(torch, X, y)
+  sd = torch.sqrt(torch.diagonal(cov))
+  burn = done // 2
+  S = torch.stack([eng.sample_trace(c)[burn:done] for c in range(eng.C)]).double()
+  cm = S.mean(1)
+  pooled = cm.mean(0)
+  z = (pooled - mu) / sd
+  within = ((S - cm[:, None]) ** 2).mean((0, 1))
+  rms = lambda v: float(torch.sqrt(torch.mean(v * v)).item())
+  z0 = (torch.as_tensor(theta0, dtype=torch.float64, device=mu.device) - mu) / sd
+  return {"reference": "Laplace approximation of the logistic posterior (Newton MAP, inverse "
+                       "Hessian; last Newton step %.1e)" % last_step,
+          "iterations": int(done), "burn_in": int(burn), "chains": int(eng.C),
+          "start_error_rms_sd": rms(z0),
+          "states_finite": bool(torch.isfinite(S).all().item()),
+          "mean_error_rms_sd": rms(z), "mean_error_max_sd": float(z.abs().max().item()),
+          "mc_standard_error_rms_sd": rms(cm.std(0) / np.sqrt(eng.C) / sd),
+          "variance_over_laplace_median": float((within / sd ** 2).median().item())}
+
+
 def run_mh(args, kind):
   """AMAGOLD or SGGMC on the headline's logistic data (N=1e6, D=512, batch 4096), 64
   chains, 5 leapfrog steps (reversible leapfrog, or OBABO with two minibatches per
@@ -1261,6 +1285,9 @@ def run_mh(args, kind):
                          "avg_launch_us": step_us,
                          "achieved_GBs": step_bytes / (step_us / 1e6) / 1e9,
                          "bytes_per_launch": step_bytes}}
+  if not args.no_posterior:
+    log("posterior check (saved post-test states against the Laplace approximation)")
+    out["posterior"] = mh_posterior(torch, eng, X, y, done, theta0)
   if not args.no_cpu:
     log("CPU baseline (oracle C port)")
     out["cpu_baseline"] = mh_cpu_baseline(kind, X.cpu().numpy(), y.cpu().numpy(), theta0, keys,
@@ -1279,7 +1306,8 @@ def main():
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
   ap.add_argument("--no-posterior", action="store_true",
-                  help="resgld-ladder: skip the cold-rung posterior check")
+                  help="resgld-ladder, amagold-logistic, sggmc-logistic: skip the "
+                       "posterior check")
   ap.add_argument("--tiles", default=None,
                   help="MLP workloads: a fixed GEMM plan instead of the tuner, five 'k x s' "
                        "entries for G1..G5 (e.g. 128x2,128x1,32x1,32x1,128x1)")
